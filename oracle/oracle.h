@@ -1,0 +1,101 @@
+/*
+ * oracle.h - CPU restatement of GPUseed's seeding path (seed_gen.cu).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the
+ * CPU baseline ("port") that bench.py times; the product (libgpuseed.so, the
+ * seed_gen command line) never links, loads or calls it.
+ *
+ * Every function cites the seed_gen.cu lines whose behaviour it restates.
+ */
+#ifndef GS_ORACLE_H
+#define GS_ORACLE_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_index or_index;
+
+/* bwt_restore_bwt + bwt_restore_sa (seed_gen.cu:2685-2736).
+ * Returns 0, -4 (missing file), -5 (primary/seq_len mismatch) or -2 (memory). */
+int or_index_load(const char* prefix, or_index** out);
+void or_index_free(or_index* idx);
+uint32_t or_index_seq_len(const or_index* idx);
+uint32_t or_index_primary(const or_index* idx);
+uint32_t or_index_sa_intv(const or_index* idx);
+void or_index_L2(const or_index* idx, uint32_t L2[5]);
+
+/* Rank primitives, exposed for the known-answer tests.
+ * or_occ: bwt_occ_gpu (seed_gen.cu:195-216); row in [0, seq_len] or 0xFFFFFFFF.
+ * or_step: one backward-extension step through find_occ_gpu (329-368) as used
+ *          at 2169-2174: returns (l', u') packed as l' | u' << 32.
+ * or_lf: bwt_inv_psi_gpu (455-484) including its primary+1 behaviour.
+ * or_locate: locate_seeds_gpu (668-684) for one SA row. */
+uint32_t or_occ(const or_index* idx, uint32_t row, uint32_t c);
+uint64_t or_step(const or_index* idx, uint32_t l, uint32_t u, uint32_t c);
+uint32_t or_lf(const or_index* idx, uint32_t row);
+uint32_t or_locate(const or_index* idx, uint32_t row);
+
+/* pack_4bit_fow / pack_4bit_rev code of one input byte (seed_gen.cu:2536-2578). */
+uint32_t or_code_fwd(uint8_t ascii);
+uint32_t or_code_rev(uint8_t ascii);
+
+typedef struct {
+  int k;            /* -k, minimum seed length (seed_gen.cu:2821, 2829)          */
+  int smem;         /* -s (2832)                                                  */
+  int prune;        /* 0: every task extended to its end (the definition);
+                       1: distance-1 pruning (the baseline algorithm)            */
+  uint32_t max_occ; /* 0: locate every row (the reference); else seeds whose
+                       interval is larger get no positions                        */
+  int threads;      /* 0: all online CPUs                                        */
+  int use_table;    /* 1: precompute the 13-mer table like 2887-2895            */
+} or_opts;
+
+typedef struct {
+  uint32_t qbeg, qend; /* bit 31 set on both for the reverse strand (2402)   */
+  uint32_t l, u;       /* SA interval rows                                     */
+  uint32_t n_pos;      /* positions emitted for this seed                      */
+  uint32_t pad;
+} or_seed;
+
+typedef struct {
+  uint64_t n_reads;
+  uint32_t* read_nseeds; /* seeds with at least one position, per read      */
+  uint64_t* read_seed_off;
+  uint32_t* read_npos;
+  uint64_t* read_pos_off;
+  uint64_t n_seeds;
+  or_seed* seeds;
+  uint64_t n_pos;
+  uint32_t* pos;
+  /* work counters (distance-1 pruning, whatever opts.prune says) */
+  uint64_t w_tasks;       /* tasks with a valid 13-base window               */
+  uint64_t w_steps;       /* backward-extension steps                        */
+  uint64_t w_blocks64;    /* distinct 64-base occ blocks read by those steps */
+  uint64_t w_blocks96;    /* distinct 96-base blocks (device layout)        */
+  uint64_t w_lf;          /* LF steps of the SA walk                          */
+  uint64_t w_sa;          /* SA samples read                                  */
+  uint64_t w_steps_unpruned; /* steps with no pruning at all                 */
+  uint64_t n_short;       /* reads shorter than k (no tasks)                  */
+} or_result;
+
+/* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line
+ * bytes, '\r' included, seed_gen.cu:2955-2990).  Reads are numbered in input
+ * order; the MEM look-ahead (624-667) stays inside the reference's batches of
+ * >= 1,000,000 bases (2910-2991), which are recomputed from lens[]. */
+int or_seed_reads(const or_index* idx, const char* bases, const uint64_t* offsets,
+                  const uint32_t* lens, uint64_t n, const or_opts* opts, or_result* res);
+void or_result_free(or_result* res);
+
+/* The -o text output of one run (seed_gen.cu:3625-3648), reads numbered from
+ * first_read_no (1-based). */
+int or_format_text(const or_result* res, int smem, uint64_t first_read_no, FILE* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,158 @@
+"""ctypes view of the oracle (oracle/_build/liboracle.so) for the tests.
+
+The oracle is test infrastructure: the checker for the HIP path and the CPU
+baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use
+it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class OrOpts(C.Structure):
+    _fields_ = [("k", C.c_int), ("smem", C.c_int), ("prune", C.c_int),
+                ("max_occ", C.c_uint32), ("threads", C.c_int), ("use_table", C.c_int)]
+
+
+class OrSeed(C.Structure):
+    _fields_ = [("qbeg", C.c_uint32), ("qend", C.c_uint32), ("l", C.c_uint32),
+                ("u", C.c_uint32), ("n_pos", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class OrResult(C.Structure):
+    _fields_ = [("n_reads", C.c_uint64),
+                ("read_nseeds", C.POINTER(C.c_uint32)), ("read_seed_off", C.POINTER(C.c_uint64)),
+                ("read_npos", C.POINTER(C.c_uint32)), ("read_pos_off", C.POINTER(C.c_uint64)),
+                ("n_seeds", C.c_uint64), ("seeds", C.POINTER(OrSeed)),
+                ("n_pos", C.c_uint64), ("pos", C.POINTER(C.c_uint32)),
+                ("w_tasks", C.c_uint64), ("w_steps", C.c_uint64), ("w_blocks64", C.c_uint64),
+                ("w_blocks96", C.c_uint64), ("w_lf", C.c_uint64), ("w_sa", C.c_uint64),
+                ("w_steps_unpruned", C.c_uint64), ("n_short", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.or_index_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.or_index_free.argtypes = [C.c_void_p]
+        for f in ("or_index_seq_len", "or_index_primary", "or_index_sa_intv"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_uint32
+        L.or_index_L2.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+        L.or_occ.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.or_occ.restype = C.c_uint32
+        L.or_step.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.or_step.restype = C.c_uint64
+        L.or_lf.argtypes = [C.c_void_p, C.c_uint32]
+        L.or_lf.restype = C.c_uint32
+        L.or_locate.argtypes = [C.c_void_p, C.c_uint32]
+        L.or_locate.restype = C.c_uint32
+        L.or_code_fwd.argtypes = [C.c_uint8]
+        L.or_code_fwd.restype = C.c_uint32
+        L.or_code_rev.argtypes = [C.c_uint8]
+        L.or_code_rev.restype = C.c_uint32
+        L.or_seed_reads.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(OrOpts),
+                                    C.POINTER(OrResult)]
+        L.or_result_free.argtypes = [C.POINTER(OrResult)]
+        _lib = L
+    return _lib
+
+
+class Index:
+    def __init__(self, prefix):
+        self.h = C.c_void_p()
+        rc = lib().or_index_load(prefix.encode(), C.byref(self.h))
+        if rc != 0:
+            raise OSError(rc, "or_index_load failed for %s" % prefix)
+
+    def close(self):
+        if self.h:
+            lib().or_index_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def seq_len(self):
+        return lib().or_index_seq_len(self.h)
+
+    @property
+    def primary(self):
+        return lib().or_index_primary(self.h)
+
+    @property
+    def sa_intv(self):
+        return lib().or_index_sa_intv(self.h)
+
+    def L2(self):
+        a = (C.c_uint32 * 5)()
+        lib().or_index_L2(self.h, a)
+        return list(a)
+
+    def occ(self, row, c):
+        return lib().or_occ(self.h, row & 0xFFFFFFFF, c)
+
+    def step(self, l, u, c):
+        v = lib().or_step(self.h, l & 0xFFFFFFFF, u & 0xFFFFFFFF, c)
+        return v & 0xFFFFFFFF, v >> 32
+
+    def lf(self, row):
+        return lib().or_lf(self.h, row)
+
+    def locate(self, row):
+        return lib().or_locate(self.h, row)
+
+
+def pack_reads(reads):
+    """reads: list of bytes -> (buffer, offsets u64, lens u32)"""
+    lens = np.array([len(r) for r in reads], dtype=np.uint32)
+    offs = np.zeros(len(reads), dtype=np.uint64)
+    if len(reads):
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    buf = b"".join(reads)
+    return buf, offs, lens
+
+
+def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0):
+    """Returns (per-read list of (qbeg, qend, l, u, [positions])), counters dict)."""
+    buf, offs, lens = pack_reads(reads)
+    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table)
+    res = OrResult()
+    rc = lib().or_seed_reads(index.h, buf if buf else b"\0",
+                             offs.ctypes.data_as(C.POINTER(C.c_uint64)),
+                             lens.ctypes.data_as(C.POINTER(C.c_uint32)), len(reads),
+                             C.byref(o), C.byref(res))
+    if rc != 0:
+        raise RuntimeError("or_seed_reads failed: %d" % rc)
+    try:
+        out = []
+        for r in range(len(reads)):
+            s0 = res.read_seed_off[r]
+            p = res.read_pos_off[r]
+            lst = []
+            for j in range(res.read_nseeds[r]):
+                sd = res.seeds[s0 + j]
+                pos = [res.pos[p + q] for q in range(sd.n_pos)]
+                p += sd.n_pos
+                lst.append((sd.qbeg, sd.qend, sd.l, sd.u, pos))
+            out.append(lst)
+        counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
+                                                 "w_lf", "w_sa", "w_steps_unpruned", "n_short")}
+        return out, counters
+    finally:
+        lib().or_result_free(C.byref(res))
