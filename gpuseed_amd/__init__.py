@@ -52,6 +52,36 @@ class SynthReadParams(C.Structure):
                 ("both_strands", C.c_int)]
 
 
+class IndexInfo(C.Structure):
+    _fields_ = [("seq_len", C.c_uint32), ("primary", C.c_uint32), ("sa_intv", C.c_uint32),
+                ("L2", C.c_uint32 * 5), ("n_blocks", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("t_read", C.c_double), ("t_upload", C.c_double), ("t_repack", C.c_double),
+                ("t_table", C.c_double)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
+                ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("reserved", C.c_int * 3)]
+
+
+class SeedRec(C.Structure):
+    _fields_ = [("qbeg", C.c_uint32), ("qend", C.c_uint32), ("l", C.c_uint32), ("u", C.c_uint32),
+                ("n_pos", C.c_uint32), ("pad", C.c_uint32)]
+
+
+SEED_DTYPE = np.dtype([("qbeg", "<u4"), ("qend", "<u4"), ("l", "<u4"), ("u", "<u4"),
+                       ("n_pos", "<u4"), ("pad", "<u4")])
+
+
+class Result(C.Structure):
+    _fields_ = [("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
+                ("read_nseeds", C.POINTER(C.c_uint32)), ("read_npos", C.POINTER(C.c_uint32)),
+                ("read_seed_off", C.POINTER(C.c_uint64)), ("read_pos_off", C.POINTER(C.c_uint64)),
+                ("seeds", C.POINTER(SeedRec)), ("pos", C.POINTER(C.c_uint32)),
+                ("t_h2d", C.c_double), ("t_seed", C.c_double), ("t_post", C.c_double),
+                ("t_locate", C.c_double), ("t_d2h", C.c_double)]
+
+
 _lib = None
 
 
@@ -73,6 +103,16 @@ def lib():
         L.gs_synth_preset.argtypes = [C.c_int, C.POINTER(SynthRefParams),
                                       C.POINTER(SynthReadParams)]
         L.gs_build_index_host.argtypes = [u8p, C.c_uint64, C.c_uint32, C.c_char_p]
+        L.gs_index_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.gs_index_close.argtypes = [C.c_void_p]
+        L.gs_index_get_info.argtypes = [C.c_void_p, C.POINTER(IndexInfo)]
+        L.gs_index_read_occ.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
+        L.gs_index_read_kmer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
+        L.gs_seed.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint64, C.POINTER(Opts),
+                              C.POINTER(Result)]
+        L.gs_result_free.argtypes = [C.POINTER(Result)]
+        L.gs_result_free.restype = None
+        L.gs_format_text.argtypes = [C.POINTER(Result), C.c_int, C.c_uint64, C.c_int]
         _lib = L
     return _lib
 
@@ -130,3 +170,106 @@ def roofline_random(buffer_bytes, line_bytes=16, mode=0, inflight=1, grid=8192, 
     _check("gs_roofline_random", lib().gs_roofline_random(buffer_bytes, line_bytes, mode, inflight,
                                                           grid, iters, reps, C.byref(r)))
     return r
+
+
+class Index:
+    """A device-resident index (gs_index_open)."""
+
+    def __init__(self, prefix, device=0):
+        self.h = C.c_void_p()
+        _check("gs_index_open", lib().gs_index_open(prefix.encode(), device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gs_index_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = IndexInfo()
+        _check("gs_index_get_info", lib().gs_index_get_info(self.h, C.byref(i)))
+        return i
+
+    def read_occ(self, first, count):
+        out = np.empty((count, 4), dtype=np.uint32)
+        _check("gs_index_read_occ", lib().gs_index_read_occ(self.h, first, count, _ptr(out, C.c_uint32)))
+        return out
+
+    def read_kmer(self, first, count):
+        out = np.empty((count, 2), dtype=np.uint32)
+        _check("gs_index_read_kmer", lib().gs_index_read_kmer(self.h, first, count, _ptr(out, C.c_uint32)))
+        return out
+
+
+class SeedOutput:
+    """Host copy of a gs_result as numpy arrays."""
+
+    def __init__(self, res):
+        n = res.n_reads
+        self.n_reads = n
+        self.read_nseeds = np.ctypeslib.as_array(res.read_nseeds, (n,)).copy() if n else np.zeros(0, np.uint32)
+        self.read_npos = np.ctypeslib.as_array(res.read_npos, (n,)).copy() if n else np.zeros(0, np.uint32)
+        self.read_seed_off = np.ctypeslib.as_array(res.read_seed_off, (n + 1,)).copy()
+        self.read_pos_off = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
+        ns, npos = res.n_seeds, res.n_pos
+        if ns:
+            raw = np.ctypeslib.as_array(C.cast(res.seeds, C.POINTER(C.c_uint32)), (ns * 6,))
+            self.seeds = raw.copy().view(SEED_DTYPE)
+        else:
+            self.seeds = np.zeros(0, SEED_DTYPE)
+        self.pos = np.ctypeslib.as_array(res.pos, (npos,)).copy() if npos else np.zeros(0, np.uint32)
+        self.times = dict(h2d=res.t_h2d, seed=res.t_seed, post=res.t_post, locate=res.t_locate,
+                          d2h=res.t_d2h)
+
+    def read(self, r):
+        """[(qbeg, qend, l, u, [positions])] of read r"""
+        out = []
+        p = int(self.read_pos_off[r])
+        for s in self.seeds[int(self.read_seed_off[r]):int(self.read_seed_off[r + 1])]:
+            n = int(s["n_pos"])
+            out.append((int(s["qbeg"]), int(s["qend"]), int(s["l"]), int(s["u"]),
+                        [int(v) for v in self.pos[p:p + n]]))
+            p += n
+        return out
+
+    def as_lists(self):
+        return [self.read(r) for r in range(self.n_reads)]
+
+
+def pack_reads(reads):
+    lens = np.array([len(r) for r in reads], dtype=np.uint32)
+    offs = np.zeros(len(reads), dtype=np.uint64)
+    if len(reads):
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    return np.frombuffer(b"".join(reads) or b"\0", dtype=np.uint8), offs, lens
+
+
+def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
+         text_fd=None):
+    """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
+    buf, offs, lens = arrays if arrays is not None else pack_reads(reads)
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel)
+    res = Result()
+    _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
+                                    _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32), len(lens),
+                                    C.byref(o), C.byref(res)))
+    try:
+        if text_fd is not None:
+            _check("gs_format_text", lib().gs_format_text(C.byref(res), 1 if smem else 0, 1, text_fd))
+        return SeedOutput(res)
+    finally:
+        lib().gs_result_free(C.byref(res))

@@ -1,0 +1,298 @@
+// Device-resident FM-index: load, validate, repack into occ96, 13-mer table.
+//
+// Replaces the reference's index path: bwt_restore_bwt / bwt_restore_sa
+// (seed_gen.cu:2685-2736), the cudaMemcpy upload (2864-2876) and
+// pre_calc_seed_intervals_gpu (2648-2675, launched at 2887-2895).
+#include <string.h>
+#include <sys/time.h>
+
+#include "gs_internal.h"
+#include "host/gs_host.h"
+
+using gs::DevIndex;
+
+static double wall() {
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return tv.tv_sec + tv.tv_usec * 1e-6;
+}
+
+uint64_t gs_reciprocal64(uint32_t d) {
+  if (d <= 1) return 0;
+  const unsigned __int128 one = (unsigned __int128)1 << 64;
+  unsigned __int128 q = one / d;
+  if (q * d != one) q += 1;
+  return (uint64_t)q;
+}
+
+DevIndex gs_index::dev() const {
+  DevIndex d;
+  d.occ = occ;
+  d.kmer = kmer;
+  d.sa = sa;
+  d.primary = primary;
+  d.seq_len = seq_len;
+  d.sa_intv = sa_intv;
+  d.pad = 0;
+  d.sa_div = gs_reciprocal64(sa_intv);
+  for (int c = 0; c < 5; ++c) d.L2[c] = L2[c];
+  return d;
+}
+
+namespace {
+
+// symbol at $-removed position p of a reference-layout body (seed_gen.cu:52-58)
+__device__ __forceinline__ uint32_t ref_word(const uint32_t* bwt, uint64_t p) {
+  return bwt[(p >> 6) * 8 + 4 + ((p & 63) >> 4)];
+}
+
+// occurrences of c among the 16 bases of one reference word (pop_count_full, 152-158)
+__device__ __forceinline__ uint32_t count_in_word(uint32_t w, uint32_t c) {
+  uint32_t odd = ((c & 2) ? w : ~w) >> 1;
+  uint32_t even = (c & 1) ? w : ~w;
+  return __popc(odd & even & 0x55555555u);
+}
+
+// Every 64-base block's counts must equal the previous block's counts plus its
+// bases, and the closing counts (the last four words of the body) must equal
+// the last block's counts plus its bases; any mismatch sets *err.  Files
+// written by build_index.sh satisfy this.
+__global__ void check_ref_counts(const uint32_t* __restrict__ bwt, uint32_t n, uint64_t n64,
+                                 uint64_t tail, uint32_t* __restrict__ err) {
+  const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (j > n64) return;
+  if (j == 0) {
+    if (bwt[0] | bwt[1] | bwt[2] | bwt[3]) atomicOr(err, 1u);
+    return;
+  }
+  const uint32_t* prev = bwt + (j - 1) * 8;
+  const uint32_t* cur = j == n64 ? bwt + tail : bwt + j * 8;
+  const uint64_t first = (j - 1) * 64;
+  const uint32_t valid = n - first < 64 ? (uint32_t)(n - first) : 64u;
+  uint32_t s[4] = {prev[0], prev[1], prev[2], prev[3]};
+  for (uint32_t q = 0; q < valid; ++q) {
+    const uint32_t w = prev[4 + (q >> 4)];
+    s[(w >> ((15 - (q & 15)) << 1)) & 3u]++;
+  }
+  for (uint32_t c = 0; c < 4; ++c)
+    if (s[c] != cur[c]) atomicOr(err, 1u);
+}
+
+__global__ void repack_occ96(const uint32_t* __restrict__ bwt, uint32_t n, uint64_t n_blocks,
+                             uint4* __restrict__ occ) {
+  const uint64_t b = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (b >= n_blocks) return;
+  const uint64_t p0 = b * gs::kBlockBases;
+  const uint64_t j = p0 >> 6;
+  uint32_t cnt[4];
+  for (int c = 0; c < 4; ++c) cnt[c] = bwt[j * 8 + c];
+  for (uint64_t p = j << 6; p < p0; p += 16) {
+    const uint32_t w = ref_word(bwt, p);
+    for (uint32_t c = 0; c < 4; ++c) cnt[c] += count_in_word(w, c);
+  }
+  uint32_t bits[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (uint32_t w6 = 0; w6 < 6; ++w6) {
+    const uint64_t p = p0 + 16 * w6;
+    if (p >= n) break;
+    const uint32_t word = ref_word(bwt, p);
+    const uint32_t m = n - p < 16 ? (uint32_t)(n - p) : 16u;
+    for (uint32_t q = 0; q < m; ++q) {
+      const uint32_t c = (word >> ((15 - q) << 1)) & 3u;
+      const uint32_t pos = 16 * w6 + q;
+      bits[c][pos >> 5] |= 1u << (pos & 31);
+    }
+  }
+  for (uint32_t c = 0; c < 4; ++c) occ[b * 4 + c] = make_uint4(cnt[c], bits[c][0], bits[c][1], bits[c][2]);
+}
+
+// One level of the 13-mer table: entries of level i+1 from level i.  Entry p
+// after i characters depends on the low 2i bits of p; character i is applied
+// next; an empty interval is carried unchanged (the break at 2666-2668).
+__global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uint2* __restrict__ next,
+                           uint32_t level, uint64_t count) {
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const uint64_t q = t & ((1ull << (2 * level)) - 1);
+  const uint32_t c = (uint32_t)(t >> (2 * level));
+  uint32_t l = 0, u = X.seq_len;
+  if (level > 0) {
+    const uint2 iv = prev[q];
+    l = iv.x;
+    u = iv.y;
+  }
+  if (l <= u) gs::extend(X, c, l, u);
+  next[t] = make_uint2(l, u);
+}
+
+inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int gs_build_kmer_table(gs_index* X) {
+  const uint64_t full = 1ull << (2 * gs::kPrecalc);
+  uint2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&X->kmer, full * sizeof(uint2)) != hipSuccess) return GS_ENOMEM;
+  X->device_bytes += full * sizeof(uint2);
+  if (hipMalloc(&a, (full / 4) * sizeof(uint2)) != hipSuccess) return GS_ENOMEM;
+  if (hipMalloc(&b, (full / 4) * sizeof(uint2)) != hipSuccess) {
+    (void)hipFree(a);
+    return GS_ENOMEM;
+  }
+  const DevIndex D = X->dev();
+  const uint2* prev = nullptr;
+  for (uint32_t lev = 0; lev < (uint32_t)gs::kPrecalc; ++lev) {
+    const uint64_t cnt = 1ull << (2 * (lev + 1));
+    uint2* next = lev + 1 == (uint32_t)gs::kPrecalc ? X->kmer : ((lev & 1) ? b : a);
+    hipLaunchKernelGGL(kmer_level, dim3(grid_for(cnt, 256)), dim3(256), 0, X->stream, D, prev, next,
+                       lev, cnt);
+    prev = next;
+  }
+  hipError_t e = hipStreamSynchronize(X->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return e == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+static void free_index(gs_index* X) {
+  if (!X) return;
+  (void)hipFree(X->occ);
+  (void)hipFree(X->sa);
+  (void)hipFree(X->kmer);
+  if (X->stream) (void)hipStreamDestroy(X->stream);
+  delete X;
+}
+
+static int pick_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GS_ENODEV;
+  if (device < 0 || device >= n) return GS_ENODEV;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GS_ENODEV;
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GS_ENODEV;
+  if (hipSetDevice(device) != hipSuccess) return GS_ENODEV;
+  return GS_OK;
+}
+
+// Upload a host index in the reference layout and build every device structure.
+static int index_from_host(const gs_host_index* H, int device, gs_index** out) {
+  int rc = pick_device(device);
+  if (rc) return rc;
+  gs_index* X = new gs_index();
+  X->device = device;
+  X->primary = H->primary;
+  X->seq_len = H->seq_len;
+  X->sa_intv = H->sa_intv;
+  for (int c = 0; c < 5; ++c) X->L2[c] = H->L2[c];
+  X->n_sa = H->n_sa;
+  X->n_blocks = ((uint64_t)H->seq_len + gs::kBlockBases - 1) / gs::kBlockBases;
+  uint32_t* d_bwt = nullptr;
+  uint32_t* d_err = nullptr;
+  uint32_t err = 0;
+  const double t0 = wall();
+  if (hipStreamCreateWithFlags(&X->stream, hipStreamNonBlocking) != hipSuccess) {
+    free_index(X);
+    return GS_EHIP;
+  }
+  // the occ entries cover every block plus one spare so that no load can leave the array
+  if (hipMalloc(&X->occ, (X->n_blocks + 1) * 4 * sizeof(uint4)) != hipSuccess ||
+      hipMalloc(&X->sa, X->n_sa * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&d_bwt, H->bwt_words * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&d_err, sizeof(uint32_t)) != hipSuccess) {
+    (void)hipFree(d_bwt);
+    (void)hipFree(d_err);
+    free_index(X);
+    return GS_ENOMEM;
+  }
+  X->device_bytes = (X->n_blocks + 1) * 4 * sizeof(uint4) + X->n_sa * sizeof(uint32_t);
+  rc = GS_OK;
+  if (hipMemcpyAsync(d_bwt, H->bwt, H->bwt_words * 4, hipMemcpyHostToDevice, X->stream) != hipSuccess ||
+      hipMemcpyAsync(X->sa, H->sa, X->n_sa * 4, hipMemcpyHostToDevice, X->stream) != hipSuccess ||
+      hipMemsetAsync(d_err, 0, 4, X->stream) != hipSuccess ||
+      hipMemsetAsync(X->occ + X->n_blocks * 4, 0, 4 * sizeof(uint4), X->stream) != hipSuccess ||
+      hipStreamSynchronize(X->stream) != hipSuccess)
+    rc = GS_EHIP;
+  const double t1 = wall();
+  if (!rc) {
+    const uint64_t n64 = ((uint64_t)H->seq_len + 63) / 64;
+    hipLaunchKernelGGL(check_ref_counts, dim3(grid_for(n64 + 1, 256)), dim3(256), 0, X->stream, d_bwt,
+                       H->seq_len, n64, gs_bwt_words(H->seq_len) - 4, d_err);
+    hipLaunchKernelGGL(repack_occ96, dim3(grid_for(X->n_blocks, 256)), dim3(256), 0, X->stream, d_bwt,
+                       H->seq_len, X->n_blocks, X->occ);
+    if (hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, X->stream) != hipSuccess ||
+        hipStreamSynchronize(X->stream) != hipSuccess)
+      rc = GS_EHIP;
+  }
+  (void)hipFree(d_bwt);
+  (void)hipFree(d_err);
+  const double t2 = wall();
+  if (!rc && err) rc = GS_EFORMAT;
+  if (!rc) rc = gs_build_kmer_table(X);
+  const double t3 = wall();
+  if (rc) {
+    free_index(X);
+    return rc;
+  }
+  X->t_upload = t1 - t0;
+  X->t_repack = t2 - t1;
+  X->t_table = t3 - t2;
+  *out = X;
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_index_open(const char* prefix, int device, gs_index** out) {
+  if (!prefix || !out) return GS_EINVAL;
+  *out = nullptr;
+  const double t0 = wall();
+  gs_host_index H;
+  int rc = gs_host_index_load(prefix, &H);
+  if (rc) return rc;
+  // the reference's counts must agree with L2 (the totals close the body)
+  const uint64_t tail = H.bwt_words >= 4 ? gs_bwt_words(H.seq_len) - 4 : 0;
+  for (int c = 0; c < 4 && !rc; ++c)
+    if (H.bwt[tail + c] != H.L2[c + 1] - H.L2[c]) rc = GS_EFORMAT;
+  for (int c = 0; c < 4 && !rc; ++c)
+    if (H.L2[c + 1] < H.L2[c]) rc = GS_EFORMAT;
+  const double t1 = wall();
+  if (!rc) rc = index_from_host(&H, device, out);
+  if (!rc) (*out)->t_read = t1 - t0;
+  gs_host_index_free(&H);
+  return rc;
+}
+
+extern "C" GS_API int gs_index_close(gs_index* X) {
+  if (!X) return GS_EINVAL;
+  free_index(X);
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) {
+  if (!X || !info) return GS_EINVAL;
+  memset(info, 0, sizeof(*info));
+  info->seq_len = X->seq_len;
+  info->primary = X->primary;
+  info->sa_intv = X->sa_intv;
+  for (int c = 0; c < 5; ++c) info->L2[c] = X->L2[c];
+  info->device_bytes = X->device_bytes;
+  info->n_blocks = X->n_blocks;
+  info->t_read = X->t_read;
+  info->t_upload = X->t_upload;
+  info->t_repack = X->t_repack;
+  info->t_table = X->t_table;
+  return GS_OK;
+}
+
+// Test access to the device structures: copies occ entries [first, first+count)
+// of block-major order and table entries to host buffers.
+extern "C" GS_API int gs_index_read_occ(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out4) {
+  if (!X || !out4 || first + count > (X->n_blocks + 1) * 4) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  GS_HIP(hipMemcpy(out4, X->occ + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_index_read_kmer(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out2) {
+  if (!X || !out2 || first + count > (1ull << (2 * gs::kPrecalc))) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  GS_HIP(hipMemcpy(out2, X->kmer + first, count * sizeof(uint2), hipMemcpyDeviceToHost));
+  return GS_OK;
+}

@@ -1,0 +1,797 @@
+// Seeding kernels: backward extension, SMEM/MEM finalisation, SA walk.
+//
+// Reference path replaced (seed_gen.cu): find_seed_intervals_gpu (2087-2413),
+// DeviceSelect/DeviceScan/DeviceReduce and sum_arrays (3383-3430),
+// filter/count kernels and the device-side segmented sort (865-1251),
+// seeds_to_threads(_mem) and locate_seeds_gpu (555-724).
+//
+// Design (one read = two extension streams, forward and reverse):
+//  * A lane owns one stream.  Streams are written in a common "virtual" form:
+//    the forward stream reads the forward codes left-to-right, the reverse
+//    stream reads the reverse-complement codes of the reversed read, and both
+//    run the reference's forward-task procedure (2132-2264) on task ends
+//    e = L, L-1, ..., k.  A virtual candidate [b, e) of the reverse stream is
+//    the reference's reverse seed (L-e, L-b) (2272-2404).
+//  * Tasks of a stream run one after another in the same lane, so pruning is
+//    exact distance-1 pruning: task e stops at left end x as soon as its
+//    interval size equals the size the previous tasks recorded for x (then
+//    every occurrence of read[x,e) continues with read[e], so the task's seed
+//    has the same left end as a longer one).  The reference approximates this
+//    with 30-lane shuffles (2192-2212); its output does not depend on which
+//    candidates are pruned, and neither does this one.
+//  * Every loop iteration issues at most one memory operation per lane: a
+//    13-mer table read or one occ entry (two when l-1 and u fall in different
+//    96-base blocks).  The loop stays converged on the loads; all pointer
+//    chasing between loads is LDS-only work on the staged read bytes and the
+//    per-position size record.
+//  * SMEM filtering happens in the lane: tasks finish in decreasing end order
+//    and left ends never increase, so a task is a survivor exactly when its
+//    left end is smaller than the previous finished task's (991-1018).
+#include <fcntl.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include <vector>
+
+#include "gs_internal.h"
+
+using gs::DevIndex;
+
+namespace {
+
+struct SeedParams {
+  DevIndex X;
+  const uint8_t* bases;     // batch bytes, padded by 16
+  const uint64_t* offs;     // per read
+  const uint32_t* lens;     // per read
+  const uint64_t* task_off; // per read: sum of tasks of earlier reads
+  uint32_t n_reads, k, smem, pad;
+  uint4* cand;              // per read: 2*task_off .. : forward region then reverse region
+  uint32_t* nF;
+  uint32_t* nR;
+  void* scratch;            // size records of reads longer than the LDS slots
+};
+
+template <typename AT>
+struct SatMax {
+  static constexpr uint32_t value = (uint32_t)(AT)(~(AT)0);
+};
+
+constexpr int kSeedBlock = 128;
+
+template <typename AT, int LMAX>
+struct Slot {
+  static constexpr int bytes = ((LMAX + 2 * LMAX * (int)sizeof(AT)) + 15) & ~15;
+};
+
+template <typename AT, int LMAX>
+__global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
+  constexpr int PAIR = Slot<AT, LMAX>::bytes;
+  constexpr uint32_t SAT = SatMax<AT>::value;
+  __shared__ __align__(16) uint8_t lds[(kSeedBlock / 2) * PAIR];
+  const uint32_t gid = blockIdx.x * kSeedBlock + threadIdx.x;
+  const uint32_t r = gid >> 1;
+  const uint32_t strand = gid & 1;
+  uint8_t* slot = lds + (threadIdx.x >> 1) * PAIR;
+  const bool live = r < P.n_reads;
+  const uint32_t L = live ? P.lens[r] : 0;
+  const uint64_t off = live ? P.offs[r] : 0;
+  const bool in_lds = L <= (uint32_t)LMAX;
+
+  // stage the read bytes: the two lanes of a pair take alternate aligned 16-byte chunks
+  if (live && in_lds && L) {
+    const uint64_t c0 = off >> 4, c1 = (off + L - 1) >> 4;
+    for (uint64_t ch = c0 + strand; ch <= c1; ch += 2) {
+      const uint4 v = reinterpret_cast<const uint4*>(P.bases)[ch];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+      for (int q = 0; q < 16; ++q) {
+        const uint64_t pos = ch * 16 + q;
+        if (pos >= off && pos < off + L) slot[pos - off] = (uint8_t)(w[q >> 2] >> ((q & 3) * 8));
+      }
+    }
+  }
+  __syncthreads();
+
+  const uint8_t* src = in_lds ? slot : P.bases + off;
+  AT* A = in_lds ? reinterpret_cast<AT*>(slot + LMAX) + strand * LMAX
+                 : reinterpret_cast<AT*>(P.scratch) + 2 * off + strand * L;
+  auto V = [&](int32_t i) -> uint32_t {
+    return strand == 0 ? gs::code_fwd(src[i]) : gs::code_rev(src[L - 1 - i]);
+  };
+
+  const uint32_t k = P.k;
+  const uint32_t ntask = L >= k ? L - k + 1 : 0;
+  bool done = !live || L < k;
+  for (uint32_t q = 0; !done && q < L; ++q) A[q] = 0;
+
+  int32_t e = (int32_t)L;
+  uint32_t p = 0, nbad = 0;
+  if (!done)
+    for (int j = 0; j < gs::kPrecalc; ++j) {
+      const uint32_t c = V((int32_t)L - 1 - j);
+      nbad += c > 3;
+      p |= (c & 3u) << (2 * j);
+    }
+  int phase = 0;  // 0: task e waits for its 13-mer lookup; 1: task e is extending
+  int32_t x = 0;
+  uint32_t l = 0, u = 0, b = 0, s = 0, last_b = 0xFFFFFFFFu, nout = 0;
+  uint4* outF = P.cand + (live ? 2 * P.task_off[r] : 0);
+
+  auto next_task = [&]() -> bool {
+    if ((uint32_t)(e - 1) < k) return false;
+    const uint32_t cn = V(e - 14), co = V(e - 1);
+    nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
+    p = (p >> 2) | ((cn & 3u) << 24);
+    --e;
+    return true;
+  };
+  // size record at x: prune if the previous tasks recorded the same size there
+  auto prune_at = [&](int32_t pos, uint32_t size) -> bool {
+    const uint32_t sv = size >= SAT ? SAT : size;
+    if (size < SAT && (uint32_t)A[pos] == sv) return true;
+    A[pos] = (AT)sv;
+    return false;
+  };
+  auto finish = [&]() {
+    if ((uint32_t)e - b >= k && (!P.smem || b < last_b)) {
+      if (strand == 0) {
+        outF[ntask - 1 - nout] = make_uint4(b, (uint32_t)e, l, u);
+      } else {
+        outF[ntask + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
+      }
+      ++nout;
+    }
+    last_b = b;
+  };
+
+  while (true) {
+    int op = 0;
+    uint32_t c = 0;
+    while (!done) {
+      if (phase == 0) {
+        if (nbad == 0) {
+          op = 1;
+          break;
+        }
+        if (!next_task()) done = true;
+        continue;
+      }
+      if (x < 0) {
+        finish();
+        phase = 0;
+        if (!next_task()) done = true;
+        continue;
+      }
+      c = V(x);
+      if (c <= 3) {
+        op = 2;
+        break;
+      }
+      b = (uint32_t)x;  // code > 3 after the window: interval kept, position consumed
+      if (prune_at(x, s)) {
+        phase = 0;
+        if (!next_task()) done = true;
+        continue;
+      }
+      --x;
+    }
+    if (__ballot(op != 0) == 0) break;
+
+    // one memory operation per lane
+    const uint4* a1 = nullptr;
+    const uint4* a2 = nullptr;
+    uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
+    bool have1 = false;
+    if (op == 1) {
+      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);
+    } else if (op == 2) {
+      const uint32_t r1 = l - 1u;
+      have1 = r1 != 0xFFFFFFFFu;
+      i2 = gs::row_to_idx(u, P.X.primary);
+      b2 = gs::div96(i2);
+      i1 = have1 ? gs::row_to_idx(r1, P.X.primary) : 0u;
+      b1 = gs::div96(i1);
+      a1 = P.X.occ + (size_t)b2 * 4 + c;
+      if (have1 && b1 != b2) a2 = P.X.occ + (size_t)b1 * 4 + c;
+    }
+    uint4 d1 = make_uint4(0, 0, 0, 0), d2;
+    if (a1) d1 = *a1;
+    d2 = d1;
+    if (a2) d2 = *a2;
+
+    if (op == 1) {
+      const uint32_t il = (p & 1) ? d1.z : d1.x, iu = (p & 1) ? d1.w : d1.y;
+      if (il > iu) {
+        if (!next_task()) done = true;
+      } else {
+        l = il;
+        u = iu;
+        s = u - l + 1;
+        b = (uint32_t)e - gs::kPrecalc;
+        if (prune_at((int32_t)b, s)) {
+          if (!next_task()) done = true;
+        } else {
+          x = e - gs::kPrecalc - 1;
+          phase = 1;
+        }
+      }
+    } else if (op == 2) {
+      const uint32_t o2 = gs::rank_in_entry(d1, i2 - b2 * gs::kBlockBases);
+      const uint32_t o1 = have1 ? gs::rank_in_entry(d2, i1 - b1 * gs::kBlockBases) : 0u;
+      const uint32_t base = P.X.L2[c];
+      const uint32_t l2 = base + o1 + 1u, u2 = base + o2;
+      if (l2 > u2) {
+        finish();
+        phase = 0;
+        if (!next_task()) done = true;
+      } else {
+        l = l2;
+        u = u2;
+        s = u - l + 1;
+        b = (uint32_t)x;
+        if (prune_at(x, s)) {
+          phase = 0;
+          if (!next_task()) done = true;
+        } else {
+          --x;
+        }
+      }
+    }
+  }
+  if (live) (strand == 0 ? P.nF : P.nR)[r] = nout;
+}
+
+// ---- finalisation ---------------------------------------------------------
+
+struct PostParams {
+  const uint4* cand;
+  const uint32_t* nF;
+  const uint32_t* nR;
+  const uint64_t* task_off;
+  const uint32_t* lens;
+  const uint32_t* batch;  // reference batch of each read
+  uint32_t n_reads, k, smem, max_occ;
+  uint32_t* nseeds;       // count pass
+  uint32_t* npos;
+  const uint64_t* seed_off;  // write pass
+  const uint64_t* pos_off;
+  gs_seed_rec* seeds;
+  uint32_t* rows;
+};
+
+struct ReadLists {
+  const uint4* F;  // increasing qend
+  const uint4* R;  // increasing qbeg
+  uint32_t nF, nR;
+};
+
+__device__ __forceinline__ ReadLists lists_of(const PostParams& P, uint32_t r) {
+  ReadLists Ls;
+  const uint32_t L = P.lens[r];
+  const uint32_t ntask = L >= P.k ? L - P.k + 1 : 0;
+  const uint4* base = P.cand + 2 * P.task_off[r];
+  Ls.nF = P.nF[r];
+  Ls.nR = P.nR[r];
+  Ls.F = base + ntask - Ls.nF;
+  Ls.R = base + ntask;
+  return Ls;
+}
+
+__device__ __forceinline__ uint32_t isize(uint4 c) { return c.w - c.z + 1u; }
+
+// MEM filter (filter_seed_intervals_gpu_mem, 1103-1139): an entry is dropped
+// when its neighbour towards the longer seed has the same qbeg/qend and size.
+__device__ __forceinline__ bool mem_killed(const ReadLists& Ls, int rev, uint32_t i) {
+  if (!rev) {
+    const uint4 a = Ls.F[i];
+    if (i + 1 >= Ls.nF) return false;
+    const uint4 n = Ls.F[i + 1];
+    return n.x == a.x && isize(n) == isize(a);
+  }
+  if (i == 0) return false;
+  const uint4 a = Ls.R[i], pv = Ls.R[i - 1];
+  return pv.y == a.y && isize(pv) == isize(a);
+}
+
+// rows an entry contributes (count_seed_intervals_gpu_mem, 927-989; in-read)
+__device__ uint32_t mem_count(const ReadLists& Ls, int rev, uint32_t i) {
+  if (mem_killed(Ls, rev, i)) return 0;
+  const uint4 a = rev ? Ls.R[i] : Ls.F[i];
+  const uint32_t sz = isize(a);
+  if (!rev) {
+    for (uint32_t j = i + 1; j < Ls.nF && Ls.F[j].x == a.x; ++j)
+      if (!mem_killed(Ls, 0, j)) return sz - isize(Ls.F[j]);
+  } else {
+    for (uint32_t j = i; j-- > 0 && Ls.R[j].y == a.y;)
+      if (!mem_killed(Ls, 1, j)) return sz - isize(Ls.R[j]);
+  }
+  return sz;
+}
+
+// interval whose rows an entry skips (seeds_to_threads_mem, 624-667): the next
+// non-empty entry with the same qbeg (forward, scanning on) or qend (reverse,
+// scanning back) anywhere in the same reference batch; (1, 0) if none.
+__device__ uint2 mem_skip(const PostParams& P, uint32_t r, int rev, uint32_t i) {
+  const ReadLists L0 = lists_of(P, r);
+  const uint4 a = rev ? L0.R[i] : L0.F[i];
+  uint2 skip = make_uint2(1, 0);
+  const uint32_t bid = P.batch[r];
+  if (!rev) {
+    uint32_t rr = r;
+    int part = 0;  // 0: F, 1: R
+    uint32_t j = i + 1;
+    while (rr < P.n_reads && P.batch[rr] == bid) {
+      const ReadLists Ls = lists_of(P, rr);
+      const uint32_t n = part == 0 ? Ls.nF : Ls.nR;
+      if (j < n) {
+        const uint4 en = part == 0 ? Ls.F[j] : Ls.R[j];
+        if (en.x != a.x) break;
+        if (!mem_killed(Ls, part, j)) {
+          skip = make_uint2(en.z, en.w);
+          break;
+        }
+        ++j;
+        continue;
+      }
+      j = 0;
+      if (part == 0) {
+        part = 1;
+      } else {
+        part = 0;
+        ++rr;
+      }
+    }
+  } else {
+    int64_t rr = r;
+    int part = 1;
+    int64_t j = (int64_t)i - 1;
+    while (rr >= 0 && P.batch[rr] == bid) {
+      const ReadLists Ls = lists_of(P, (uint32_t)rr);
+      if (j >= 0) {
+        const uint4 en = part == 0 ? Ls.F[j] : Ls.R[j];
+        if (en.y != a.y) break;
+        if (!mem_killed(Ls, part, (uint32_t)j)) {
+          skip = make_uint2(en.z, en.w);
+          break;
+        }
+        --j;
+        continue;
+      }
+      if (part == 1) {
+        part = 0;
+        j = (int64_t)Ls.nF - 1;
+      } else {
+        --rr;
+        if (rr < 0 || P.batch[rr] != bid) break;
+        part = 1;
+        j = (int64_t)lists_of(P, (uint32_t)rr).nR - 1;
+      }
+    }
+  }
+  return skip;
+}
+
+// One lane per read.  Count pass (write == false): seeds and positions per
+// read.  Write pass: seed records and suffix-array rows at the scanned offsets,
+// merged by qbeg with forward entries first on ties (the stable segmented sort
+// at 1233 and 3570).
+template <bool WRITE>
+__global__ void __launch_bounds__(256) finalize_reads(PostParams P) {
+  const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= P.n_reads) return;
+  const ReadLists Ls = lists_of(P, r);
+  uint32_t ns = 0, np = 0;
+  uint64_t so = WRITE ? P.seed_off[r] : 0, po = WRITE ? P.pos_off[r] : 0;
+  uint32_t a = 0, z = 0;
+  while (a < Ls.nF || z < Ls.nR) {
+    const bool takeF = z >= Ls.nR || (a < Ls.nF && Ls.F[a].x <= (Ls.R[z].x & 0x7FFFFFFFu));
+    const int rev = takeF ? 0 : 1;
+    const uint32_t i = takeF ? a++ : z++;
+    const uint4 en = takeF ? Ls.F[i] : Ls.R[i];
+    const uint32_t sz = isize(en);
+    uint32_t cnt = P.smem ? sz : mem_count(Ls, rev, i);
+    if (cnt == 0) continue;
+    if (P.max_occ && sz > P.max_occ) cnt = 0;
+    if (WRITE) {
+      gs_seed_rec rec;
+      rec.qbeg = en.x;
+      rec.qend = en.y;
+      rec.l = en.z;
+      rec.u = en.w;
+      rec.n_pos = cnt;
+      rec.pad = 0;
+      P.seeds[so + ns] = rec;
+      if (cnt) {
+        uint2 skip = make_uint2(1, 0);
+        if (!P.smem) skip = mem_skip(P, r, rev, i);
+        uint32_t row = en.z, got = 0;
+        while (got < cnt) {
+          if (row < skip.x || row > skip.y) P.rows[po + np + got++] = row;
+          ++row;
+        }
+      }
+    }
+    ++ns;
+    np += cnt;
+  }
+  if (!WRITE) {
+    P.nseeds[r] = ns;
+    P.npos[r] = np;
+  }
+}
+
+// ---- exclusive scan of per-read counts into 64-bit offsets ---------------
+// Three launches: per-workgroup totals, a single-workgroup scan of those
+// totals, then per-workgroup local scans offset by the scanned totals.  Waves
+// scan with shuffles; the workgroup combines its four wave totals in LDS.
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;  // per thread
+constexpr uint32_t kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of v across the workgroup; returns the workgroup total in *tot
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* tot) {
+  __shared__ uint64_t wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    if (w < wid) before += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  *tot = all;
+  return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_tile_totals(const uint32_t* __restrict__ in, uint64_t n,
+                                                               uint64_t* __restrict__ tiles) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint64_t s = 0;
+  for (int q = 0; q < kScanItems; ++q)
+    if (base + q < n) s += in[base + q];
+  uint64_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_tiles(uint64_t* __restrict__ tiles, uint64_t ntiles,
+                                                         uint64_t* __restrict__ total) {
+  uint64_t carry = 0;
+  for (uint64_t base = 0; base < ntiles; base += kScanBlock) {
+    const uint64_t i = base + threadIdx.x;
+    const uint64_t v = i < ntiles ? tiles[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan(v, &tot);
+    if (i < ntiles) tiles[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_apply(const uint32_t* __restrict__ in, uint64_t n,
+                                                         const uint64_t* __restrict__ tiles,
+                                                         uint64_t* __restrict__ out) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint64_t s = 0;
+  for (int q = 0; q < kScanItems; ++q) {
+    v[q] = base + q < n ? in[base + q] : 0u;
+    s += v[q];
+  }
+  uint64_t tot;
+  uint64_t run = tiles[blockIdx.x] + block_excl_scan(s, &tot);
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < n) out[base + q] = run;
+    run += v[q];
+  }
+  if (base + kScanItems >= n && base < n) out[n] = run;  // the thread holding the last item
+}
+
+// ---- suffix-array walk (locate_seeds_gpu, 668-684) -------------------------
+__global__ void __launch_bounds__(256) locate_rows(const DevIndex X, uint32_t* __restrict__ rows,
+                                                   uint64_t n) {
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (t >= n) return;
+  uint32_t row = rows[t], itr = 0;
+  // Rows past the last one come only from the reference's batch-wide MEM row
+  // skip (seed_gen.cu:637/645) and have no position; a walk on a consistent
+  // index reaches a sampled row within seq_len + 1 steps.
+  if (row > X.seq_len) {
+    rows[t] = 0xFFFFFFFFu;
+    return;
+  }
+  while (true) {
+    const uint32_t q = gs::div_sa(X, row);
+    if (row == q * X.sa_intv) {
+      rows[t] = X.sa[q] + itr;
+      break;
+    }
+    if (itr > X.seq_len) {
+      rows[t] = 0xFFFFFFFFu;
+      break;
+    }
+    ++itr;
+    row = gs::lf_walk(X, row);
+  }
+}
+
+inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---- device buffers of one gs_seed call -------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return GS_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&p, want) != hipSuccess) return GS_ENOMEM;
+    cap = want;
+    return GS_OK;
+  }
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int scan_counts(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* tiles, uint64_t* total,
+                hipStream_t st) {
+  const uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+  if (ntiles == 0) return GS_OK;
+  hipLaunchKernelGGL(scan_tile_totals, dim3((unsigned)ntiles), dim3(kScanBlock), 0, st, in, n, tiles);
+  hipLaunchKernelGGL(scan_tiles, dim3(1), dim3(kScanBlock), 0, st, tiles, ntiles, total);
+  hipLaunchKernelGGL(scan_apply, dim3((unsigned)ntiles), dim3(kScanBlock), 0, st, in, n, tiles, out);
+  return GS_OK;
+}
+
+template <typename AT, int LMAX>
+void launch_seed(const SeedParams& P, hipStream_t st) {
+  const uint64_t lanes = 2ull * P.n_reads;
+  hipLaunchKernelGGL((seed_streams<AT, LMAX>), dim3(grid_for(lanes, kSeedBlock)), dim3(kSeedBlock), 0,
+                     st, P);
+}
+
+}  // namespace
+
+// The batch of one gs_seed call, with device memory owned by the call.
+extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* offsets,
+                              const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
+                              gs_result* out) {
+  if (!X || !opts || !out || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
+  memset(out, 0, sizeof(*out));
+  if (opts->k < gs::kPrecalc + 1 || opts->k > 1 << 20) return GS_EINVAL;
+  if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  const uint32_t k = (uint32_t)opts->k;
+  const uint32_t batch_bases = opts->batch_bases ? opts->batch_bases : 1000000u;
+
+  // host-side layout: byte span, task offsets, reference batches
+  std::vector<uint64_t> task_off(n_reads + 1);
+  std::vector<uint32_t> batch(n_reads ? n_reads : 1);
+  uint64_t span = 0, tasks = 0, acc = 0;
+  uint32_t bid = 0, maxlen = 0;
+  for (uint64_t i = 0; i < n_reads; ++i) {
+    const uint64_t end = offsets[i] + lens[i];
+    if (end > span) span = end;
+    task_off[i] = tasks;
+    tasks += lens[i] >= k ? lens[i] - k + 1 : 0;
+    batch[i] = bid;
+    acc += lens[i];
+    if (acc >= batch_bases) {  // the batch closes after the read that reaches the limit (2955)
+      ++bid;
+      acc = 0;
+    }
+    if (lens[i] > maxlen) maxlen = lens[i];
+  }
+  task_off[n_reads] = tasks;
+
+  DevBuf d_bases, d_offs, d_lens, d_toff, d_batch, d_cand, d_nF, d_nR, d_ns, d_np, d_soff, d_poff,
+      d_tiles, d_tot, d_seeds, d_rows, d_scratch;
+  const uint64_t nr1 = n_reads + 1;
+  if (d_bases.ensure(span + 32) || d_offs.ensure(8 * nr1) || d_lens.ensure(4 * nr1) ||
+      d_toff.ensure(8 * nr1) || d_batch.ensure(4 * nr1) || d_cand.ensure(16 * (2 * tasks + 1)) ||
+      d_nF.ensure(4 * nr1) || d_nR.ensure(4 * nr1) || d_ns.ensure(4 * nr1) || d_np.ensure(4 * nr1) ||
+      d_soff.ensure(8 * nr1) || d_poff.ensure(8 * nr1) ||
+      d_tiles.ensure(8 * (nr1 / kScanTile + 2)) || d_tot.ensure(16))
+    return GS_ENOMEM;
+  const bool long_reads = maxlen > 256;
+  if (long_reads && d_scratch.ensure(4 * (span + 16))) return GS_ENOMEM;
+
+  hipStream_t st = X->stream;
+  hipEvent_t ev[6];
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return GS_EHIP;
+  auto cleanup_events = [&]() {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  };
+  int rc = GS_OK;
+  do {
+    if (hipEventRecord(ev[0], st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (span && hipMemcpyAsync(d_bases.p, bases, span, hipMemcpyHostToDevice, st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (hipMemsetAsync(d_bases.as<uint8_t>() + span, 0, 32, st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (n_reads &&
+        (hipMemcpyAsync(d_offs.p, offsets, 8 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(d_lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(d_toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(d_batch.p, batch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess)) {
+      rc = GS_EHIP;
+      break;
+    }
+    if (hipEventRecord(ev[1], st) != hipSuccess) { rc = GS_EHIP; break; }
+
+    SeedParams S;
+    S.X = X->dev();
+    S.bases = d_bases.as<uint8_t>();
+    S.offs = d_offs.as<uint64_t>();
+    S.lens = d_lens.as<uint32_t>();
+    S.task_off = d_toff.as<uint64_t>();
+    S.n_reads = (uint32_t)n_reads;
+    S.k = k;
+    S.smem = opts->smem ? 1 : 0;
+    S.pad = 0;
+    S.cand = d_cand.as<uint4>();
+    S.nF = d_nF.as<uint32_t>();
+    S.nR = d_nR.as<uint32_t>();
+    S.scratch = d_scratch.p;
+    if (n_reads) {
+      if (maxlen <= 160)
+        launch_seed<uint16_t, 160>(S, st);
+      else
+        launch_seed<uint16_t, 256>(S, st);
+    }
+    if (hipEventRecord(ev[2], st) != hipSuccess) { rc = GS_EHIP; break; }
+
+    PostParams Q;
+    Q.cand = S.cand;
+    Q.nF = S.nF;
+    Q.nR = S.nR;
+    Q.task_off = S.task_off;
+    Q.lens = S.lens;
+    Q.batch = d_batch.as<uint32_t>();
+    Q.n_reads = (uint32_t)n_reads;
+    Q.k = k;
+    Q.smem = S.smem;
+    Q.max_occ = opts->max_occ;
+    Q.nseeds = d_ns.as<uint32_t>();
+    Q.npos = d_np.as<uint32_t>();
+    Q.seed_off = d_soff.as<uint64_t>();
+    Q.pos_off = d_poff.as<uint64_t>();
+    if (n_reads) {
+      hipLaunchKernelGGL(finalize_reads<false>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
+      scan_counts(Q.nseeds, n_reads, d_soff.as<uint64_t>(), d_tiles.as<uint64_t>(), d_tot.as<uint64_t>(), st);
+      scan_counts(Q.npos, n_reads, d_poff.as<uint64_t>(), d_tiles.as<uint64_t>(), d_tot.as<uint64_t>() + 1, st);
+    }
+    uint64_t tot[2] = {0, 0};
+    if (n_reads && (hipMemcpyAsync(tot, d_tot.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess)) {
+      rc = GS_EHIP;
+      break;
+    }
+    out->n_reads = n_reads;
+    out->n_seeds = tot[0];
+    out->n_pos = tot[1];
+    if (d_seeds.ensure(sizeof(gs_seed_rec) * (tot[0] + 1)) || d_rows.ensure(4 * (tot[1] + 1))) {
+      rc = GS_ENOMEM;
+      break;
+    }
+    Q.seeds = d_seeds.as<gs_seed_rec>();
+    Q.rows = d_rows.as<uint32_t>();
+    if (n_reads) hipLaunchKernelGGL(finalize_reads<true>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
+    if (hipEventRecord(ev[3], st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (tot[1]) hipLaunchKernelGGL(locate_rows, dim3(grid_for(tot[1], 256)), dim3(256), 0, st, S.X, Q.rows, tot[1]);
+    if (hipEventRecord(ev[4], st) != hipSuccess) { rc = GS_EHIP; break; }
+
+    out->read_nseeds = (uint32_t*)malloc(4 * nr1);
+    out->read_npos = (uint32_t*)malloc(4 * nr1);
+    out->read_seed_off = (uint64_t*)malloc(8 * nr1);
+    out->read_pos_off = (uint64_t*)malloc(8 * nr1);
+    out->seeds = (gs_seed_rec*)malloc(sizeof(gs_seed_rec) * (tot[0] + 1));
+    out->pos = (uint32_t*)malloc(4 * (tot[1] + 1));
+    if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
+        !out->pos) {
+      rc = GS_ENOMEM;
+      break;
+    }
+    if (n_reads &&
+        (hipMemcpyAsync(out->read_nseeds, Q.nseeds, 4 * n_reads, hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(out->read_npos, Q.npos, 4 * n_reads, hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(out->read_seed_off, d_soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+         hipMemcpyAsync(out->read_pos_off, d_poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess)) {
+      rc = GS_EHIP;
+      break;
+    }
+    if (tot[0] && hipMemcpyAsync(out->seeds, Q.seeds, sizeof(gs_seed_rec) * tot[0], hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (tot[1] && hipMemcpyAsync(out->pos, Q.rows, 4 * tot[1], hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (hipEventRecord(ev[5], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (!n_reads) {
+      out->read_seed_off[0] = 0;
+      out->read_pos_off[0] = 0;
+    }
+    float ms[5];
+    for (int q = 0; q < 5; ++q) (void)hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
+    out->t_h2d = ms[0] * 1e-3;
+    out->t_seed = ms[1] * 1e-3;
+    out->t_post = ms[2] * 1e-3;
+    out->t_locate = ms[3] * 1e-3;
+    out->t_d2h = ms[4] * 1e-3;
+  } while (0);
+  cleanup_events();
+  if (rc) gs_result_free(out);
+  return rc;
+}
+
+extern "C" GS_API void gs_result_free(gs_result* r) {
+  if (!r) return;
+  free(r->read_nseeds);
+  free(r->read_npos);
+  free(r->read_seed_off);
+  free(r->read_pos_off);
+  free(r->seeds);
+  free(r->pos);
+  memset(r, 0, sizeof(*r));
+}
+
+// -o output (seed_gen.cu:3625-3648), buffered.
+extern "C" GS_API int gs_format_text(const gs_result* res, int smem, uint64_t first_read_no, int fd) {
+  if (!res || fd < 0) return GS_EINVAL;
+  std::vector<char> buf;
+  buf.reserve(1 << 20);
+  char tmp[256];
+  auto flush = [&]() -> bool {
+    size_t off = 0;
+    while (off < buf.size()) {
+      ssize_t w = write(fd, buf.data() + off, buf.size() - off);
+      if (w <= 0) return false;
+      off += (size_t)w;
+    }
+    buf.clear();
+    return true;
+  };
+  static const char sign[2] = {'+', '-'};
+  for (uint64_t r = 0; r < res->n_reads; ++r) {
+    int n = snprintf(tmp, sizeof tmp,
+                     "/*===================================%s seeds in read no. %d (Read[begin, End] -> "
+                     "starting position(s) on the reference)===================================*/\n",
+                     smem ? "SMEM" : "MEM", (int)(first_read_no + r));
+    buf.insert(buf.end(), tmp, tmp + n);
+    int32_t pb = -1, pe = -1;
+    const gs_seed_rec* sd = res->seeds + res->read_seed_off[r];
+    const uint32_t* pos = res->pos + res->read_pos_off[r];
+    for (uint32_t s = 0; s < res->read_nseeds[r]; ++s) {
+      for (uint32_t q = 0; q < sd[s].n_pos; ++q) {
+        const int32_t x = (int32_t)sd[s].qbeg, y = (int32_t)sd[s].qend;
+        if (pb == x && pe == y)
+          n = snprintf(tmp, sizeof tmp, ", %c%u", sign[(uint32_t)y >> 31], *pos);
+        else
+          n = snprintf(tmp, sizeof tmp, "\nRead[%d, %d] -> %c%u", (int32_t)(sd[s].qbeg & 0x7FFFFFFFu),
+                       (int32_t)(sd[s].qend & 0x7FFFFFFFu), sign[(uint32_t)y >> 31], *pos);
+        buf.insert(buf.end(), tmp, tmp + n);
+        pb = x;
+        pe = y;
+        ++pos;
+      }
+    }
+    buf.push_back('\n');
+    if (buf.size() > (1u << 20) && !flush()) return GS_EIO;
+  }
+  return flush() ? GS_OK : GS_EIO;
+}

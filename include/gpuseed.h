@@ -113,6 +113,75 @@ GS_API int gs_synth_preset(int config, gs_synth_ref_params* ref, gs_synth_read_p
  * CPU SA-IS; n < 2^32 - 2. */
 GS_API int gs_build_index_host(const uint8_t* codes, uint64_t n, uint32_t sa_intv, const char* prefix);
 
+/* ---- the device-resident index -----------------------------------------
+ * gs_index_open replaces bwt_restore_bwt + bwt_restore_sa + the upload and the
+ * 13-mer table of seed_gen (seed_gen.cu:2685-2736, 2840-2895).  It reads
+ * <prefix>.bwt and <prefix>.sa, checks them like the reference (GS_EIO for a
+ * missing file, GS_EFORMAT for primary/seq_len disagreement, seed_gen.cu:
+ * 2692-2707) and additionally rejects occurrence counts that disagree with the
+ * BWT, then repacks the occurrence blocks into the device layout. */
+typedef struct gs_index gs_index;
+
+typedef struct {
+  uint32_t seq_len, primary, sa_intv;
+  uint32_t L2[5];
+  uint64_t n_blocks;     /* 96-base occurrence blocks on the device          */
+  uint64_t device_bytes; /* HBM held by the index                            */
+  double t_read, t_upload, t_repack, t_table; /* seconds                      */
+} gs_index_info;
+
+GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
+GS_API int gs_index_close(gs_index* idx);
+GS_API int gs_index_get_info(const gs_index* idx, gs_index_info* info);
+/* Copies device structures back for inspection (tests): occurrence entries
+ * (4 u32 each: count, 96-bit bitmap) and 13-mer table entries (2 u32: l, u). */
+GS_API int gs_index_read_occ(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
+GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
+
+/* ---- seeding --------------------------------------------------------------
+ * gs_seed replaces one run of seed_gen's per-batch pipeline (seed_gen.cu:
+ * 2910-3790: find_seed_intervals_gpu, the compaction and filters, the
+ * segmented sort, seeds_to_threads(_mem) and locate_seeds_gpu).  Seeds come out
+ * per read in the reference's order: merged by qbeg with forward seeds first
+ * on ties, positions of a seed in suffix-array row order. */
+typedef struct {
+  int k;            /* -k: minimum seed length, >= 14 (seed_gen.cu:2821)       */
+  int smem;         /* -s: SMEM (1) or MEM (0) mode (2832)                     */
+  uint32_t max_occ; /* 0: every row located (the reference); else seeds whose
+                       interval is larger get no positions                      */
+  uint32_t batch_bases; /* MEM look-ahead stays inside batches closed at this
+                       many bases (seed_gen.cu:2814, 2955); 0 = 1,000,000     */
+  int kernel;       /* 0: plain one-stream-per-lane kernel                     */
+  int reserved[3];
+} gs_opts;
+
+typedef struct {
+  uint32_t qbeg, qend; /* bit 31 set on both for the reverse strand (2402)   */
+  uint32_t l, u;       /* suffix-array interval rows                           */
+  uint32_t n_pos;      /* positions reported for this seed                     */
+  uint32_t pad;
+} gs_seed_rec;
+
+typedef struct {
+  uint64_t n_reads, n_seeds, n_pos;
+  uint32_t* read_nseeds;   /* [n_reads]                                       */
+  uint32_t* read_npos;     /* [n_reads]                                       */
+  uint64_t* read_seed_off; /* [n_reads + 1]                                   */
+  uint64_t* read_pos_off;  /* [n_reads + 1]                                   */
+  gs_seed_rec* seeds;      /* [n_seeds]                                       */
+  uint32_t* pos;           /* [n_pos] reference positions                     */
+  /* device time of each phase, seconds (HIP events on the index's stream) */
+  double t_h2d, t_seed, t_post, t_locate, t_d2h;
+} gs_result;
+
+GS_API int gs_seed(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
+                   uint64_t n_reads, const gs_opts* opts, gs_result* out);
+GS_API void gs_result_free(gs_result* res);
+
+/* The -o text of a result (seed_gen.cu:3625-3648) written to file descriptor
+ * fd, reads numbered from first_read_no. */
+GS_API int gs_format_text(const gs_result* res, int smem, uint64_t first_read_no, int fd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -191,6 +191,9 @@ uint32_t or_lf(const or_index* X, uint32_t k) {
 /* locate_seeds_gpu (668-684) for one row */
 static uint32_t locate_count(const or_index* X, uint32_t row, uint64_t* lf_steps) {
   uint32_t itr = 0;
+  /* rows past the last one only arise from the batch-wide MEM row skip
+   * (624-667) and have no position; the reference would read past its arrays */
+  if (row > X->seq_len) return 0xFFFFFFFFu;
   while (row % X->sa_intv) {
     ++itr;
     row = or_lf(X, row);

@@ -90,6 +90,8 @@ class BruteIndex:
         """Position the reference's walk reports for `row` (seed_gen.cu:668-684).
         The walk's LF returns 0 for rows primary and primary+1 (455-457); for
         every other row it is the true LF.  Row 0 holds -1 (sa[0] = -1, 2711)."""
+        if row > self.n:  # only from the batch-wide MEM row skip; no position
+            return 0xFFFFFFFF
         itr = 0
         while row % self.sa_intv:
             itr += 1
