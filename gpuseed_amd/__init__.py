@@ -56,7 +56,8 @@ class IndexInfo(C.Structure):
     _fields_ = [("seq_len", C.c_uint32), ("primary", C.c_uint32), ("sa_intv", C.c_uint32),
                 ("L2", C.c_uint32 * 5), ("n_blocks", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("t_read", C.c_double), ("t_upload", C.c_double), ("t_repack", C.c_double),
-                ("t_table", C.c_double)]
+                ("t_table", C.c_double), ("t_sa", C.c_double), ("t_write", C.c_double),
+                ("sa_rounds", C.c_int), ("pad", C.c_int)]
 
 
 class Opts(C.Structure):
@@ -105,6 +106,8 @@ def lib():
         L.gs_build_index_host.argtypes = [u8p, C.c_uint64, C.c_uint32, C.c_char_p]
         L.gs_index_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
         L.gs_index_close.argtypes = [C.c_void_p]
+        L.gs_index_build.argtypes = [u8p, C.c_uint64, C.c_uint32, C.c_int, C.c_char_p,
+                                     C.POINTER(C.c_void_p)]
         L.gs_index_get_info.argtypes = [C.c_void_p, C.POINTER(IndexInfo)]
         L.gs_index_read_occ.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_read_kmer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
@@ -175,9 +178,16 @@ def roofline_random(buffer_bytes, line_bytes=16, mode=0, inflight=1, grid=8192, 
 class Index:
     """A device-resident index (gs_index_open)."""
 
-    def __init__(self, prefix, device=0):
+    def __init__(self, prefix=None, device=0, codes=None, sa_intv=7, write_prefix=None):
+        """Opens <prefix>.bwt/.sa, or builds the index of `codes` on the device."""
         self.h = C.c_void_p()
-        _check("gs_index_open", lib().gs_index_open(prefix.encode(), device, C.byref(self.h)))
+        if codes is None:
+            _check("gs_index_open", lib().gs_index_open(prefix.encode(), device, C.byref(self.h)))
+        else:
+            codes = np.ascontiguousarray(codes, dtype=np.uint8)
+            wp = write_prefix.encode() if write_prefix else None
+            _check("gs_index_build", lib().gs_index_build(_ptr(codes, C.c_uint8), len(codes), sa_intv,
+                                                          device, wp, C.byref(self.h)))
 
     def close(self):
         if self.h:
@@ -273,3 +283,65 @@ def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0
         return SeedOutput(res)
     finally:
         lib().gs_result_free(C.byref(res))
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("t_h2d", C.c_double), ("t_seed", C.c_double), ("t_count", C.c_double),
+                ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
+                ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64)]
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    L.gs_batch_create.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(Opts),
+                                  C.POINTER(C.c_void_p)]
+    L.gs_batch_run.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+    L.gs_batch_fetch.argtypes = [C.c_void_p, C.POINTER(Result)]
+    L.gs_batch_destroy.argtypes = [C.c_void_p]
+    L._batch_bound = True
+
+
+class Batch:
+    """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
+
+    def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0):
+        L = lib()
+        _bind_batch(L)
+        buf, offs, lens = arrays
+        self._keep = (np.ascontiguousarray(buf, dtype=np.uint8),
+                      np.ascontiguousarray(offs, dtype=np.uint64),
+                      np.ascontiguousarray(lens, dtype=np.uint32))
+        buf, offs, lens = self._keep
+        self.smem = smem
+        o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel)
+        self.h = C.c_void_p()
+        _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),
+                                                    _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32),
+                                                    len(lens), C.byref(o), C.byref(self.h)))
+        self._keep = None  # the device holds its own copy now
+
+    def run(self):
+        s = BatchStats()
+        _check("gs_batch_run", lib().gs_batch_run(self.h, C.byref(s)))
+        return s
+
+    def fetch(self):
+        res = Result()
+        _check("gs_batch_fetch", lib().gs_batch_fetch(self.h, C.byref(res)))
+        try:
+            return SeedOutput(res)
+        finally:
+            lib().gs_result_free(C.byref(res))
+
+    def close(self):
+        if self.h:
+            lib().gs_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

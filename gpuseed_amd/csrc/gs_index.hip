@@ -278,6 +278,9 @@ extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) 
   info->t_upload = X->t_upload;
   info->t_repack = X->t_repack;
   info->t_table = X->t_table;
+  info->t_sa = X->t_sa;
+  info->t_write = X->t_write;
+  info->sa_rounds = X->sa_rounds;
   return GS_OK;
 }
 

@@ -23,7 +23,8 @@ struct gs_index {
   uint32_t* sa = nullptr;
   uint2* kmer = nullptr;
   uint64_t device_bytes = 0;
-  double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0;
+  double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0;
+  int sa_rounds = 0;
   gs::DevIndex dev() const;
 };
 

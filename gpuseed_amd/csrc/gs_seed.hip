@@ -572,170 +572,264 @@ void launch_seed(const SeedParams& P, hipStream_t st) {
 
 }  // namespace
 
-// The batch of one gs_seed call, with device memory owned by the call.
-extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* offsets,
-                              const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
-                              gs_result* out) {
+// ---- device-resident batches ------------------------------------------------
+
+struct gs_batch {
+  gs_index* X = nullptr;
+  gs_opts opts;
+  uint64_t n_reads = 0, span = 0, tasks = 0;
+  uint32_t maxlen = 0;
+  DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
+  uint64_t n_seeds = 0, n_pos = 0;
+  bool ran = false;
+  hipEvent_t ev[7] = {};
+  double t_h2d = 0;
+};
+
+static void destroy_batch(gs_batch* B) {
+  if (!B) return;
+  for (auto& e : B->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete B;
+}
+
+extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint64_t* offsets,
+                                      const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
+                                      gs_batch** out) {
   if (!X || !opts || !out || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
-  memset(out, 0, sizeof(*out));
+  *out = nullptr;
   if (opts->k < gs::kPrecalc + 1 || opts->k > 1 << 20) return GS_EINVAL;
   if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  gs_batch* B = new gs_batch();
+  B->X = X;
+  B->opts = *opts;
+  B->n_reads = n_reads;
   const uint32_t k = (uint32_t)opts->k;
   const uint32_t batch_bases = opts->batch_bases ? opts->batch_bases : 1000000u;
-
-  // host-side layout: byte span, task offsets, reference batches
+  // host-side layout: byte span, task offsets, the reference's batches (2955-2991)
   std::vector<uint64_t> task_off(n_reads + 1);
-  std::vector<uint32_t> batch(n_reads ? n_reads : 1);
-  uint64_t span = 0, tasks = 0, acc = 0;
-  uint32_t bid = 0, maxlen = 0;
+  std::vector<uint32_t> rbatch(n_reads ? n_reads : 1);
+  uint64_t acc = 0;
+  uint32_t bid = 0;
   for (uint64_t i = 0; i < n_reads; ++i) {
     const uint64_t end = offsets[i] + lens[i];
-    if (end > span) span = end;
-    task_off[i] = tasks;
-    tasks += lens[i] >= k ? lens[i] - k + 1 : 0;
-    batch[i] = bid;
+    if (end > B->span) B->span = end;
+    task_off[i] = B->tasks;
+    B->tasks += lens[i] >= k ? lens[i] - k + 1 : 0;
+    rbatch[i] = bid;
     acc += lens[i];
-    if (acc >= batch_bases) {  // the batch closes after the read that reaches the limit (2955)
+    if (acc >= batch_bases) {  // a batch closes after the read that reaches the limit
       ++bid;
       acc = 0;
     }
-    if (lens[i] > maxlen) maxlen = lens[i];
+    if (lens[i] > B->maxlen) B->maxlen = lens[i];
   }
-  task_off[n_reads] = tasks;
-
-  DevBuf d_bases, d_offs, d_lens, d_toff, d_batch, d_cand, d_nF, d_nR, d_ns, d_np, d_soff, d_poff,
-      d_tiles, d_tot, d_seeds, d_rows, d_scratch;
+  task_off[n_reads] = B->tasks;
   const uint64_t nr1 = n_reads + 1;
-  if (d_bases.ensure(span + 32) || d_offs.ensure(8 * nr1) || d_lens.ensure(4 * nr1) ||
-      d_toff.ensure(8 * nr1) || d_batch.ensure(4 * nr1) || d_cand.ensure(16 * (2 * tasks + 1)) ||
-      d_nF.ensure(4 * nr1) || d_nR.ensure(4 * nr1) || d_ns.ensure(4 * nr1) || d_np.ensure(4 * nr1) ||
-      d_soff.ensure(8 * nr1) || d_poff.ensure(8 * nr1) ||
-      d_tiles.ensure(8 * (nr1 / kScanTile + 2)) || d_tot.ensure(16))
+  for (auto& e : B->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      destroy_batch(B);
+      return GS_EHIP;
+    }
+  if (B->bases.ensure(B->span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
+      B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
+      B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
+      B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
+      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16)))) {
+    destroy_batch(B);
     return GS_ENOMEM;
-  const bool long_reads = maxlen > 256;
-  if (long_reads && d_scratch.ensure(4 * (span + 16))) return GS_ENOMEM;
-
+  }
   hipStream_t st = X->stream;
-  hipEvent_t ev[6];
-  for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) return GS_EHIP;
-  auto cleanup_events = [&]() {
-    for (auto& e : ev) (void)hipEventDestroy(e);
-  };
+  int rc = GS_OK;
+  if (hipEventRecord(B->ev[0], st) != hipSuccess ||
+      (B->span && hipMemcpyAsync(B->bases.p, bases, B->span, hipMemcpyHostToDevice, st) != hipSuccess) ||
+      hipMemsetAsync(B->bases.as<uint8_t>() + B->span, 0, 32, st) != hipSuccess ||
+      (n_reads &&
+       (hipMemcpyAsync(B->offs.p, offsets, 8 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(B->lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(B->toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(B->batch.p, rbatch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess)) ||
+      hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    rc = GS_EHIP;
+  if (rc) {
+    destroy_batch(B);
+    return rc;
+  }
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
+  B->t_h2d = ms * 1e-3;
+  *out = B;
+  return GS_OK;
+}
+
+// Runs the whole device pipeline on an uploaded batch; results stay in HBM.
+extern "C" GS_API int gs_batch_run(gs_batch* B, gs_batch_stats* stats) {
+  if (!B) return GS_EINVAL;
+  gs_index* X = B->X;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  hipStream_t st = X->stream;
+  const uint64_t n_reads = B->n_reads;
+  SeedParams S;
+  S.X = X->dev();
+  S.bases = B->bases.as<uint8_t>();
+  S.offs = B->offs.as<uint64_t>();
+  S.lens = B->lens.as<uint32_t>();
+  S.task_off = B->toff.as<uint64_t>();
+  S.n_reads = (uint32_t)n_reads;
+  S.k = (uint32_t)B->opts.k;
+  S.smem = B->opts.smem ? 1 : 0;
+  S.pad = 0;
+  S.cand = B->cand.as<uint4>();
+  S.nF = B->nF.as<uint32_t>();
+  S.nR = B->nR.as<uint32_t>();
+  S.scratch = B->scratch.p;
+  PostParams Q;
+  Q.cand = S.cand;
+  Q.nF = S.nF;
+  Q.nR = S.nR;
+  Q.task_off = S.task_off;
+  Q.lens = S.lens;
+  Q.batch = B->batch.as<uint32_t>();
+  Q.n_reads = (uint32_t)n_reads;
+  Q.k = S.k;
+  Q.smem = S.smem;
+  Q.max_occ = B->opts.max_occ;
+  Q.nseeds = B->ns.as<uint32_t>();
+  Q.npos = B->np.as<uint32_t>();
+  Q.seed_off = B->soff.as<uint64_t>();
+  Q.pos_off = B->poff.as<uint64_t>();
+  GS_HIP(hipEventRecord(B->ev[0], st));
+  if (n_reads) {
+    if (B->maxlen <= 160)
+      launch_seed<uint16_t, 160>(S, st);
+    else
+      launch_seed<uint16_t, 256>(S, st);
+  }
+  GS_HIP(hipEventRecord(B->ev[1], st));
+  uint64_t* tot = nullptr;
+  GS_HIP(hipHostMalloc((void**)&tot, 16, hipHostMallocDefault));
+  tot[0] = tot[1] = 0;
   int rc = GS_OK;
   do {
-    if (hipEventRecord(ev[0], st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (span && hipMemcpyAsync(d_bases.p, bases, span, hipMemcpyHostToDevice, st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (hipMemsetAsync(d_bases.as<uint8_t>() + span, 0, 32, st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (n_reads &&
-        (hipMemcpyAsync(d_offs.p, offsets, 8 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-         hipMemcpyAsync(d_lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-         hipMemcpyAsync(d_toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
-         hipMemcpyAsync(d_batch.p, batch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess)) {
-      rc = GS_EHIP;
-      break;
-    }
-    if (hipEventRecord(ev[1], st) != hipSuccess) { rc = GS_EHIP; break; }
-
-    SeedParams S;
-    S.X = X->dev();
-    S.bases = d_bases.as<uint8_t>();
-    S.offs = d_offs.as<uint64_t>();
-    S.lens = d_lens.as<uint32_t>();
-    S.task_off = d_toff.as<uint64_t>();
-    S.n_reads = (uint32_t)n_reads;
-    S.k = k;
-    S.smem = opts->smem ? 1 : 0;
-    S.pad = 0;
-    S.cand = d_cand.as<uint4>();
-    S.nF = d_nF.as<uint32_t>();
-    S.nR = d_nR.as<uint32_t>();
-    S.scratch = d_scratch.p;
-    if (n_reads) {
-      if (maxlen <= 160)
-        launch_seed<uint16_t, 160>(S, st);
-      else
-        launch_seed<uint16_t, 256>(S, st);
-    }
-    if (hipEventRecord(ev[2], st) != hipSuccess) { rc = GS_EHIP; break; }
-
-    PostParams Q;
-    Q.cand = S.cand;
-    Q.nF = S.nF;
-    Q.nR = S.nR;
-    Q.task_off = S.task_off;
-    Q.lens = S.lens;
-    Q.batch = d_batch.as<uint32_t>();
-    Q.n_reads = (uint32_t)n_reads;
-    Q.k = k;
-    Q.smem = S.smem;
-    Q.max_occ = opts->max_occ;
-    Q.nseeds = d_ns.as<uint32_t>();
-    Q.npos = d_np.as<uint32_t>();
-    Q.seed_off = d_soff.as<uint64_t>();
-    Q.pos_off = d_poff.as<uint64_t>();
     if (n_reads) {
       hipLaunchKernelGGL(finalize_reads<false>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
-      scan_counts(Q.nseeds, n_reads, d_soff.as<uint64_t>(), d_tiles.as<uint64_t>(), d_tot.as<uint64_t>(), st);
-      scan_counts(Q.npos, n_reads, d_poff.as<uint64_t>(), d_tiles.as<uint64_t>(), d_tot.as<uint64_t>() + 1, st);
+      scan_counts(Q.nseeds, n_reads, B->soff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
+      scan_counts(Q.npos, n_reads, B->poff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>() + 1, st);
+      if (hipMemcpyAsync(tot, B->tot.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
+    } else {
+      const uint64_t zero[2] = {0, 0};
+      if (hipMemcpyAsync(B->soff.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(B->poff.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) { rc = GS_EHIP; break; }
     }
-    uint64_t tot[2] = {0, 0};
-    if (n_reads && (hipMemcpyAsync(tot, d_tot.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                    hipStreamSynchronize(st) != hipSuccess)) {
-      rc = GS_EHIP;
-      break;
-    }
-    out->n_reads = n_reads;
-    out->n_seeds = tot[0];
-    out->n_pos = tot[1];
-    if (d_seeds.ensure(sizeof(gs_seed_rec) * (tot[0] + 1)) || d_rows.ensure(4 * (tot[1] + 1))) {
+    if (hipEventRecord(B->ev[2], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
+    B->n_seeds = tot[0];
+    B->n_pos = tot[1];
+    if (B->seeds.ensure(sizeof(gs_seed_rec) * (B->n_seeds + 1)) || B->rows.ensure(4 * (B->n_pos + 1))) {
       rc = GS_ENOMEM;
       break;
     }
-    Q.seeds = d_seeds.as<gs_seed_rec>();
-    Q.rows = d_rows.as<uint32_t>();
+    Q.seeds = B->seeds.as<gs_seed_rec>();
+    Q.rows = B->rows.as<uint32_t>();
+    if (hipEventRecord(B->ev[3], st) != hipSuccess) { rc = GS_EHIP; break; }
     if (n_reads) hipLaunchKernelGGL(finalize_reads<true>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
-    if (hipEventRecord(ev[3], st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (tot[1]) hipLaunchKernelGGL(locate_rows, dim3(grid_for(tot[1], 256)), dim3(256), 0, st, S.X, Q.rows, tot[1]);
-    if (hipEventRecord(ev[4], st) != hipSuccess) { rc = GS_EHIP; break; }
-
-    out->read_nseeds = (uint32_t*)malloc(4 * nr1);
-    out->read_npos = (uint32_t*)malloc(4 * nr1);
-    out->read_seed_off = (uint64_t*)malloc(8 * nr1);
-    out->read_pos_off = (uint64_t*)malloc(8 * nr1);
-    out->seeds = (gs_seed_rec*)malloc(sizeof(gs_seed_rec) * (tot[0] + 1));
-    out->pos = (uint32_t*)malloc(4 * (tot[1] + 1));
-    if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
-        !out->pos) {
-      rc = GS_ENOMEM;
-      break;
-    }
-    if (n_reads &&
-        (hipMemcpyAsync(out->read_nseeds, Q.nseeds, 4 * n_reads, hipMemcpyDeviceToHost, st) != hipSuccess ||
-         hipMemcpyAsync(out->read_npos, Q.npos, 4 * n_reads, hipMemcpyDeviceToHost, st) != hipSuccess ||
-         hipMemcpyAsync(out->read_seed_off, d_soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
-         hipMemcpyAsync(out->read_pos_off, d_poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess)) {
-      rc = GS_EHIP;
-      break;
-    }
-    if (tot[0] && hipMemcpyAsync(out->seeds, Q.seeds, sizeof(gs_seed_rec) * tot[0], hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (tot[1] && hipMemcpyAsync(out->pos, Q.rows, 4 * tot[1], hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (hipEventRecord(ev[5], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (!n_reads) {
-      out->read_seed_off[0] = 0;
-      out->read_pos_off[0] = 0;
-    }
-    float ms[5];
-    for (int q = 0; q < 5; ++q) (void)hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
-    out->t_h2d = ms[0] * 1e-3;
-    out->t_seed = ms[1] * 1e-3;
-    out->t_post = ms[2] * 1e-3;
-    out->t_locate = ms[3] * 1e-3;
-    out->t_d2h = ms[4] * 1e-3;
+    if (hipEventRecord(B->ev[4], st) != hipSuccess) { rc = GS_EHIP; break; }
+    if (B->n_pos)
+      hipLaunchKernelGGL(locate_rows, dim3(grid_for(B->n_pos, 256)), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+    if (hipEventRecord(B->ev[5], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
   } while (0);
-  cleanup_events();
-  if (rc) gs_result_free(out);
+  (void)hipHostFree(tot);
+  if (rc) return rc;
+  B->ran = true;
+  if (stats) {
+    float ms[5] = {0, 0, 0, 0, 0};
+    (void)hipEventElapsedTime(&ms[0], B->ev[0], B->ev[1]);
+    (void)hipEventElapsedTime(&ms[1], B->ev[1], B->ev[2]);
+    (void)hipEventElapsedTime(&ms[2], B->ev[3], B->ev[4]);
+    (void)hipEventElapsedTime(&ms[3], B->ev[4], B->ev[5]);
+    (void)hipEventElapsedTime(&ms[4], B->ev[0], B->ev[5]);
+    memset(stats, 0, sizeof(*stats));
+    stats->t_seed = ms[0] * 1e-3;
+    stats->t_count = ms[1] * 1e-3;
+    stats->t_write = ms[2] * 1e-3;
+    stats->t_locate = ms[3] * 1e-3;
+    stats->t_total = ms[4] * 1e-3;
+    stats->t_h2d = B->t_h2d;
+    stats->n_reads = n_reads;
+    stats->n_seeds = B->n_seeds;
+    stats->n_pos = B->n_pos;
+  }
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
+  if (!B || !out || !B->ran) return GS_EINVAL;
+  memset(out, 0, sizeof(*out));
+  if (hipSetDevice(B->X->device) != hipSuccess) return GS_EHIP;
+  hipStream_t st = B->X->stream;
+  const uint64_t n = B->n_reads, nr1 = n + 1;
+  out->n_reads = n;
+  out->n_seeds = B->n_seeds;
+  out->n_pos = B->n_pos;
+  out->read_nseeds = (uint32_t*)malloc(4 * nr1);
+  out->read_npos = (uint32_t*)malloc(4 * nr1);
+  out->read_seed_off = (uint64_t*)malloc(8 * nr1);
+  out->read_pos_off = (uint64_t*)malloc(8 * nr1);
+  out->seeds = (gs_seed_rec*)malloc(sizeof(gs_seed_rec) * (B->n_seeds + 1));
+  out->pos = (uint32_t*)malloc(4 * (B->n_pos + 1));
+  if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
+      !out->pos) {
+    gs_result_free(out);
+    return GS_ENOMEM;
+  }
+  hipEvent_t e0 = B->ev[5], e1 = B->ev[6];
+  int rc = GS_OK;
+  if (hipEventRecord(e0, st) != hipSuccess ||
+      (n && (hipMemcpyAsync(out->read_nseeds, B->ns.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipMemcpyAsync(out->read_npos, B->np.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+      hipMemcpyAsync(out->read_seed_off, B->soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(out->read_pos_off, B->poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      (B->n_seeds && hipMemcpyAsync(out->seeds, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds,
+                                    hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      (B->n_pos && hipMemcpyAsync(out->pos, B->rows.p, 4 * B->n_pos, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    rc = GS_EHIP;
+  if (rc) {
+    gs_result_free(out);
+    return rc;
+  }
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  out->t_d2h = ms * 1e-3;
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_batch_destroy(gs_batch* B) {
+  if (!B) return GS_EINVAL;
+  (void)hipSetDevice(B->X->device);
+  destroy_batch(B);
+  return GS_OK;
+}
+
+// One batch from host memory to host memory: create, run, fetch.
+extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* offsets,
+                              const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
+                              gs_result* out) {
+  if (!out) return GS_EINVAL;
+  memset(out, 0, sizeof(*out));
+  gs_batch* B = nullptr;
+  int rc = gs_batch_create(X, bases, offsets, lens, n_reads, opts, &B);
+  if (rc) return rc;
+  gs_batch_stats s;
+  rc = gs_batch_run(B, &s);
+  if (!rc) rc = gs_batch_fetch(B, out);
+  if (!rc) {
+    out->t_h2d = s.t_h2d;
+    out->t_seed = s.t_seed;
+    out->t_post = s.t_count + s.t_write;
+    out->t_locate = s.t_locate;
+  }
+  gs_batch_destroy(B);
   return rc;
 }
 

@@ -128,6 +128,9 @@ typedef struct {
   uint64_t n_blocks;     /* 96-base occurrence blocks on the device          */
   uint64_t device_bytes; /* HBM held by the index                            */
   double t_read, t_upload, t_repack, t_table; /* seconds                      */
+  double t_sa, t_write; /* gs_index_build: suffix sorting, file writing      */
+  int sa_rounds;        /* gs_index_build: prefix-doubling rounds            */
+  int pad;
 } gs_index_info;
 
 GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
@@ -137,6 +140,14 @@ GS_API int gs_index_get_info(const gs_index* idx, gs_index_info* info);
  * (4 u32 each: count, 96-bit bitmap) and 13-mer table entries (2 u32: l, u). */
 GS_API int gs_index_read_occ(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
 GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
+
+/* Builds the index of codes[0..n) (values 0..3) on the device: suffix array
+ * by prefix doubling, then the same device structures gs_index_open makes.
+ * With write_prefix, also writes <write_prefix>.bwt and .sa in the reference
+ * layout (byte-identical to gs_build_index_host).  Replaces the bwa fork that
+ * build_index.sh drives (build_index.sh:54-75).  n < 2^32 - 2. */
+GS_API int gs_index_build(const uint8_t* codes, uint64_t n, uint32_t sa_intv, int device,
+                          const char* write_prefix, gs_index** out);
 
 /* ---- seeding --------------------------------------------------------------
  * gs_seed replaces one run of seed_gen's per-batch pipeline (seed_gen.cu:
@@ -177,6 +188,31 @@ typedef struct {
 GS_API int gs_seed(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
                    uint64_t n_reads, const gs_opts* opts, gs_result* out);
 GS_API void gs_result_free(gs_result* res);
+
+/* Device-resident batches: gs_seed is gs_batch_create + gs_batch_run +
+ * gs_batch_fetch.  gs_batch_create uploads the reads once; gs_batch_run runs
+ * the whole device pipeline and leaves the results in HBM (it may be called
+ * again, e.g. for timing); gs_batch_fetch copies them to a gs_result.  One
+ * batch at a time per index (they share the index's stream).  No counterpart
+ * in the reference, whose batches are host round trips (seed_gen.cu:3108-3601). */
+typedef struct gs_batch gs_batch;
+
+typedef struct {
+  double t_h2d;    /* upload at gs_batch_create                              */
+  double t_seed;   /* backward-extension kernel                              */
+  double t_count;  /* per-read counts and scans                              */
+  double t_write;  /* seed records and suffix-array rows                      */
+  double t_locate; /* suffix-array walk                                       */
+  double t_total;  /* first kernel start to last kernel end                   */
+  uint64_t n_reads, n_seeds, n_pos;
+} gs_batch_stats;
+
+GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,
+                           const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
+                           gs_batch** out);
+GS_API int gs_batch_run(gs_batch* batch, gs_batch_stats* stats);
+GS_API int gs_batch_fetch(gs_batch* batch, gs_result* out);
+GS_API int gs_batch_destroy(gs_batch* batch);
 
 /* The -o text of a result (seed_gen.cu:3625-3648) written to file descriptor
  * fd, reads numbered from first_read_no. */
