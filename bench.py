@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X seeder on one GPU.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--config 3] ...
+
+Workload (default --config 3): a 3.1 Gbp synthetic reference (41 % GC, 45 %
+interspersed repeats, 3 % tandem repeats, 5 % segmental duplications) indexed
+on the GPU with suffix-array sampling 7, and 150 bp reads (0.5 %
+substitutions, 0.05 % indels, both strands) seeded in SMEM mode with -k 19,
+reporting positions for intervals with at most 500 occurrences.  A step is
+one pass of the seeding pipeline (extension kernel, SMEM filtering, scans,
+suffix-array walk) over one batch of --batch reads that were uploaded to HBM
+before the timed region; results stay in HBM.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import shutil
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+T0 = time.time()
+
+
+def log(*a):
+    print("[bench %7.1fs]" % (time.time() - T0), *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
+    """The oracle (C restatement of the reference path, distance-1 pruning,
+    13-mer table built before timing) on `sample` reads of the first batch;
+    also checks those reads bit for bit against the GPU output."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import oracle_bind as ob
+    buf, offs, lens = arrays
+    n = min(sample, len(lens))
+    reads = [buf[int(offs[i]):int(offs[i]) + int(lens[i])].tobytes() for i in range(n)]
+    ox = ob.Index(prefix)
+    t = time.time()
+    ob.seed(ox, reads[:1], k, smem=True, prune=1, max_occ=max_occ, threads=threads, use_table=1)
+    log("oracle: index loaded, 13-mer table built in %.1fs" % (time.time() - t))
+    t = time.time()
+    got, counters = ob.seed(ox, reads, k, smem=True, prune=1, max_occ=max_occ, threads=threads,
+                            use_table=1)
+    dt = time.time() - t
+    log("oracle: %d reads in %.2fs on %d threads" % (n, dt, threads))
+    bad = 0
+    first_bad = None
+    for r in range(n):
+        want = [(x, y, l, u, list(p)) for (x, y, l, u, p) in got[r]]
+        if gpu_out.read(r) != want:
+            bad += 1
+            if first_bad is None:
+                first_bad = r
+    ox.close()
+    del np
+    return dict(reads=n, seconds=dt, reads_per_s=n / dt, counters=counters, mismatches=bad,
+                first_mismatch=first_bad)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3])
+    ap.add_argument("--batch", type=int, default=1000000, help="reads per step")
+    ap.add_argument("--k", type=int, default=0, help="default: 20 for config 1, 19 otherwise")
+    ap.add_argument("--max-occ", type=int, default=-1, help="default: 0 for config 1, 500 otherwise")
+    ap.add_argument("--ref-length", type=int, default=0, help="override the reference length")
+    ap.add_argument("--cpu-sample", type=int, default=30000, help="reads timed on the CPU")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this project has exactly one MI355X, "
+              "and reads shard across GPUs without exchange (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import gpuseed_amd as g
+    smem = True
+    k = args.k or (20 if args.config == 1 else 19)
+    max_occ = args.max_occ if args.max_occ >= 0 else (0 if args.config == 1 else 500)
+    rp, qp = g.preset(args.config)
+    if args.ref_length:
+        rp.length = args.ref_length
+    n_batches = args.warmup + args.steps
+    qp.n_reads = n_batches * args.batch
+
+    log("generating %.2f Gbp reference (config %d)" % (rp.length / 1e9, args.config))
+    ref = g.synth_reference(rp)
+    need = rp.length * 1.2 + (1 << 30)  # .bwt + .sa files
+    where = [d for d in (tempfile.gettempdir(), "/dev/shm", ROOT)
+             if os.path.isdir(d) and os.access(d, os.W_OK) and shutil.disk_usage(d).free > need]
+    tmpdir = tempfile.mkdtemp(prefix="gsbench_", dir=where[0] if where else None)
+    prefix = os.path.join(tmpdir, "ref")
+    log("building the index on the GPU (files under %s, %.0f GB free)" %
+        (tmpdir, shutil.disk_usage(tmpdir).free / 1e9))
+    t = time.time()
+    X = g.Index(codes=ref, sa_intv=7, write_prefix=prefix)
+    info = X.info()
+    t_build = time.time() - t
+    log("index: %.1fs total; suffix sort %.1fs in %d rounds, occ %.1fs, files %.1fs, table %.2fs, "
+        "%.2f GB on device" % (t_build, info.t_sa, info.sa_rounds, info.t_repack, info.t_write,
+                               info.t_table, info.device_bytes / 1e9))
+    log("generating %d reads" % qp.n_reads)
+    buf, offs, lens, _, _ = g.synth_reads(ref, qp)
+    del ref
+    batches = []
+    for b in range(n_batches):
+        lo, hi = b * args.batch, (b + 1) * args.batch
+        base = int(offs[lo])
+        end = int(offs[hi - 1] + lens[hi - 1])
+        arrays = (buf[base:end], offs[lo:hi] - np.uint64(base), lens[lo:hi])
+        batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ)))
+    log("%d batches of %d reads resident in HBM" % (n_batches, args.batch))
+
+    for b in range(args.warmup):
+        s = batches[b][1].run()
+        log("warmup %d: %.1f ms (seed %.1f ms)" % (b, s.t_total * 1e3, s.t_seed * 1e3))
+    stats = []
+    t0 = time.perf_counter()
+    for b in range(args.warmup, n_batches):
+        stats.append(batches[b][1].run())
+    t1 = time.perf_counter()
+    for i, s in enumerate(stats):
+        log("step %d: %.1f ms total, seed %.1f, count %.1f, write %.1f, locate %.1f; %d seeds, %d positions"
+            % (i, s.t_total * 1e3, s.t_seed * 1e3, s.t_count * 1e3, s.t_write * 1e3, s.t_locate * 1e3,
+               s.n_seeds, s.n_pos))
+    elapsed = t1 - t0
+    reads = args.steps * args.batch
+    value = reads / elapsed
+    log("timed: %d reads in %.3f s -> %.0f reads/s" % (reads, elapsed, value))
+    if args.profile_steps:
+        print(json.dumps({"metric": "profile run", "value": value}))
+        return
+
+    seed_s = sum(s.t_seed for s in stats) / len(stats)
+    first_arrays, first_batch = batches[args.warmup]
+    gpu_out = first_batch.fetch()
+    for _, bt in batches:
+        bt.close()
+
+    # random-access ceiling of this GPU (16-byte dependent requests, 8 GiB)
+    rr = g.roofline_random(8 << 30, line_bytes=16, mode=1, inflight=1, grid=2048, iters=512, reps=3)
+    r_max = rr.lines_per_s
+    log("random-access ceiling: %.1f G requests/s" % (r_max / 1e9))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(prefix, first_arrays, gpu_out, args.cpu_sample, threads, k, max_occ)
+        cpu["threads"] = threads
+        log("cpu baseline: %.0f reads/s on %d threads; parity %d/%d reads differ"
+            % (cpu["reads_per_s"], threads, cpu["mismatches"], cpu["reads"]))
+    X.close()
+    shutil.rmtree(tmpdir, ignore_errors=True)
+
+    roofline = None
+    extra = {}
+    if cpu:
+        c = cpu["counters"]
+        n = cpu["reads"]
+        # extension kernel, algorithmic: one 8-byte table entry per task with a
+        # clean 13-base window, one 16-byte occ entry per distinct 96-base block
+        # touched by each backward step (exact distance-1 pruning)
+        req_per_read = (c["w_tasks"] + c["w_blocks96"]) / n
+        bytes_per_read = (8.0 * c["w_tasks"] + 16.0 * c["w_blocks96"]) / n
+        achieved = bytes_per_read * args.batch / seed_s
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
+                    "frac": achieved / 8e12, "traffic": None,
+                    "kernel": "seed_streams", "kernel_ms": seed_s * 1e3,
+                    "random_access": {"requests_per_read": req_per_read,
+                                      "achieved_requests_per_s": req_per_read * args.batch / seed_s,
+                                      "measured_ceiling_requests_per_s": r_max,
+                                      "frac": req_per_read * args.batch / seed_s / r_max}}
+        extra["parity"] = {"reads_checked": n, "mismatches": cpu["mismatches"],
+                           "first_mismatch": cpu["first_mismatch"]}
+    mean_ms = elapsed / args.steps * 1e3
+    out = {
+        "metric": "SMEM reads/s (150bp,k=19,3.1Gbp) batched vs plain, bit-exact; p99 ms",
+        "value": value,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mean_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "config %d: SMEM -k %d, %d bp reads, %.2f Gbp synthetic reference, "
+                               "positions for intervals <= %d, plain kernel"
+                               % (args.config, k, int(lens[0]), rp.length / 1e9, max_occ),
+                   "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
+                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7, "kernel": "plain"},
+        "roofline": roofline,
+        "cpu_baseline": None if not cpu else {
+            "value": cpu["reads_per_s"], "unit": "reads/s", "cores": cpu["threads"], "kind": "port",
+            "sample": "first %d reads of the first timed batch, oracle with distance-1 pruning and the "
+                      "13-mer table (table build excluded), %s" % (cpu["reads"], cpu_model())},
+        "index_build_s": t_build,
+        "kernel_ms": {"seed": seed_s * 1e3,
+                      "count": sum(s.t_count for s in stats) / len(stats) * 1e3,
+                      "write": sum(s.t_write for s in stats) / len(stats) * 1e3,
+                      "locate": sum(s.t_locate for s in stats) / len(stats) * 1e3},
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

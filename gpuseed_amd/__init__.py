@@ -288,7 +288,8 @@ def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0
 class BatchStats(C.Structure):
     _fields_ = [("t_h2d", C.c_double), ("t_seed", C.c_double), ("t_count", C.c_double),
                 ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
-                ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64)]
+                ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
+                ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64)]
 
 
 def _bind_batch(L):
@@ -297,7 +298,7 @@ def _bind_batch(L):
     L.gs_batch_create.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(Opts),
                                   C.POINTER(C.c_void_p)]
-    L.gs_batch_run.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+    L.gs_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(BatchStats)]
     L.gs_batch_fetch.argtypes = [C.c_void_p, C.POINTER(Result)]
     L.gs_batch_destroy.argtypes = [C.c_void_p]
     L._batch_bound = True
@@ -322,9 +323,9 @@ class Batch:
                                                     len(lens), C.byref(o), C.byref(self.h)))
         self._keep = None  # the device holds its own copy now
 
-    def run(self):
+    def run(self, kernel=-1):
         s = BatchStats()
-        _check("gs_batch_run", lib().gs_batch_run(self.h, C.byref(s)))
+        _check("gs_batch_run", lib().gs_batch_run(self.h, kernel, C.byref(s)))
         return s
 
     def fetch(self):

@@ -28,6 +28,32 @@ struct gs_index {
   gs::DevIndex dev() const;
 };
 
+namespace gs {
+
+// Parameters of the persistent extension kernel (gs_persist.hip).
+struct PersistParams {
+  DevIndex X;
+  const uint4* codes;        // 4-bit virtual codes, per read: forward chunks then reverse chunks
+  const uint64_t* code_off;  // per read, in 16-byte chunks
+  const uint32_t* lens;
+  const uint64_t* task_off;
+  uint32_t n_reads, k, smem, pad;
+  uint4* cand;               // same candidate layout as the plain kernel
+  uint32_t* nF;
+  uint32_t* nR;
+  uint32_t* queue;           // work-queue counter
+  uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads
+};
+
+constexpr uint32_t kPersistLmax = 160;  // longest read held in a persistent lane's LDS slot
+
+int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t* lens, const uint64_t* code_off,
+                      uint32_t n_reads, uint32_t lmax, uint4* codes, hipStream_t st);
+// variant 0: 8-bit size records, 1: 16-bit size records
+int launch_persistent(const PersistParams& P, int variant, hipStream_t st);
+
+}  // namespace gs
+
 // ceil(2^64 / d) for d >= 2 (exact quotient for every 32-bit numerator), 0 for d == 1
 uint64_t gs_reciprocal64(uint32_t d);
 

@@ -1,0 +1,313 @@
+// Persistent extension kernel (the batched form of the seeding path).
+//
+// Same stream procedure and output as seed_streams in gs_seed.hip (the plain
+// one-stream-per-lane kernel), restructured for throughput on gfx950:
+//  * a fixed grid that fills the chip; every lane pulls (read, strand)
+//    streams from a device work queue (one atomic per wave refill, leader
+//    lane + ballot) and starts the next stream as soon as its current one
+//    ends, so no lane idles while its wave finishes a long stream;
+//  * reads are pre-packed to 4-bit codes of both virtual strands by
+//    pack_codes, so a refill is a few 16-byte loads and LDS stores;
+//  * a lane's LDS slot holds its stream's codes (4 bits per base) and its
+//    size record (one AT per position), indexed directly in LDS;
+//  * the loop body is one memory operation (a 13-mer table entry or one or
+//    two occ entries) followed by a short processing step; the rare
+//    transitions (windows with a code > 3, N-transparent positions, reaching
+//    the read start) are resolved inside the processing step.
+#include "gs_internal.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
+  return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
+}
+
+template <typename AT, int LMAX, int BS>
+__global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
+  static_assert(LMAX % 32 == 0, "LMAX must be a multiple of 32");
+  constexpr int CODE_BYTES = LMAX / 2;
+  constexpr int A_BYTES = LMAX * (int)sizeof(AT);
+  constexpr int SLOT = CODE_BYTES + A_BYTES;
+  static_assert(SLOT % 16 == 0, "slot must be 16-byte aligned");
+  constexpr uint32_t SAT = (uint32_t)(AT)(~(AT)0);
+  __shared__ __align__(16) uint8_t lds[BS * SLOT];
+  const uint32_t sbase = threadIdx.x * SLOT;
+  AT* const A = reinterpret_cast<AT*>(lds + sbase + CODE_BYTES);
+  const int lane = threadIdx.x & 63;
+  const uint32_t k = P.k;
+  const uint32_t L2_0 = P.X.L2[0], L2_1 = P.X.L2[1], L2_2 = P.X.L2[2], L2_3 = P.X.L2[3];
+  const uint32_t primary = P.X.primary;
+  const uint64_t n_items = 2ull * P.n_reads;
+
+  auto code = [&](int32_t i) -> uint32_t { return (lds[sbase + (i >> 1)] >> ((i & 1) << 2)) & 15u; };
+
+  bool have = false, finished = false;
+  uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0, p = 0, nbad = 0;
+  uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
+  int32_t e = 0, x = 0;
+  int phase = 0;
+  uint4* out = nullptr;
+  uint64_t n_table = 0, n_occ = 0;
+
+  auto end_stream = [&]() {
+    (strand ? P.nR : P.nF)[r] = nout;
+    have = false;
+  };
+  auto advance = [&]() -> bool {
+    if ((uint32_t)(e - 1) < k) return false;
+    const uint32_t cn = code(e - 14), co = code(e - 1);
+    nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
+    p = (p >> 2) | ((cn & 3u) << 24);
+    --e;
+    return true;
+  };
+  // next task with a clean 13-base window, or end of the stream
+  auto next_task = [&]() {
+    phase = 0;
+    do {
+      if (!advance()) {
+        end_stream();
+        return;
+      }
+    } while (nbad != 0);
+  };
+  auto prune_at = [&](int32_t pos, uint32_t size) -> bool {
+    const uint32_t sv = size >= SAT ? SAT : size;
+    if (size < SAT && (uint32_t)A[pos] == sv) return true;
+    A[pos] = (AT)sv;
+    return false;
+  };
+  auto finish_task = [&]() {
+    if ((uint32_t)e - b >= k && (!P.smem || b < last_b)) {
+      if (strand == 0)
+        out[ntask - 1 - nout] = make_uint4(b, (uint32_t)e, l, u);
+      else
+        out[ntask + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
+      ++nout;
+    }
+    last_b = b;
+  };
+  // phase 1: settle x on the next position with a base code
+  auto prepare_x = [&]() {
+    while (true) {
+      if (x < 0) {
+        finish_task();
+        next_task();
+        return;
+      }
+      const uint32_t c = code(x);
+      if (c <= 3) {
+        cx = c;
+        return;
+      }
+      b = (uint32_t)x;  // code > 3 after the window: interval kept, position consumed
+      if (prune_at(x, s)) {
+        next_task();
+        return;
+      }
+      --x;
+    }
+  };
+  auto start_stream = [&](uint32_t item) {
+    r = item >> 1;
+    strand = item & 1u;
+    L = P.lens[r];
+    if (L > (uint32_t)LMAX) return;  // longer reads go through the device-memory kernel
+    if (L < k) {
+      (strand ? P.nR : P.nF)[r] = 0;
+      return;
+    }
+    const uint32_t nch = (L + 31) >> 5;
+    const uint4* src = P.codes + P.code_off[r] + strand * nch;
+    for (uint32_t c = 0; c < nch; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * c) = src[c];
+    const uint32_t za = (L * (uint32_t)sizeof(AT) + 15) >> 4;
+    for (uint32_t c = 0; c < za; ++c)
+      *reinterpret_cast<uint4*>(lds + sbase + CODE_BYTES + 16 * c) = make_uint4(0, 0, 0, 0);
+    ntask = L - k + 1;
+    out = P.cand + 2 * P.task_off[r];
+    nout = 0;
+    last_b = 0xFFFFFFFFu;
+    e = (int32_t)L;
+    p = 0;
+    nbad = 0;
+    for (int j = 0; j < gs::kPrecalc; ++j) {
+      const uint32_t c = code((int32_t)L - 1 - j);
+      nbad += c > 3 ? 1u : 0u;
+      p |= (c & 3u) << (2 * j);
+    }
+    have = true;
+    phase = 0;
+    if (nbad != 0) next_task();
+  };
+
+  while (true) {
+    // lanes without a stream take the next items of the queue
+    const bool need = !have && !finished;
+    const uint64_t needmask = __ballot(need);
+    if (needmask) {
+      const int leader = __ffsll((unsigned long long)needmask) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(P.queue, (uint32_t)__popcll(needmask));
+      base = __shfl(base, leader);
+      if (need) {
+        const uint64_t item = (uint64_t)base + __popcll(needmask & ((1ull << lane) - 1ull));
+        if (item >= n_items)
+          finished = true;
+        else
+          start_stream((uint32_t)item);
+      }
+    }
+    const int op = have ? (phase == 0 ? 1 : 2) : 0;
+    if (__ballot(op != 0) == 0) {
+      if (__ballot(!finished) == 0) break;
+      continue;
+    }
+    // one memory operation per lane
+    const uint4* a1 = nullptr;
+    const uint4* a2 = nullptr;
+    uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
+    bool have1 = false;
+    if (op == 1) {
+      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);
+      ++n_table;
+    } else if (op == 2) {
+      const uint32_t r1 = l - 1u;
+      have1 = r1 != 0xFFFFFFFFu;
+      i2 = u - (u >= primary ? 1u : 0u);
+      b2 = gs::div96(i2);
+      i1 = have1 ? r1 - (r1 >= primary ? 1u : 0u) : 0u;
+      b1 = gs::div96(i1);
+      a1 = P.X.occ + (size_t)b2 * 4 + cx;
+      if (have1 && b1 != b2) a2 = P.X.occ + (size_t)b1 * 4 + cx;
+      n_occ += a2 ? 2 : 1;
+    }
+    uint4 d1 = make_uint4(0, 0, 0, 0);
+    if (a1) d1 = *a1;
+    uint4 d2 = d1;
+    if (a2) d2 = *a2;
+
+    if (op == 1) {
+      const uint32_t il = (p & 1u) ? d1.z : d1.x, iu = (p & 1u) ? d1.w : d1.y;
+      if (il > iu) {
+        next_task();
+      } else {
+        l = il;
+        u = iu;
+        s = u - l + 1u;
+        b = (uint32_t)e - gs::kPrecalc;
+        if (prune_at((int32_t)b, s)) {
+          next_task();
+        } else {
+          x = e - gs::kPrecalc - 1;
+          phase = 1;
+          prepare_x();
+        }
+      }
+    } else if (op == 2) {
+      const uint32_t o2 = gs::rank_in_entry(d1, i2 - b2 * gs::kBlockBases);
+      const uint32_t o1 = have1 ? gs::rank_in_entry(d2, i1 - b1 * gs::kBlockBases) : 0u;
+      const uint32_t base = pick4(cx, L2_0, L2_1, L2_2, L2_3);
+      const uint32_t l2 = base + o1 + 1u, u2 = base + o2;
+      if (l2 > u2) {
+        finish_task();
+        next_task();
+      } else {
+        l = l2;
+        u = u2;
+        s = u - l + 1u;
+        b = (uint32_t)x;
+        if (prune_at(x, s)) {
+          next_task();
+        } else {
+          --x;
+          prepare_x();
+        }
+      }
+    }
+  }
+  if (P.counters) {
+    // wave totals, one atomic per wave
+    for (int d = 32; d > 0; d >>= 1) {
+      n_table += __shfl_down(n_table, d, 64);
+      n_occ += __shfl_down(n_occ, d, 64);
+    }
+    if (lane == 0) {
+      atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)n_table);
+      atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)n_occ);
+    }
+  }
+}
+
+// 4-bit codes of both virtual strands of every read of length <= lmax:
+// forward v[i] = code of byte i, reverse v[i] = reverse-complement code of
+// byte L-1-i (pack_4bit_fow / pack_4bit_rev, seed_gen.cu:2536-2578); two codes
+// per byte, even position in the low nibble, 32 codes per 16-byte chunk.
+__global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ bases, const uint64_t* __restrict__ offs,
+                                                  const uint32_t* __restrict__ lens,
+                                                  const uint64_t* __restrict__ code_off, uint32_t n_reads,
+                                                  uint32_t lmax, uint4* __restrict__ codes) {
+  const uint32_t cmax = lmax >> 5;
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  const uint64_t r = t / (2 * cmax);
+  if (r >= n_reads) return;
+  const uint32_t rest = (uint32_t)(t - r * 2 * cmax);
+  const uint32_t strand = rest / cmax, c = rest - strand * cmax;
+  const uint32_t L = lens[r];
+  if (L > lmax) return;
+  const uint32_t nch = (L + 31) >> 5;
+  if (c >= nch) return;
+  const uint8_t* s = bases + offs[r];
+  uint32_t w[4] = {0, 0, 0, 0};
+  for (uint32_t j = 0; j < 32; ++j) {
+    const uint32_t i = c * 32 + j;
+    if (i >= L) break;
+    const uint32_t v = strand ? gs::code_rev(s[L - 1 - i]) : gs::code_fwd(s[i]);
+    w[j >> 3] |= v << ((j & 7) << 2);
+  }
+  codes[code_off[r] + strand * nch + c] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__global__ void zero_words(uint32_t* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <typename AT, int LMAX, int BS>
+int launch_persistent_t(const gs::PersistParams& P, hipStream_t st) {
+  int dev = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return GS_EHIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return GS_EHIP;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, seed_persistent<AT, LMAX, BS>, BS, 0) != hipSuccess ||
+      per_cu < 1)
+    return GS_EHIP;
+  const unsigned grid = (unsigned)(per_cu * prop.multiProcessorCount);
+  hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, P);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+}  // namespace
+
+namespace gs {
+
+int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t* lens, const uint64_t* code_off,
+                      uint32_t n_reads, uint32_t lmax, uint4* codes, hipStream_t st) {
+  const uint64_t threads = (uint64_t)n_reads * 2 * (lmax >> 5);
+  if (!threads) return GS_OK;
+  hipLaunchKernelGGL(pack_codes, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, bases, offs, lens,
+                     code_off, n_reads, lmax, codes);
+  return GS_OK;
+}
+
+int launch_persistent(const PersistParams& P, int variant, hipStream_t st) {
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, P.queue, 1u);
+  if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 4u);
+  switch (variant) {
+    case 1:
+      return launch_persistent_t<uint16_t, 160, 128>(P, st);
+    default:
+      return launch_persistent_t<uint8_t, 160, 128>(P, st);
+  }
+}
+
+}  // namespace gs

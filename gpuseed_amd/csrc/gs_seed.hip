@@ -46,7 +46,7 @@ struct SeedParams {
   const uint64_t* offs;     // per read
   const uint32_t* lens;     // per read
   const uint64_t* task_off; // per read: sum of tasks of earlier reads
-  uint32_t n_reads, k, smem, pad;
+  uint32_t n_reads, k, smem, min_len;  // min_len > 0: only reads longer than min_len
   uint4* cand;              // per read: 2*task_off .. : forward region then reverse region
   uint32_t* nF;
   uint32_t* nR;
@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
   const uint32_t r = gid >> 1;
   const uint32_t strand = gid & 1;
   uint8_t* slot = lds + (threadIdx.x >> 1) * PAIR;
-  const bool live = r < P.n_reads;
-  const uint32_t L = live ? P.lens[r] : 0;
+  const uint32_t L0 = r < P.n_reads ? P.lens[r] : 0;
+  const bool live = r < P.n_reads && (P.min_len == 0 || L0 > P.min_len);
+  const uint32_t L = live ? L0 : 0;
   const uint64_t off = live ? P.offs[r] : 0;
   const bool in_lds = L <= (uint32_t)LMAX;
 
@@ -580,6 +581,8 @@ struct gs_batch {
   uint64_t n_reads = 0, span = 0, tasks = 0;
   uint32_t maxlen = 0;
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
+  DevBuf codes, code_off, queue, counters;  // persistent kernel
+  uint64_t code_chunks = 0;
   uint64_t n_seeds = 0, n_pos = 0;
   bool ran = false;
   hipEvent_t ev[7] = {};
@@ -627,6 +630,17 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
   }
   task_off[n_reads] = B->tasks;
   const uint64_t nr1 = n_reads + 1;
+  // 4-bit code chunks of reads short enough for the persistent kernel's LDS slots
+  std::vector<uint64_t> code_off(nr1);
+  for (uint64_t i = 0; i < n_reads; ++i) {
+    code_off[i] = B->code_chunks;
+    if (lens[i] <= gs::kPersistLmax) B->code_chunks += 2 * ((lens[i] + 31) / 32);
+  }
+  code_off[n_reads] = B->code_chunks;
+  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1)) {
+    destroy_batch(B);
+    return GS_ENOMEM;
+  }
   for (auto& e : B->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       destroy_batch(B);
@@ -636,7 +650,8 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
       B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
       B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
       B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
-      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16)))) {
+      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
+      B->queue.ensure(16) || B->counters.ensure(32)) {
     destroy_batch(B);
     return GS_ENOMEM;
   }
@@ -649,7 +664,8 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
        (hipMemcpyAsync(B->offs.p, offsets, 8 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(B->lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(B->toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->batch.p, rbatch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess)) ||
+        hipMemcpyAsync(B->batch.p, rbatch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(B->code_off.p, code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess)) ||
       hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
     rc = GS_EHIP;
   if (rc) {
@@ -664,7 +680,7 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
 }
 
 // Runs the whole device pipeline on an uploaded batch; results stay in HBM.
-extern "C" GS_API int gs_batch_run(gs_batch* B, gs_batch_stats* stats) {
+extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_stats* stats) {
   if (!B) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
@@ -679,11 +695,11 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, gs_batch_stats* stats) {
   S.n_reads = (uint32_t)n_reads;
   S.k = (uint32_t)B->opts.k;
   S.smem = B->opts.smem ? 1 : 0;
-  S.pad = 0;
   S.cand = B->cand.as<uint4>();
   S.nF = B->nF.as<uint32_t>();
   S.nR = B->nR.as<uint32_t>();
   S.scratch = B->scratch.p;
+  S.min_len = 0;
   PostParams Q;
   Q.cand = S.cand;
   Q.nF = S.nF;
@@ -700,7 +716,37 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, gs_batch_stats* stats) {
   Q.seed_off = B->soff.as<uint64_t>();
   Q.pos_off = B->poff.as<uint64_t>();
   GS_HIP(hipEventRecord(B->ev[0], st));
-  if (n_reads) {
+  const int kernel = kernel_override >= 0 ? kernel_override : B->opts.kernel;
+  if (n_reads && kernel >= 1) {
+    // batched form: pack codes, persistent kernel for reads that fit its LDS
+    // slots, the plain kernel for the longer ones
+    gs::PersistParams PP;
+    PP.X = S.X;
+    PP.codes = B->codes.as<uint4>();
+    PP.code_off = B->code_off.as<uint64_t>();
+    PP.lens = S.lens;
+    PP.task_off = S.task_off;
+    PP.n_reads = S.n_reads;
+    PP.k = S.k;
+    PP.smem = S.smem;
+    PP.pad = 0;
+    PP.cand = S.cand;
+    PP.nF = S.nF;
+    PP.nR = S.nR;
+    PP.queue = B->queue.as<uint32_t>();
+    PP.counters = B->counters.as<uint64_t>();
+    int rc0 = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, gs::kPersistLmax,
+                                    B->codes.as<uint4>(), st);
+    if (!rc0) rc0 = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, st);
+    if (rc0) return rc0;
+    if (B->maxlen > gs::kPersistLmax) {
+      S.min_len = gs::kPersistLmax;
+      if (B->maxlen <= 256)
+        launch_seed<uint16_t, 256>(S, st);
+      else
+        launch_seed<uint16_t, 256>(S, st);
+    }
+  } else if (n_reads) {
     if (B->maxlen <= 160)
       launch_seed<uint16_t, 160>(S, st);
     else
@@ -758,6 +804,13 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, gs_batch_stats* stats) {
     stats->n_reads = n_reads;
     stats->n_seeds = B->n_seeds;
     stats->n_pos = B->n_pos;
+    if (kernel >= 1 && n_reads) {
+      uint64_t c[2] = {0, 0};
+      if (hipMemcpy(c, B->counters.p, 16, hipMemcpyDeviceToHost) == hipSuccess) {
+        stats->n_table_reads = c[0];
+        stats->n_occ_reads = c[1];
+      }
+    }
   }
   return GS_OK;
 }
@@ -821,7 +874,7 @@ extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* of
   int rc = gs_batch_create(X, bases, offsets, lens, n_reads, opts, &B);
   if (rc) return rc;
   gs_batch_stats s;
-  rc = gs_batch_run(B, &s);
+  rc = gs_batch_run(B, -1, &s);
   if (!rc) rc = gs_batch_fetch(B, out);
   if (!rc) {
     out->t_h2d = s.t_h2d;

@@ -2,6 +2,7 @@
  * gs_roofline - measures the random-access roofline of the current GPU.
  *
  *   gs_roofline [-b GiB] [-i iters] [-g grid] [-r reps] [-l line] [-m mode] [-f inflight]
+ *   gs_roofline -c GiB [-r reps]     (streaming copy of GiB, reads GiB and writes GiB)
  *
  * Modes: 0 independent whole-line reads, 1 dependent chains, 2 one 16-byte
  * chunk of a random line, 3 cooperative (LINE/16 lanes per line), 4 random
@@ -34,7 +35,8 @@ static int run(double gib, int line, int mode, int inflight, int grid, int iters
 int main(int argc, char** argv) {
   double gib = 8.0;
   int iters = 256, grid = 8192, reps = 3, line = 0, mode = -1, inflight = 0, c;
-  while ((c = getopt(argc, argv, "b:i:g:r:l:m:f:")) >= 0) {
+  double copy_gib = 0;
+  while ((c = getopt(argc, argv, "b:i:g:r:l:m:f:c:")) >= 0) {
     switch (c) {
       case 'b': gib = atof(optarg); break;
       case 'i': iters = atoi(optarg); break;
@@ -43,8 +45,17 @@ int main(int argc, char** argv) {
       case 'l': line = atoi(optarg); break;
       case 'm': mode = atoi(optarg); break;
       case 'f': inflight = atoi(optarg); break;
+      case 'c': copy_gib = atof(optarg); break;
       default: return 2;
     }
+  }
+  if (copy_gib > 0) { /* streaming copy of known size, for hardware byte-counter calibration */
+    double sec = 0;
+    uint64_t bytes = (uint64_t)(copy_gib * 1073741824.0);
+    if (gs_stream_copy_bytes(bytes, reps, &sec) != GS_OK) return 1;
+    printf("{\"copy_bytes\": %llu, \"seconds\": %.6f, \"gbytes_per_s\": %.1f}\n",
+           (unsigned long long)bytes, sec, 2.0 * bytes / sec * 1e-9);
+    return 0;
   }
   static const int lines[] = {16, 32, 64, 128};
   static const int infl[] = {1, 2, 4};

@@ -162,7 +162,9 @@ typedef struct {
                        interval is larger get no positions                      */
   uint32_t batch_bases; /* MEM look-ahead stays inside batches closed at this
                        many bases (seed_gen.cu:2814, 2955); 0 = 1,000,000     */
-  int kernel;       /* 0: plain one-stream-per-lane kernel                     */
+  int kernel;       /* 0: plain kernel, one stream per lane (the drop-in);
+                       1: batched persistent kernel, 8-bit size records;
+                       2: batched persistent kernel, 16-bit size records      */
   int reserved[3];
 } gs_opts;
 
@@ -205,12 +207,14 @@ typedef struct {
   double t_locate; /* suffix-array walk                                       */
   double t_total;  /* first kernel start to last kernel end                   */
   uint64_t n_reads, n_seeds, n_pos;
+  uint64_t n_table_reads, n_occ_reads; /* memory operations of the batched
+                      extension kernel (0 for the plain kernel)                */
 } gs_batch_stats;
 
 GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,
                            const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
                            gs_batch** out);
-GS_API int gs_batch_run(gs_batch* batch, gs_batch_stats* stats);
+GS_API int gs_batch_run(gs_batch* batch, int kernel, gs_batch_stats* stats); /* kernel -1: as created */
 GS_API int gs_batch_fetch(gs_batch* batch, gs_result* out);
 GS_API int gs_batch_destroy(gs_batch* batch);
 

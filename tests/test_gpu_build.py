@@ -79,7 +79,13 @@ def test_batch_api_matches_gs_seed(built, tmp_path):
         s1 = bt.run()
         s2 = bt.run()
         two = bt.fetch()
+        s3 = bt.run(kernel=1)
+        three = bt.fetch()
+        s4 = bt.run(kernel=2)
+        four = bt.fetch()
         bt.close()
-    assert s1.n_seeds == s2.n_seeds == len(one.seeds)
-    assert np.array_equal(one.seeds, two.seeds) and np.array_equal(one.pos, two.pos)
+    assert s1.n_seeds == s2.n_seeds == s3.n_seeds == s4.n_seeds == len(one.seeds)
+    for o in (two, three, four):
+        assert np.array_equal(one.seeds, o.seeds) and np.array_equal(one.pos, o.pos)
     assert s2.t_seed > 0 and s2.t_total >= s2.t_seed
+    assert s3.n_table_reads > 0 and s3.n_occ_reads > 0

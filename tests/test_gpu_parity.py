@@ -88,11 +88,12 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("kernel", [0, 1, 2])
 @pytest.mark.parametrize("ci", range(len(CASES)))
 @pytest.mark.parametrize("smem", [True, False])
-def test_tiny_cases_match_oracle_and_brute_force(g, tmp_path, ci, smem):
+def test_tiny_cases_match_oracle_and_brute_force(g, tmp_path, ci, smem, kernel):
     seed, n, rep, sa_intv, nr, (lmin, lmax), k = CASES[ci]
-    say("case %d smem=%d" % (ci, smem))
+    say("case %d smem=%d kernel=%d" % (ci, smem, kernel))
     rng = random.Random(seed)
     text = repetitive_text(rng, n) if rep else random_text(rng, n)
     reads = sample_reads(rng, text, nr, lmin, lmax)
@@ -105,12 +106,13 @@ def test_tiny_cases_match_oracle_and_brute_force(g, tmp_path, ci, smem):
     if n <= 3000:
         assert want == (brute.smem if smem else brute.mem)(brute.BruteIndex(text, sa_intv), reads, k)
     with g.Index(prefix) as X:
-        got = g.seed(X, reads, k=k, smem=smem).as_lists()
+        got = g.seed(X, reads, k=k, smem=smem, kernel=kernel).as_lists()
     for i, (a, b) in enumerate(zip(got, want)):
         assert a == b, "read %d %r" % (i, reads[i])
 
 
-def test_mem_batch_lookahead_on_device(g, tmp_path):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_mem_batch_lookahead_on_device(g, tmp_path, kernel):
     rng = random.Random(99)
     text = random_text(rng, 3000)
     prefix = build_index(tmp_path, text, 1)
@@ -119,7 +121,7 @@ def test_mem_batch_lookahead_on_device(g, tmp_path):
     rev = revcomp(text[2000:2040]).encode()
     reads = [fwd, fwd, rev, rev, fwd]
     with g.Index(prefix) as X:
-        got = g.seed(X, reads, k=20, smem=False).as_lists()
+        got = g.seed(X, reads, k=20, smem=False, kernel=kernel).as_lists()
     assert got == oracle_lists(ox, reads, 20, False)
 
 
@@ -148,18 +150,19 @@ def config1(g, tmp_path_factory):
     return dict(prefix=prefix, reads=reads, arrays=(buf, offs, lens), dir=str(d))
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("smem,k", [(False, 20), (True, 20), (True, 19)])
-def test_config1_records_and_text_match_oracle(g, config1, smem, k):
+def test_config1_records_and_text_match_oracle(g, config1, smem, k, kernel):
     """Configuration 1 (1 Mbp reference, 10,000 x 100 bp reads), all records
     and the -o text byte for byte."""
-    say("config1 smem=%d k=%d" % (smem, k))
+    say("config1 smem=%d k=%d kernel=%d" % (smem, k, kernel))
     ox = ob.Index(config1["prefix"])
     want = oracle_lists(ox, config1["reads"], k, smem)
     txt = os.path.join(config1["dir"], "gpu_%d_%d.txt" % (smem, k))
     with g.Index(config1["prefix"]) as X:
         fd = os.open(txt, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o644)
         try:
-            out = g.seed(X, arrays=config1["arrays"], k=k, smem=smem, text_fd=fd)
+            out = g.seed(X, arrays=config1["arrays"], k=k, smem=smem, text_fd=fd, kernel=kernel)
         finally:
             os.close(fd)
     got = out.as_lists()
