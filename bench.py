@@ -44,36 +44,56 @@ def cpu_model():
 
 def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
     """The oracle (C restatement of the reference path, distance-1 pruning,
-    13-mer table built before timing) on `sample` reads of the first batch;
-    also checks those reads bit for bit against the GPU output."""
+    13-mer table built before timing) on the first `sample` reads of a batch;
+    its output is also compared bit for bit with the GPU's for those reads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
 
     import oracle_bind as ob
     buf, offs, lens = arrays
     n = min(sample, len(lens))
-    reads = [buf[int(offs[i]):int(offs[i]) + int(lens[i])].tobytes() for i in range(n)]
+    end = int(offs[n - 1] + lens[n - 1])
+    sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
     t = time.time()
-    ob.seed(ox, reads[:1], k, smem=True, prune=1, max_occ=max_occ, threads=threads, use_table=1)
+    ob.seed_arrays(ox, buf[:int(lens[0])], offs[:1], lens[:1], k, True, max_occ=max_occ, threads=threads)
     log("oracle: index loaded, 13-mer table built in %.1fs" % (time.time() - t))
     t = time.time()
-    got, counters = ob.seed(ox, reads, k, smem=True, prune=1, max_occ=max_occ, threads=threads,
-                            use_table=1)
+    seeds, pos, so, po, counters = ob.seed_arrays(ox, *sub, k, True, max_occ=max_occ, threads=threads)
     dt = time.time() - t
     log("oracle: %d reads in %.2fs on %d threads" % (n, dt, threads))
-    bad = 0
-    first_bad = None
-    for r in range(n):
-        want = [(x, y, l, u, list(p)) for (x, y, l, u, p) in got[r]]
-        if gpu_out.read(r) != want:
-            bad += 1
-            if first_bad is None:
-                first_bad = r
     ox.close()
-    del np
+    g_seeds = gpu_out.seeds.view(np.uint32).reshape(-1, 6)[:int(gpu_out.read_seed_off[n])]
+    g_pos = gpu_out.pos[:int(gpu_out.read_pos_off[n])]
+    same_off = (np.array_equal(so, gpu_out.read_seed_off[:n + 1]) and
+                np.array_equal(po, gpu_out.read_pos_off[:n + 1]))
+    same = same_off and np.array_equal(seeds, g_seeds) and np.array_equal(pos, g_pos)
+    bad = 0 if same else n
+    if not same and same_off:
+        # count the reads whose seeds or positions differ
+        sd = np.any(seeds != g_seeds, axis=1)
+        pd = pos != g_pos
+        rs = set(np.searchsorted(so, np.nonzero(sd)[0], side="right") - 1)
+        rs |= set(np.searchsorted(po, np.nonzero(pd)[0], side="right") - 1)
+        bad = len(rs)
     return dict(reads=n, seconds=dt, reads_per_s=n / dt, counters=counters, mismatches=bad,
-                first_mismatch=first_bad)
+                bit_exact=bool(same))
+
+
+def timed_steps(batches, warmup, kernel):
+    for b in range(warmup):
+        s = batches[b][1].run(kernel)
+        log("kernel %d warmup %d: %.1f ms (seed %.1f ms)" % (kernel, b, s.t_total * 1e3, s.t_seed * 1e3))
+    stats = []
+    t0 = time.perf_counter()
+    for b in range(warmup, len(batches)):
+        stats.append(batches[b][1].run(kernel))
+    t1 = time.perf_counter()
+    for i, s in enumerate(stats):
+        log("kernel %d step %d: %.1f ms total, seed %.1f, count %.1f, write %.1f, locate %.1f; "
+            "%d seeds, %d positions" % (kernel, i, s.t_total * 1e3, s.t_seed * 1e3, s.t_count * 1e3,
+                                        s.t_write * 1e3, s.t_locate * 1e3, s.n_seeds, s.n_pos))
+    return t1 - t0, stats
 
 
 def main():
@@ -86,9 +106,11 @@ def main():
     ap.add_argument("--k", type=int, default=0, help="default: 20 for config 1, 19 otherwise")
     ap.add_argument("--max-occ", type=int, default=-1, help="default: 0 for config 1, 500 otherwise")
     ap.add_argument("--ref-length", type=int, default=0, help="override the reference length")
-    ap.add_argument("--cpu-sample", type=int, default=30000, help="reads timed on the CPU")
+    ap.add_argument("--cpu-sample", type=int, default=1000000, help="reads timed on the CPU")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,25 +158,21 @@ def main():
         batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ)))
     log("%d batches of %d reads resident in HBM" % (n_batches, args.batch))
 
-    for b in range(args.warmup):
-        s = batches[b][1].run()
-        log("warmup %d: %.1f ms (seed %.1f ms)" % (b, s.t_total * 1e3, s.t_seed * 1e3))
-    stats = []
-    t0 = time.perf_counter()
-    for b in range(args.warmup, n_batches):
-        stats.append(batches[b][1].run())
-    t1 = time.perf_counter()
-    for i, s in enumerate(stats):
-        log("step %d: %.1f ms total, seed %.1f, count %.1f, write %.1f, locate %.1f; %d seeds, %d positions"
-            % (i, s.t_total * 1e3, s.t_seed * 1e3, s.t_count * 1e3, s.t_write * 1e3, s.t_locate * 1e3,
-               s.n_seeds, s.n_pos))
-    elapsed = t1 - t0
+    elapsed, stats = timed_steps(batches, args.warmup, args.kernel)
     reads = args.steps * args.batch
     value = reads / elapsed
-    log("timed: %d reads in %.3f s -> %.0f reads/s" % (reads, elapsed, value))
+    log("batched kernel %d: %d reads in %.3f s -> %.0f reads/s" % (args.kernel, reads, elapsed, value))
     if args.profile_steps:
         print(json.dumps({"metric": "profile run", "value": value}))
         return
+    plain = None
+    if not args.no_plain:
+        p_elapsed, p_stats = timed_steps(batches, min(1, args.warmup), 0)
+        plain = dict(reads_per_s=(len(batches) - min(1, args.warmup)) * args.batch / p_elapsed,
+                     seed_ms=sum(s.t_seed for s in p_stats) / len(p_stats) * 1e3)
+        log("plain kernel: %.0f reads/s" % plain["reads_per_s"])
+        # leave the batched results in place for the parity check below
+        batches[args.warmup][1].run(args.kernel)
 
     seed_s = sum(s.t_seed for s in stats) / len(stats)
     first_arrays, first_batch = batches[args.warmup]
@@ -186,17 +204,19 @@ def main():
         # clean 13-base window, one 16-byte occ entry per distinct 96-base block
         # touched by each backward step (exact distance-1 pruning)
         req_per_read = (c["w_tasks"] + c["w_blocks96"]) / n
+        act = sum(st.n_table_reads + st.n_occ_reads for st in stats) / (len(stats) * args.batch)
         bytes_per_read = (8.0 * c["w_tasks"] + 16.0 * c["w_blocks96"]) / n
         achieved = bytes_per_read * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
                     "frac": achieved / 8e12, "traffic": None,
-                    "kernel": "seed_streams", "kernel_ms": seed_s * 1e3,
+                    "kernel": "seed_persistent", "kernel_ms": seed_s * 1e3,
                     "random_access": {"requests_per_read": req_per_read,
+                                      "issued_requests_per_read": act,
                                       "achieved_requests_per_s": req_per_read * args.batch / seed_s,
                                       "measured_ceiling_requests_per_s": r_max,
                                       "frac": req_per_read * args.batch / seed_s / r_max}}
         extra["parity"] = {"reads_checked": n, "mismatches": cpu["mismatches"],
-                           "first_mismatch": cpu["first_mismatch"]}
+                           "bit_exact": cpu["bit_exact"]}
     mean_ms = elapsed / args.steps * 1e3
     out = {
         "metric": "SMEM reads/s (150bp,k=19,3.1Gbp) batched vs plain, bit-exact; p99 ms",
@@ -212,16 +232,20 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": "config %d: SMEM -k %d, %d bp reads, %.2f Gbp synthetic reference, "
-                               "positions for intervals <= %d, plain kernel"
+                               "positions for intervals <= %d, batched persistent kernel"
                                % (args.config, k, int(lens[0]), rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
-                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7, "kernel": "plain"},
+                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7,
+                   "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {
             "value": cpu["reads_per_s"], "unit": "reads/s", "cores": cpu["threads"], "kind": "port",
             "sample": "first %d reads of the first timed batch, oracle with distance-1 pruning and the "
-                      "13-mer table (table build excluded), %s" % (cpu["reads"], cpu_model())},
+                      "13-mer table (table build excluded), %s, compared bit for bit with the GPU output"
+                      % (cpu["reads"], cpu_model())},
         "index_build_s": t_build,
+        "plain": None if not plain else {"reads_per_s": plain["reads_per_s"], "seed_kernel_ms": plain["seed_ms"],
+                                         "batched_over_plain": value / plain["reads_per_s"]},
         "kernel_ms": {"seed": seed_s * 1e3,
                       "count": sum(s.t_count for s in stats) / len(stats) * 1e3,
                       "write": sum(s.t_write for s in stats) / len(stats) * 1e3,

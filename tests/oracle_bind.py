@@ -156,3 +156,32 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0):
         return out, counters
     finally:
         lib().or_result_free(C.byref(res))
+
+
+def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, use_table=1):
+    """Flat numpy copies of the oracle's result: (seeds[n,6] u32, pos u32,
+    read_seed_off u64[n+1], read_pos_off u64[n+1]) plus the work counters."""
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table)
+    res = OrResult()
+    rc = lib().or_seed_reads(index.h, C.cast(buf.ctypes.data, C.c_char_p),
+                             offs.ctypes.data_as(C.POINTER(C.c_uint64)),
+                             lens.ctypes.data_as(C.POINTER(C.c_uint32)), len(lens), C.byref(o),
+                             C.byref(res))
+    if rc != 0:
+        raise RuntimeError("or_seed_reads failed: %d" % rc)
+    try:
+        n = len(lens)
+        ns, npos = res.n_seeds, res.n_pos
+        seeds = (np.ctypeslib.as_array(C.cast(res.seeds, C.POINTER(C.c_uint32)), (ns * 6,)).copy()
+                 .reshape(ns, 6) if ns else np.zeros((0, 6), np.uint32))
+        pos = np.ctypeslib.as_array(res.pos, (npos,)).copy() if npos else np.zeros(0, np.uint32)
+        so = np.ctypeslib.as_array(res.read_seed_off, (n + 1,)).copy()
+        po = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
+        counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
+                                                 "w_lf", "w_sa", "w_steps_unpruned", "n_short")}
+        return seeds, pos, so, po, counters
+    finally:
+        lib().or_result_free(C.byref(res))
