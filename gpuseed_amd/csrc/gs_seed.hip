@@ -27,10 +27,8 @@
 //  * SMEM filtering happens in the lane: tasks finish in decreasing end order
 //    and left ends never increase, so a task is a survivor exactly when its
 //    left end is smaller than the previous finished task's (991-1018).
-#include <fcntl.h>
 #include <stdlib.h>
 #include <string.h>
-#include <unistd.h>
 
 #include <vector>
 
@@ -895,50 +893,4 @@ extern "C" GS_API void gs_result_free(gs_result* r) {
   free(r->seeds);
   free(r->pos);
   memset(r, 0, sizeof(*r));
-}
-
-// -o output (seed_gen.cu:3625-3648), buffered.
-extern "C" GS_API int gs_format_text(const gs_result* res, int smem, uint64_t first_read_no, int fd) {
-  if (!res || fd < 0) return GS_EINVAL;
-  std::vector<char> buf;
-  buf.reserve(1 << 20);
-  char tmp[256];
-  auto flush = [&]() -> bool {
-    size_t off = 0;
-    while (off < buf.size()) {
-      ssize_t w = write(fd, buf.data() + off, buf.size() - off);
-      if (w <= 0) return false;
-      off += (size_t)w;
-    }
-    buf.clear();
-    return true;
-  };
-  static const char sign[2] = {'+', '-'};
-  for (uint64_t r = 0; r < res->n_reads; ++r) {
-    int n = snprintf(tmp, sizeof tmp,
-                     "/*===================================%s seeds in read no. %d (Read[begin, End] -> "
-                     "starting position(s) on the reference)===================================*/\n",
-                     smem ? "SMEM" : "MEM", (int)(first_read_no + r));
-    buf.insert(buf.end(), tmp, tmp + n);
-    int32_t pb = -1, pe = -1;
-    const gs_seed_rec* sd = res->seeds + res->read_seed_off[r];
-    const uint32_t* pos = res->pos + res->read_pos_off[r];
-    for (uint32_t s = 0; s < res->read_nseeds[r]; ++s) {
-      for (uint32_t q = 0; q < sd[s].n_pos; ++q) {
-        const int32_t x = (int32_t)sd[s].qbeg, y = (int32_t)sd[s].qend;
-        if (pb == x && pe == y)
-          n = snprintf(tmp, sizeof tmp, ", %c%u", sign[(uint32_t)y >> 31], *pos);
-        else
-          n = snprintf(tmp, sizeof tmp, "\nRead[%d, %d] -> %c%u", (int32_t)(sd[s].qbeg & 0x7FFFFFFFu),
-                       (int32_t)(sd[s].qend & 0x7FFFFFFFu), sign[(uint32_t)y >> 31], *pos);
-        buf.insert(buf.end(), tmp, tmp + n);
-        pb = x;
-        pe = y;
-        ++pos;
-      }
-    }
-    buf.push_back('\n');
-    if (buf.size() > (1u << 20) && !flush()) return GS_EIO;
-  }
-  return flush() ? GS_OK : GS_EIO;
 }
