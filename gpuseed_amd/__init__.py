@@ -62,7 +62,8 @@ class IndexInfo(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
-                ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("reserved", C.c_int * 3)]
+                ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("seg_tasks", C.c_uint32),
+                ("reserved", C.c_int * 2)]
 
 
 class SeedRec(C.Structure):
@@ -266,13 +267,13 @@ def pack_reads(reads):
 
 
 def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
-         text_fd=None):
+         text_fd=None, seg_tasks=0):
     """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
     buf, offs, lens = arrays if arrays is not None else pack_reads(reads)
     buf = np.ascontiguousarray(buf, dtype=np.uint8)
     offs = np.ascontiguousarray(offs, dtype=np.uint64)
     lens = np.ascontiguousarray(lens, dtype=np.uint32)
-    o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel)
+    o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks)
     res = Result()
     _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                                     _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32), len(lens),
@@ -307,7 +308,7 @@ def _bind_batch(L):
 class Batch:
     """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
 
-    def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0):
+    def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0):
         L = lib()
         _bind_batch(L)
         buf, offs, lens = arrays
@@ -316,7 +317,7 @@ class Batch:
                       np.ascontiguousarray(lens, dtype=np.uint32))
         buf, offs, lens = self._keep
         self.smem = smem
-        o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel)
+        o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks)
         self.h = C.c_void_p()
         _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                                                     _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32),

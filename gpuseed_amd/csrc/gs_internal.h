@@ -31,26 +31,56 @@ struct gs_index {
 namespace gs {
 
 // Parameters of the persistent extension kernel (gs_persist.hip).
+//
+// Work items: one per (read, strand) for reads of at most kPersistLmax bases;
+// reads longer than that are cut into segments of seg_tasks consecutive task
+// ends, one item per (read, segment, strand).  items[i] = read | (segment << 33)
+// | (strand << 32); the items of a read are adjacent, segment-major.
 struct PersistParams {
   DevIndex X;
   const uint4* codes;        // 4-bit virtual codes, per read: forward chunks then reverse chunks
   const uint64_t* code_off;  // per read, in 16-byte chunks
   const uint32_t* lens;
   const uint64_t* task_off;
-  uint32_t n_reads, k, smem, pad;
+  const uint64_t* items;
+  uint64_t n_items;
+  uint32_t n_reads, k, smem, seg_tasks;
   uint4* cand;               // same candidate layout as the plain kernel
   uint32_t* nF;
   uint32_t* nR;
+  uint4* seg_meta;           // per item of a segmented read: {emitted, first b, last b, flags}
   uint32_t* queue;           // work-queue counter
   uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads
 };
 
-constexpr uint32_t kPersistLmax = 160;  // longest read held in a persistent lane's LDS slot
+// Joins the segments of long reads into the per-read candidate layout.
+struct JoinParams {
+  const uint32_t* long_reads;   // reads cut into segments
+  const uint64_t* item0;        // per long read: index of its first item
+  const uint32_t* lens;
+  const uint64_t* task_off;
+  const uint4* seg_meta;
+  uint4* cand;
+  uint32_t* nF;
+  uint32_t* nR;
+  uint32_t n_long, k, smem, seg_tasks;
+};
+
+constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
+constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start
+
+// segments of a read of length L (1 for reads held whole in the LDS window)
+__host__ __device__ inline uint32_t n_segments(uint32_t L, uint32_t k, uint32_t seg_tasks) {
+  if (L <= kPersistLmax || L < k) return 1;
+  const uint32_t ntask = L - k + 1;
+  return (ntask + seg_tasks - 1) / seg_tasks;
+}
 
 int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t* lens, const uint64_t* code_off,
-                      uint32_t n_reads, uint32_t lmax, uint4* codes, hipStream_t st);
+                      uint32_t n_reads, const uint32_t* long_reads, uint32_t n_long, uint4* codes, hipStream_t st);
 // variant 0: 8-bit size records, 1: 16-bit size records
 int launch_persistent(const PersistParams& P, int variant, hipStream_t st);
+int launch_join(const JoinParams& J, hipStream_t st);
 
 }  // namespace gs
 

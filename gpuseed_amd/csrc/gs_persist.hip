@@ -37,24 +37,37 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   const uint32_t k = P.k;
   const uint32_t L2_0 = P.X.L2[0], L2_1 = P.X.L2[1], L2_2 = P.X.L2[2], L2_3 = P.X.L2[3];
   const uint32_t primary = P.X.primary;
-  const uint64_t n_items = 2ull * P.n_reads;
-
-  auto code = [&](int32_t i) -> uint32_t { return (lds[sbase + (i >> 1)] >> ((i & 1) << 2)) & 15u; };
+  const uint64_t n_items = P.n_items;
 
   bool have = false, finished = false;
   uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0, p = 0, nbad = 0;
   uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
-  int32_t e = 0, x = 0;
+  int32_t e = 0, x = 0, e_min = 0, w0 = 0;
+  uint32_t t0 = 0, first_b = 0, flags = 0;
+  uint64_t item = 0;
   int phase = 0;
   uint4* out = nullptr;
+  const uint4* gcodes = nullptr;  // the stream's codes in device memory (segments only)
   uint64_t n_table = 0, n_occ = 0;
 
+  // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
+  // a segment's extension may run left of it and then reads device memory
+  auto code = [&](int32_t i) -> uint32_t {
+    const int32_t j = i - w0;
+    if (j >= 0) return (lds[sbase + (j >> 1)] >> ((j & 1) << 2)) & 15u;
+    const uint32_t w = reinterpret_cast<const uint32_t*>(gcodes + (i >> 5))[(i >> 3) & 3];
+    return (w >> ((i & 7) << 2)) & 15u;
+  };
+
   auto end_stream = [&]() {
-    (strand ? P.nR : P.nF)[r] = nout;
+    if (gcodes)
+      P.seg_meta[item] = make_uint4(nout, first_b, last_b, flags);
+    else
+      (strand ? P.nR : P.nF)[r] = nout;
     have = false;
   };
   auto advance = [&]() -> bool {
-    if ((uint32_t)(e - 1) < k) return false;
+    if (e - 1 < e_min) return false;
     const uint32_t cn = code(e - 14), co = code(e - 1);
     nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
     p = (p >> 2) | ((cn & 3u) << 24);
@@ -71,19 +84,28 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       }
     } while (nbad != 0);
   };
+  // size record at pos (positions left of the window keep none: pruning is
+  // only an early exit, so a missing record never changes the output)
   auto prune_at = [&](int32_t pos, uint32_t size) -> bool {
+    const int32_t j = pos - w0;
+    if (j < 0) return false;
     const uint32_t sv = size >= SAT ? SAT : size;
-    if (size < SAT && (uint32_t)A[pos] == sv) return true;
-    A[pos] = (AT)sv;
+    if (size < SAT && (uint32_t)A[j] == sv) return true;
+    A[j] = (AT)sv;
     return false;
   };
   auto finish_task = [&]() {
-    if ((uint32_t)e - b >= k && (!P.smem || b < last_b)) {
+    const bool emit = (uint32_t)e - b >= k && (!P.smem || b < last_b);
+    if (emit) {
       if (strand == 0)
-        out[ntask - 1 - nout] = make_uint4(b, (uint32_t)e, l, u);
+        out[ntask - 1 - t0 - nout] = make_uint4(b, (uint32_t)e, l, u);
       else
-        out[ntask + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
+        out[ntask + t0 + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
       ++nout;
+    }
+    if (!(flags & 2u)) {  // the segment's first finished task
+      first_b = b;
+      flags = 2u | (emit ? 1u : 0u);
     }
     last_b = b;
   };
@@ -108,30 +130,50 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       --x;
     }
   };
-  auto start_stream = [&](uint32_t item) {
-    r = item >> 1;
-    strand = item & 1u;
+  auto start_stream = [&](uint64_t it) {
+    item = it;
+    const uint64_t v = P.items[it];
+    r = (uint32_t)v;
+    strand = (uint32_t)(v >> 32) & 1u;
+    const uint32_t seg = (uint32_t)(v >> 33);
     L = P.lens[r];
-    if (L > (uint32_t)LMAX) return;  // longer reads go through the device-memory kernel
     if (L < k) {
       (strand ? P.nR : P.nF)[r] = 0;
       return;
     }
+    ntask = L - k + 1;
     const uint32_t nch = (L + 31) >> 5;
     const uint4* src = P.codes + P.code_off[r] + strand * nch;
-    for (uint32_t c = 0; c < nch; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * c) = src[c];
-    const uint32_t za = (L * (uint32_t)sizeof(AT) + 15) >> 4;
+    uint32_t top, tcount;
+    if (L <= (uint32_t)LMAX) {  // the whole stream in the window
+      gcodes = nullptr;
+      t0 = 0;
+      tcount = ntask;
+      top = nch << 5;
+      w0 = 0;
+    } else {                    // one segment of task ends
+      gcodes = src;
+      t0 = seg * P.seg_tasks;
+      tcount = min(P.seg_tasks, ntask - t0);
+      top = ((L - t0) + 31u) & ~31u;
+      w0 = top > (uint32_t)LMAX ? (int32_t)(top - LMAX) : 0;
+    }
+    const uint32_t c0 = (uint32_t)w0 >> 5, c1 = top >> 5;
+    for (uint32_t c = c0; c < c1; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * (c - c0)) = src[c];
+    const uint32_t za = ((top - (uint32_t)w0) * (uint32_t)sizeof(AT) + 15) >> 4;
     for (uint32_t c = 0; c < za; ++c)
       *reinterpret_cast<uint4*>(lds + sbase + CODE_BYTES + 16 * c) = make_uint4(0, 0, 0, 0);
-    ntask = L - k + 1;
     out = P.cand + 2 * P.task_off[r];
     nout = 0;
     last_b = 0xFFFFFFFFu;
-    e = (int32_t)L;
+    first_b = 0xFFFFFFFFu;
+    flags = 0;
+    e = (int32_t)(L - t0);
+    e_min = e - (int32_t)tcount + 1;
     p = 0;
     nbad = 0;
     for (int j = 0; j < gs::kPrecalc; ++j) {
-      const uint32_t c = code((int32_t)L - 1 - j);
+      const uint32_t c = code(e - 1 - j);
       nbad += c > 3 ? 1u : 0u;
       p |= (c & 3u) << (2 * j);
     }
@@ -150,11 +192,11 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       if (lane == leader) base = atomicAdd(P.queue, (uint32_t)__popcll(needmask));
       base = __shfl(base, leader);
       if (need) {
-        const uint64_t item = (uint64_t)base + __popcll(needmask & ((1ull << lane) - 1ull));
-        if (item >= n_items)
+        const uint64_t it = (uint64_t)base + __popcll(needmask & ((1ull << lane) - 1ull));
+        if (it >= n_items)
           finished = true;
         else
-          start_stream((uint32_t)item);
+          start_stream(it);
       }
     }
     const int op = have ? (phase == 0 ? 1 : 2) : 0;
@@ -238,25 +280,11 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   }
 }
 
-// 4-bit codes of both virtual strands of every read of length <= lmax:
-// forward v[i] = code of byte i, reverse v[i] = reverse-complement code of
-// byte L-1-i (pack_4bit_fow / pack_4bit_rev, seed_gen.cu:2536-2578); two codes
-// per byte, even position in the low nibble, 32 codes per 16-byte chunk.
-__global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ bases, const uint64_t* __restrict__ offs,
-                                                  const uint32_t* __restrict__ lens,
-                                                  const uint64_t* __restrict__ code_off, uint32_t n_reads,
-                                                  uint32_t lmax, uint4* __restrict__ codes) {
-  const uint32_t cmax = lmax >> 5;
-  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-  const uint64_t r = t / (2 * cmax);
-  if (r >= n_reads) return;
-  const uint32_t rest = (uint32_t)(t - r * 2 * cmax);
-  const uint32_t strand = rest / cmax, c = rest - strand * cmax;
-  const uint32_t L = lens[r];
-  if (L > lmax) return;
-  const uint32_t nch = (L + 31) >> 5;
-  if (c >= nch) return;
-  const uint8_t* s = bases + offs[r];
+// 4-bit codes of both virtual strands: forward v[i] = code of byte i, reverse
+// v[i] = reverse-complement code of byte L-1-i (pack_4bit_fow / pack_4bit_rev,
+// seed_gen.cu:2536-2578); two codes per byte, even position in the low
+// nibble, 32 codes per 16-byte chunk.
+__device__ __forceinline__ uint4 pack_chunk(const uint8_t* s, uint32_t L, uint32_t strand, uint32_t c) {
   uint32_t w[4] = {0, 0, 0, 0};
   for (uint32_t j = 0; j < 32; ++j) {
     const uint32_t i = c * 32 + j;
@@ -264,7 +292,77 @@ __global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ ba
     const uint32_t v = strand ? gs::code_rev(s[L - 1 - i]) : gs::code_fwd(s[i]);
     w[j >> 3] |= v << ((j & 7) << 2);
   }
-  codes[code_off[r] + strand * nch + c] = make_uint4(w[0], w[1], w[2], w[3]);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// reads of length <= kPersistLmax: one thread per (read, strand, chunk)
+__global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ bases, const uint64_t* __restrict__ offs,
+                                                  const uint32_t* __restrict__ lens,
+                                                  const uint64_t* __restrict__ code_off, uint32_t n_reads,
+                                                  uint4* __restrict__ codes) {
+  constexpr uint32_t cmax = gs::kPersistLmax >> 5;
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  const uint64_t r = t / (2 * cmax);
+  if (r >= n_reads) return;
+  const uint32_t rest = (uint32_t)(t - r * 2 * cmax);
+  const uint32_t strand = rest / cmax, c = rest - strand * cmax;
+  const uint32_t L = lens[r];
+  if (L > gs::kPersistLmax) return;
+  const uint32_t nch = (L + 31) >> 5;
+  if (c >= nch) return;
+  codes[code_off[r] + strand * nch + c] = pack_chunk(bases + offs[r], L, strand, c);
+}
+
+// longer reads: one workgroup per read
+__global__ void __launch_bounds__(256) pack_codes_long(const uint8_t* __restrict__ bases,
+                                                       const uint64_t* __restrict__ offs,
+                                                       const uint32_t* __restrict__ lens,
+                                                       const uint64_t* __restrict__ code_off,
+                                                       const uint32_t* __restrict__ long_reads,
+                                                       uint4* __restrict__ codes) {
+  const uint32_t r = long_reads[blockIdx.x];
+  const uint32_t L = lens[r];
+  const uint32_t nch = (L + 31) >> 5;
+  const uint8_t* s = bases + offs[r];
+  for (uint32_t q = threadIdx.x; q < 2 * nch; q += 256) {
+    const uint32_t strand = q >= nch ? 1u : 0u;
+    codes[code_off[r] + q] = pack_chunk(s, L, strand, q - strand * nch);
+  }
+}
+
+// Joins the segments of a long read's stream (one thread per stream): moves
+// each segment's candidates next to the previous segment's and settles the
+// SMEM filter at segment boundaries.  A segment filters its own tasks in the
+// lane, but its first finished task has no predecessor there: it survives iff
+// its left end is smaller than the last finished task's of earlier segments
+// (finish_task of the persistent kernel, 991-1018).  Candidates only move
+// toward the read's end of the region, in an order that reads every source
+// before it can be overwritten.
+__global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (t >= 2ull * J.n_long) return;
+  const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
+  const uint32_t L = J.lens[r];
+  const uint32_t ntask = L - J.k + 1;
+  const uint32_t S = J.seg_tasks;
+  const uint32_t nseg = (ntask + S - 1) / S;
+  uint4* out = J.cand + 2 * J.task_off[r];
+  const uint4* meta = J.seg_meta + J.item0[t >> 1] + strand;
+  uint32_t K = 0, LB = 0xFFFFFFFFu;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    const uint4 m = meta[2 * (uint64_t)g];
+    const uint32_t t0 = g * S;
+    const uint32_t drop = (J.smem && (m.w & 1u) && !(m.y < LB)) ? 1u : 0u;
+    for (uint32_t i = drop; i < m.x; ++i) {
+      if (strand == 0)
+        out[ntask - 1 - K - (i - drop)] = out[ntask - 1 - t0 - i];
+      else
+        out[ntask + K + (i - drop)] = out[ntask + t0 + i];
+    }
+    K += m.x - drop;
+    if (m.w & 2u) LB = m.z;
+  }
+  (strand ? J.nR : J.nF)[r] = K;
 }
 
 __global__ void zero_words(uint32_t* __restrict__ p, uint32_t n) {
@@ -291,12 +389,22 @@ int launch_persistent_t(const gs::PersistParams& P, hipStream_t st) {
 namespace gs {
 
 int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t* lens, const uint64_t* code_off,
-                      uint32_t n_reads, uint32_t lmax, uint4* codes, hipStream_t st) {
-  const uint64_t threads = (uint64_t)n_reads * 2 * (lmax >> 5);
-  if (!threads) return GS_OK;
-  hipLaunchKernelGGL(pack_codes, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, bases, offs, lens,
-                     code_off, n_reads, lmax, codes);
-  return GS_OK;
+                      uint32_t n_reads, const uint32_t* long_reads, uint32_t n_long, uint4* codes, hipStream_t st) {
+  const uint64_t threads = (uint64_t)n_reads * 2 * (kPersistLmax >> 5);
+  if (threads)
+    hipLaunchKernelGGL(pack_codes, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, bases, offs, lens,
+                       code_off, n_reads, codes);
+  if (n_long)
+    hipLaunchKernelGGL(pack_codes_long, dim3(n_long), dim3(256), 0, st, bases, offs, lens, code_off, long_reads,
+                       codes);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+int launch_join(const JoinParams& J, hipStream_t st) {
+  if (!J.n_long) return GS_OK;
+  const uint64_t threads = 2ull * J.n_long;
+  hipLaunchKernelGGL(join_segments, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, J);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
 int launch_persistent(const PersistParams& P, int variant, hipStream_t st) {

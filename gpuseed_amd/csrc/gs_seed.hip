@@ -580,7 +580,9 @@ struct gs_batch {
   uint32_t maxlen = 0;
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
   DevBuf codes, code_off, queue, counters;  // persistent kernel
-  uint64_t code_chunks = 0;
+  DevBuf items, seg_meta, long_reads, item0;
+  uint64_t code_chunks = 0, n_items = 0;
+  uint32_t n_long = 0, seg_tasks = 64;
   uint64_t n_seeds = 0, n_pos = 0;
   bool ran = false;
   hipEvent_t ev[7] = {};
@@ -600,6 +602,7 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
   if (!X || !opts || !out || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
   *out = nullptr;
   if (opts->k < gs::kPrecalc + 1 || opts->k > 1 << 20) return GS_EINVAL;
+  if (opts->seg_tasks > gs::kSegTasksMax) return GS_EINVAL;
   if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   gs_batch* B = new gs_batch();
@@ -628,14 +631,37 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
   }
   task_off[n_reads] = B->tasks;
   const uint64_t nr1 = n_reads + 1;
-  // 4-bit code chunks of reads short enough for the persistent kernel's LDS slots
+  // persistent kernel: 4-bit code chunks of both strands of every read, and
+  // its work items (reads longer than its LDS window become segments)
+  B->seg_tasks = opts->seg_tasks ? opts->seg_tasks : 64u;
   std::vector<uint64_t> code_off(nr1);
+  std::vector<uint64_t> items;
+  std::vector<uint32_t> long_reads;
+  std::vector<uint64_t> item0;
+  items.reserve(2 * n_reads);
   for (uint64_t i = 0; i < n_reads; ++i) {
     code_off[i] = B->code_chunks;
-    if (lens[i] <= gs::kPersistLmax) B->code_chunks += 2 * ((lens[i] + 31) / 32);
+    B->code_chunks += 2 * ((lens[i] + 31) / 32);
+    const uint32_t nseg = gs::n_segments(lens[i], k, B->seg_tasks);
+    if (nseg > 1 || (lens[i] > gs::kPersistLmax && lens[i] >= k)) {
+      long_reads.push_back((uint32_t)i);
+      item0.push_back(items.size());
+    }
+    for (uint64_t g = 0; g < nseg; ++g) {
+      items.push_back(i | (g << 33));
+      items.push_back(i | (1ull << 32) | (g << 33));
+    }
   }
   code_off[n_reads] = B->code_chunks;
-  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1)) {
+  B->n_items = items.size();
+  B->n_long = (uint32_t)long_reads.size();
+  if (B->n_items > 0xF0000000ull) {
+    destroy_batch(B);
+    return GS_ERANGE;
+  }
+  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
+      B->items.ensure(8 * (B->n_items + 1)) || B->long_reads.ensure(4 * (B->n_long + 1)) ||
+      B->item0.ensure(8 * (B->n_long + 1)) || B->seg_meta.ensure(B->n_long ? 16 * B->n_items : 16)) {
     destroy_batch(B);
     return GS_ENOMEM;
   }
@@ -663,7 +689,12 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
         hipMemcpyAsync(B->lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(B->toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(B->batch.p, rbatch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->code_off.p, code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess)) ||
+        hipMemcpyAsync(B->code_off.p, code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(B->items.p, items.data(), 8 * B->n_items, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (B->n_long &&
+         (hipMemcpyAsync(B->long_reads.p, long_reads.data(), 4 * B->n_long, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(B->item0.p, item0.data(), 8 * B->n_long, hipMemcpyHostToDevice, st) != hipSuccess)))) ||
       hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
     rc = GS_EHIP;
   if (rc) {
@@ -724,26 +755,36 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
     PP.code_off = B->code_off.as<uint64_t>();
     PP.lens = S.lens;
     PP.task_off = S.task_off;
+    PP.items = B->items.as<uint64_t>();
+    PP.n_items = B->n_items;
     PP.n_reads = S.n_reads;
     PP.k = S.k;
     PP.smem = S.smem;
-    PP.pad = 0;
+    PP.seg_tasks = B->seg_tasks;
     PP.cand = S.cand;
     PP.nF = S.nF;
     PP.nR = S.nR;
+    PP.seg_meta = B->seg_meta.as<uint4>();
     PP.queue = B->queue.as<uint32_t>();
     PP.counters = B->counters.as<uint64_t>();
-    int rc0 = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, gs::kPersistLmax,
+    gs::JoinParams J;
+    J.long_reads = B->long_reads.as<uint32_t>();
+    J.item0 = B->item0.as<uint64_t>();
+    J.lens = S.lens;
+    J.task_off = S.task_off;
+    J.seg_meta = PP.seg_meta;
+    J.cand = S.cand;
+    J.nF = S.nF;
+    J.nR = S.nR;
+    J.n_long = B->n_long;
+    J.k = S.k;
+    J.smem = S.smem;
+    J.seg_tasks = B->seg_tasks;
+    int rc0 = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
                                     B->codes.as<uint4>(), st);
     if (!rc0) rc0 = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, st);
+    if (!rc0) rc0 = gs::launch_join(J, st);
     if (rc0) return rc0;
-    if (B->maxlen > gs::kPersistLmax) {
-      S.min_len = gs::kPersistLmax;
-      if (B->maxlen <= 256)
-        launch_seed<uint16_t, 256>(S, st);
-      else
-        launch_seed<uint16_t, 256>(S, st);
-    }
   } else if (n_reads) {
     if (B->maxlen <= 160)
       launch_seed<uint16_t, 160>(S, st);

@@ -165,7 +165,10 @@ typedef struct {
   int kernel;       /* 0: plain kernel, one stream per lane (the drop-in);
                        1: batched persistent kernel, 8-bit size records;
                        2: batched persistent kernel, 16-bit size records      */
-  int reserved[3];
+  uint32_t seg_tasks; /* batched kernels: reads longer than 160 bases are cut
+                       into work items of this many task ends, 1..112
+                       (0 = 64); the output does not depend on it             */
+  int reserved[2];
 } gs_opts;
 
 typedef struct {
