@@ -12,6 +12,10 @@ one pass of the seeding pipeline (extension kernel, SMEM filtering, scans,
 suffix-array walk) over one batch of --batch reads that were uploaded to HBM
 before the timed region; results stay in HBM.
 
+--config 4 seeds 2,000,000 reads of uneven length on the same index (70 %
+100-250 bp, 25 % 1-5 kbp, 5 % 5-20 kbp, the long ones with 8 % errors) as
+one batch; every step re-runs the whole pipeline on that resident batch.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -101,12 +105,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3])
-    ap.add_argument("--batch", type=int, default=1000000, help="reads per step")
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
+    ap.add_argument("--batch", type=int, default=0,
+                    help="reads per step (default 1,000,000; configuration 4: its 2,000,000 reads)")
     ap.add_argument("--k", type=int, default=0, help="default: 20 for config 1, 19 otherwise")
     ap.add_argument("--max-occ", type=int, default=-1, help="default: 0 for config 1, 500 otherwise")
     ap.add_argument("--ref-length", type=int, default=0, help="override the reference length")
-    ap.add_argument("--cpu-sample", type=int, default=1000000, help="reads timed on the CPU")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="reads timed on the CPU (default 1,000,000; configuration 4: 20,000)")
+    ap.add_argument("--seg-tasks", type=int, default=0, help="task ends per work item of long reads")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
@@ -127,7 +134,12 @@ def main():
     rp, qp = g.preset(args.config)
     if args.ref_length:
         rp.length = args.ref_length
-    n_batches = args.warmup + args.steps
+    reuse = args.config == 4  # one resident batch, re-run every step
+    if not args.batch:
+        args.batch = qp.n_reads if reuse else 1000000
+    if not args.cpu_sample:
+        args.cpu_sample = 20000 if args.config == 4 else 1000000
+    n_batches = 1 if reuse else args.warmup + args.steps
     qp.n_reads = n_batches * args.batch
 
     log("generating %.2f Gbp reference (config %d)" % (rp.length / 1e9, args.config))
@@ -155,8 +167,10 @@ def main():
         base = int(offs[lo])
         end = int(offs[hi - 1] + lens[hi - 1])
         arrays = (buf[base:end], offs[lo:hi] - np.uint64(base), lens[lo:hi])
-        batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ)))
-    log("%d batches of %d reads resident in HBM" % (n_batches, args.batch))
+        batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks)))
+    log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (n_batches, args.batch, len(buf) / 1e9))
+    if reuse:
+        batches = batches * (args.warmup + args.steps)
 
     elapsed, stats = timed_steps(batches, args.warmup, args.kernel)
     reads = args.steps * args.batch
@@ -179,6 +193,7 @@ def main():
     gpu_out = first_batch.fetch()
     for _, bt in batches:
         bt.close()
+    del batches
 
     # random-access ceiling of this GPU (16-byte dependent requests, 8 GiB)
     rr = g.roofline_random(8 << 30, line_bytes=16, mode=1, inflight=1, grid=2048, iters=512, reps=3)
@@ -231,9 +246,11 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "config %d: SMEM -k %d, %d bp reads, %.2f Gbp synthetic reference, "
+        "config": {"workload": "config %d: SMEM -k %d, %s reads, %.2f Gbp synthetic reference, "
                                "positions for intervals <= %d, batched persistent kernel"
-                               % (args.config, k, int(lens[0]), rp.length / 1e9, max_occ),
+                               % (args.config, k, ("%d bp" % int(lens[0])) if lens.min() == lens.max() else
+                                  "%d-%d bp (mean %.0f)" % (int(lens.min()), int(lens.max()), float(lens.mean())),
+                                  rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
                    "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7,
                    "kernel": "batched-%d" % args.kernel},
