@@ -63,7 +63,12 @@ class IndexInfo(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
                 ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("seg_tasks", C.c_uint32),
-                ("reserved", C.c_int * 2)]
+                ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("reserved", C.c_int)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("opts", Opts), ("seconds", C.c_double), ("default_seconds", C.c_double),
+                ("n_tried", C.c_int), ("pad", C.c_int)]
 
 
 class SeedRec(C.Structure):
@@ -266,14 +271,23 @@ def pack_reads(reads):
     return np.frombuffer(b"".join(reads) or b"\0", dtype=np.uint8), offs, lens
 
 
+def make_opts(k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0, wg=0, walk_rows=0):
+    return Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks, wg, walk_rows, 0)
+
+
+def _arrays(arrays):
+    buf, offs, lens = arrays
+    return (np.ascontiguousarray(buf, dtype=np.uint8), np.ascontiguousarray(offs, dtype=np.uint64),
+            np.ascontiguousarray(lens, dtype=np.uint32))
+
+
 def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
-         text_fd=None, seg_tasks=0):
+         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, opts=None):
     """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
-    buf, offs, lens = arrays if arrays is not None else pack_reads(reads)
-    buf = np.ascontiguousarray(buf, dtype=np.uint8)
-    offs = np.ascontiguousarray(offs, dtype=np.uint64)
-    lens = np.ascontiguousarray(lens, dtype=np.uint32)
-    o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks)
+    buf, offs, lens = _arrays(arrays if arrays is not None else pack_reads(reads))
+    o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
+                                                walk_rows)
+    smem = bool(o.smem)
     res = Result()
     _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                                     _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32), len(lens),
@@ -302,31 +316,57 @@ def _bind_batch(L):
     L.gs_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(BatchStats)]
     L.gs_batch_fetch.argtypes = [C.c_void_p, C.POINTER(Result)]
     L.gs_batch_destroy.argtypes = [C.c_void_p]
+    L.gs_batch_capture.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint32)]
+    L.gs_batch_replay.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
+    L.gs_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                              C.c_uint64, C.POINTER(Opts), C.c_int, C.POINTER(TuneResult)]
     L._batch_bound = True
+
+
+def autotune(index, arrays, reps=3, opts=None, **kw):
+    """Tuned Opts for a sample of reads (gs_autotune) and the TuneResult."""
+    L = lib()
+    _bind_batch(L)
+    buf, offs, lens = _arrays(arrays)
+    o = opts if opts is not None else make_opts(**kw)
+    out = TuneResult()
+    _check("gs_autotune", L.gs_autotune(index.h, C.cast(buf.ctypes.data, C.c_char_p), _ptr(offs, C.c_uint64),
+                                        _ptr(lens, C.c_uint32), len(lens), C.byref(o), reps, C.byref(out)))
+    return out.opts, out
 
 
 class Batch:
     """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
 
-    def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0):
+    def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0,
+                 wg=0, walk_rows=0, opts=None):
         L = lib()
         _bind_batch(L)
-        buf, offs, lens = arrays
-        self._keep = (np.ascontiguousarray(buf, dtype=np.uint8),
-                      np.ascontiguousarray(offs, dtype=np.uint64),
-                      np.ascontiguousarray(lens, dtype=np.uint32))
-        buf, offs, lens = self._keep
-        self.smem = smem
-        o = Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks)
+        buf, offs, lens = self._keep = _arrays(arrays)
+        o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
+                                                    walk_rows)
+        self.smem = bool(o.smem)
         self.h = C.c_void_p()
         _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                                                     _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32),
                                                     len(lens), C.byref(o), C.byref(self.h)))
-        self._keep = None  # the device holds its own copy now
 
     def run(self, kernel=-1):
         s = BatchStats()
         _check("gs_batch_run", lib().gs_batch_run(self.h, kernel, C.byref(s)))
+        return s
+
+    def capture(self, kernel=-1):
+        """Captures the whole pipeline (upload ... download) as a HIP graph;
+        needs a run() first (it fixes the output sizes)."""
+        buf, offs, lens = self._keep
+        _check("gs_batch_capture", lib().gs_batch_capture(self.h, kernel, C.cast(buf.ctypes.data, C.c_char_p),
+                                                          _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32)))
+
+    def replay(self):
+        s = BatchStats()
+        _check("gs_batch_replay", lib().gs_batch_replay(self.h, C.byref(s)))
         return s
 
     def fetch(self):

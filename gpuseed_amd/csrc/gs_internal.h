@@ -78,11 +78,17 @@ __host__ __device__ inline uint32_t n_segments(uint32_t L, uint32_t k, uint32_t 
 
 int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t* lens, const uint64_t* code_off,
                       uint32_t n_reads, const uint32_t* long_reads, uint32_t n_long, uint4* codes, hipStream_t st);
-// variant 0: 8-bit size records, 1: 16-bit size records
-int launch_persistent(const PersistParams& P, int variant, hipStream_t st);
+// variant 0: 8-bit size records, 1: 16-bit size records; wg 64, 128 or 256
+// (0 = 128); grid from persistent_grid (called outside any graph capture)
+int persistent_grid(int variant, int wg, unsigned* grid);
+int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st);
 int launch_join(const JoinParams& J, hipStream_t st);
 
 }  // namespace gs
+
+// Tuning fields of an existing batch (the autotuner's knobs that need no new
+// work items): kernel variant, workgroup size, suffix-array walk rows.
+int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows);
 
 // ceil(2^64 / d) for d >= 2 (exact quotient for every 32-bit numerator), 0 for d == 1
 uint64_t gs_reciprocal64(uint32_t d);

@@ -371,7 +371,7 @@ __global__ void zero_words(uint32_t* __restrict__ p, uint32_t n) {
 }
 
 template <typename AT, int LMAX, int BS>
-int launch_persistent_t(const gs::PersistParams& P, hipStream_t st) {
+int grid_t(unsigned* grid) {
   int dev = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return GS_EHIP;
   hipDeviceProp_t prop;
@@ -379,7 +379,12 @@ int launch_persistent_t(const gs::PersistParams& P, hipStream_t st) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, seed_persistent<AT, LMAX, BS>, BS, 0) != hipSuccess ||
       per_cu < 1)
     return GS_EHIP;
-  const unsigned grid = (unsigned)(per_cu * prop.multiProcessorCount);
+  *grid = (unsigned)(per_cu * prop.multiProcessorCount);
+  return GS_OK;
+}
+
+template <typename AT, int LMAX, int BS>
+int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
   hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, P);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
@@ -407,14 +412,33 @@ int launch_join(const JoinParams& J, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
-int launch_persistent(const PersistParams& P, int variant, hipStream_t st) {
+// Workgroups that fill every CU at the kernel's occupancy (LDS slots bound it).
+int persistent_grid(int variant, int wg, unsigned* grid) {
+  constexpr int L = kPersistLmax;
+  switch (variant * 1000 + (wg ? wg : 128)) {
+    case 64: return grid_t<uint8_t, L, 64>(grid);
+    case 128: return grid_t<uint8_t, L, 128>(grid);
+    case 256: return grid_t<uint8_t, L, 256>(grid);
+    case 1064: return grid_t<uint16_t, L, 64>(grid);
+    case 1128: return grid_t<uint16_t, L, 128>(grid);
+    case 1256: return grid_t<uint16_t, L, 256>(grid);
+    default: return GS_EINVAL;
+  }
+}
+
+int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st) {
+  if (!grid) return GS_EINVAL;
   hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, P.queue, 1u);
   if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 4u);
-  switch (variant) {
-    case 1:
-      return launch_persistent_t<uint16_t, 160, 128>(P, st);
-    default:
-      return launch_persistent_t<uint8_t, 160, 128>(P, st);
+  constexpr int L = kPersistLmax;
+  switch (variant * 1000 + (wg ? wg : 128)) {
+    case 64: return launch_t<uint8_t, L, 64>(P, grid, st);
+    case 128: return launch_t<uint8_t, L, 128>(P, grid, st);
+    case 256: return launch_t<uint8_t, L, 256>(P, grid, st);
+    case 1064: return launch_t<uint16_t, L, 64>(P, grid, st);
+    case 1128: return launch_t<uint16_t, L, 128>(P, grid, st);
+    case 1256: return launch_t<uint16_t, L, 256>(P, grid, st);
+    default: return GS_EINVAL;
   }
 }
 

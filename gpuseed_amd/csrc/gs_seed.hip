@@ -44,7 +44,7 @@ struct SeedParams {
   const uint64_t* offs;     // per read
   const uint32_t* lens;     // per read
   const uint64_t* task_off; // per read: sum of tasks of earlier reads
-  uint32_t n_reads, k, smem, min_len;  // min_len > 0: only reads longer than min_len
+  uint32_t n_reads, k, smem, pad;
   uint4* cand;              // per read: 2*task_off .. : forward region then reverse region
   uint32_t* nF;
   uint32_t* nR;
@@ -72,9 +72,8 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
   const uint32_t r = gid >> 1;
   const uint32_t strand = gid & 1;
   uint8_t* slot = lds + (threadIdx.x >> 1) * PAIR;
-  const uint32_t L0 = r < P.n_reads ? P.lens[r] : 0;
-  const bool live = r < P.n_reads && (P.min_len == 0 || L0 > P.min_len);
-  const uint32_t L = live ? L0 : 0;
+  const bool live = r < P.n_reads;
+  const uint32_t L = live ? P.lens[r] : 0;
   const uint64_t off = live ? P.offs[r] : 0;
   const bool in_lds = L <= (uint32_t)LMAX;
 
@@ -500,30 +499,46 @@ __global__ void __launch_bounds__(kScanBlock) scan_apply(const uint32_t* __restr
 }
 
 // ---- suffix-array walk (locate_seeds_gpu, 668-684) -------------------------
+// R rows per thread, walked together so that their dependent lookups overlap;
+// the rows of a thread are R grid-strides apart, so every load of a round is
+// coalesced across the wave.
+template <int R>
 __global__ void __launch_bounds__(256) locate_rows(const DevIndex X, uint32_t* __restrict__ rows,
                                                    uint64_t n) {
+  const uint64_t T = (uint64_t)gridDim.x * 256;
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-  if (t >= n) return;
-  uint32_t row = rows[t], itr = 0;
-  // Rows past the last one come only from the reference's batch-wide MEM row
-  // skip (seed_gen.cu:637/645) and have no position; a walk on a consistent
-  // index reaches a sampled row within seq_len + 1 steps.
-  if (row > X.seq_len) {
-    rows[t] = 0xFFFFFFFFu;
-    return;
+  uint32_t row[R], itr[R];
+  uint32_t live = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const uint64_t i = t + j * T;
+    row[j] = 0;
+    itr[j] = 0;
+    if (i >= n) continue;
+    row[j] = rows[i];
+    // Rows past the last one come only from the reference's batch-wide MEM
+    // row skip (seed_gen.cu:637/645) and have no position.
+    if (row[j] > X.seq_len)
+      rows[i] = 0xFFFFFFFFu;
+    else
+      live |= 1u << j;
   }
-  while (true) {
-    const uint32_t q = gs::div_sa(X, row);
-    if (row == q * X.sa_intv) {
-      rows[t] = X.sa[q] + itr;
-      break;
+  while (live) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (!(live & (1u << j))) continue;
+      const uint32_t q = gs::div_sa(X, row[j]);
+      if (row[j] == q * X.sa_intv) {
+        rows[t + j * T] = X.sa[q] + itr[j];
+        live &= ~(1u << j);
+      } else if (itr[j] > X.seq_len) {  // a walk on a consistent index ends within seq_len + 1 steps
+        rows[t + j * T] = 0xFFFFFFFFu;
+        live &= ~(1u << j);
+      } else {
+        ++itr[j];
+        row[j] = gs::lf_walk(X, row[j]);
+      }
     }
-    if (itr > X.seq_len) {
-      rows[t] = 0xFFFFFFFFu;
-      break;
-    }
-    ++itr;
-    row = gs::lf_walk(X, row);
   }
 }
 
@@ -587,10 +602,28 @@ struct gs_batch {
   bool ran = false;
   hipEvent_t ev[7] = {};
   double t_h2d = 0;
+  uint64_t* h_tot = nullptr;  // pinned: totals of the count stage
+  // captured pipeline (gs_batch_capture) and its pinned staging
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_fresh = false;
+  uint64_t graph_seeds = 0, graph_pos = 0;
+  char* hs_bases = nullptr;
+  uint64_t* hs_offs = nullptr;
+  uint32_t* hs_lens = nullptr;
+  uint32_t *hr_ns = nullptr, *hr_np = nullptr, *hr_rows = nullptr;
+  uint64_t *hr_soff = nullptr, *hr_poff = nullptr;
+  gs_seed_rec* hr_seeds = nullptr;
 };
+
+static void free_graph(gs_batch* B);
+static void free_host_results(gs_batch* B);
 
 static void destroy_batch(gs_batch* B) {
   if (!B) return;
+  free_graph(B);
+  free_host_results(B);
+  if (B->h_tot) (void)hipHostFree(B->h_tot);
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
   delete B;
@@ -603,6 +636,9 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
   *out = nullptr;
   if (opts->k < gs::kPrecalc + 1 || opts->k > 1 << 20) return GS_EINVAL;
   if (opts->seg_tasks > gs::kSegTasksMax) return GS_EINVAL;
+  if (opts->wg != 0 && opts->wg != 64 && opts->wg != 128 && opts->wg != 256) return GS_EINVAL;
+  if (opts->walk_rows != 0 && opts->walk_rows != 1 && opts->walk_rows != 2 && opts->walk_rows != 4) return GS_EINVAL;
+  if (opts->kernel < 0 || opts->kernel > 2) return GS_EINVAL;
   if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   gs_batch* B = new gs_batch();
@@ -670,6 +706,10 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
       destroy_batch(B);
       return GS_EHIP;
     }
+  if (hipHostMalloc((void**)&B->h_tot, 16, hipHostMallocDefault) != hipSuccess) {
+    destroy_batch(B);
+    return GS_ENOMEM;
+  }
   if (B->bases.ensure(B->span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
       B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
       B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
@@ -708,35 +748,35 @@ extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint
   return GS_OK;
 }
 
-// Runs the whole device pipeline on an uploaded batch; results stay in HBM.
-extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_stats* stats) {
-  if (!B) return GS_EINVAL;
+// ---- the device pipeline of a batch -----------------------------------------
+// Three stages on the index's stream: extension (pack, persistent kernel,
+// segment join; or the plain kernel), counts (per-read counts, scans, totals
+// to pinned host memory), records (seed records, SA rows, SA walk).  An eager
+// run waits for the totals between counts and records to size the outputs; a
+// captured graph replays all three with the sizes of the run before it.
+
+static void fill_params(gs_batch* B, SeedParams& S, PostParams& Q) {
   gs_index* X = B->X;
-  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  hipStream_t st = X->stream;
-  const uint64_t n_reads = B->n_reads;
-  SeedParams S;
   S.X = X->dev();
   S.bases = B->bases.as<uint8_t>();
   S.offs = B->offs.as<uint64_t>();
   S.lens = B->lens.as<uint32_t>();
   S.task_off = B->toff.as<uint64_t>();
-  S.n_reads = (uint32_t)n_reads;
+  S.n_reads = (uint32_t)B->n_reads;
   S.k = (uint32_t)B->opts.k;
   S.smem = B->opts.smem ? 1 : 0;
+  S.pad = 0;
   S.cand = B->cand.as<uint4>();
   S.nF = B->nF.as<uint32_t>();
   S.nR = B->nR.as<uint32_t>();
   S.scratch = B->scratch.p;
-  S.min_len = 0;
-  PostParams Q;
   Q.cand = S.cand;
   Q.nF = S.nF;
   Q.nR = S.nR;
   Q.task_off = S.task_off;
   Q.lens = S.lens;
   Q.batch = B->batch.as<uint32_t>();
-  Q.n_reads = (uint32_t)n_reads;
+  Q.n_reads = S.n_reads;
   Q.k = S.k;
   Q.smem = S.smem;
   Q.max_occ = B->opts.max_occ;
@@ -744,88 +784,131 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
   Q.npos = B->np.as<uint32_t>();
   Q.seed_off = B->soff.as<uint64_t>();
   Q.pos_off = B->poff.as<uint64_t>();
-  GS_HIP(hipEventRecord(B->ev[0], st));
-  const int kernel = kernel_override >= 0 ? kernel_override : B->opts.kernel;
-  if (n_reads && kernel >= 1) {
-    // batched form: pack codes, persistent kernel for reads that fit its LDS
-    // slots, the plain kernel for the longer ones
-    gs::PersistParams PP;
-    PP.X = S.X;
-    PP.codes = B->codes.as<uint4>();
-    PP.code_off = B->code_off.as<uint64_t>();
-    PP.lens = S.lens;
-    PP.task_off = S.task_off;
-    PP.items = B->items.as<uint64_t>();
-    PP.n_items = B->n_items;
-    PP.n_reads = S.n_reads;
-    PP.k = S.k;
-    PP.smem = S.smem;
-    PP.seg_tasks = B->seg_tasks;
-    PP.cand = S.cand;
-    PP.nF = S.nF;
-    PP.nR = S.nR;
-    PP.seg_meta = B->seg_meta.as<uint4>();
-    PP.queue = B->queue.as<uint32_t>();
-    PP.counters = B->counters.as<uint64_t>();
-    gs::JoinParams J;
-    J.long_reads = B->long_reads.as<uint32_t>();
-    J.item0 = B->item0.as<uint64_t>();
-    J.lens = S.lens;
-    J.task_off = S.task_off;
-    J.seg_meta = PP.seg_meta;
-    J.cand = S.cand;
-    J.nF = S.nF;
-    J.nR = S.nR;
-    J.n_long = B->n_long;
-    J.k = S.k;
-    J.smem = S.smem;
-    J.seg_tasks = B->seg_tasks;
-    int rc0 = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
-                                    B->codes.as<uint4>(), st);
-    if (!rc0) rc0 = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, st);
-    if (!rc0) rc0 = gs::launch_join(J, st);
-    if (rc0) return rc0;
-  } else if (n_reads) {
+  Q.seeds = B->seeds.as<gs_seed_rec>();
+  Q.rows = B->rows.as<uint32_t>();
+}
+
+static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsigned pgrid, hipStream_t st) {
+  if (!B->n_reads) return GS_OK;
+  if (kernel == 0) {
     if (B->maxlen <= 160)
       launch_seed<uint16_t, 160>(S, st);
     else
       launch_seed<uint16_t, 256>(S, st);
+    return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
   }
-  GS_HIP(hipEventRecord(B->ev[1], st));
-  uint64_t* tot = nullptr;
-  GS_HIP(hipHostMalloc((void**)&tot, 16, hipHostMallocDefault));
-  tot[0] = tot[1] = 0;
-  int rc = GS_OK;
-  do {
-    if (n_reads) {
-      hipLaunchKernelGGL(finalize_reads<false>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
-      scan_counts(Q.nseeds, n_reads, B->soff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
-      scan_counts(Q.npos, n_reads, B->poff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>() + 1, st);
-      if (hipMemcpyAsync(tot, B->tot.p, 16, hipMemcpyDeviceToHost, st) != hipSuccess) { rc = GS_EHIP; break; }
-    } else {
-      const uint64_t zero[2] = {0, 0};
-      if (hipMemcpyAsync(B->soff.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-          hipMemcpyAsync(B->poff.p, zero, 8, hipMemcpyHostToDevice, st) != hipSuccess) { rc = GS_EHIP; break; }
-    }
-    if (hipEventRecord(B->ev[2], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
-    B->n_seeds = tot[0];
-    B->n_pos = tot[1];
-    if (B->seeds.ensure(sizeof(gs_seed_rec) * (B->n_seeds + 1)) || B->rows.ensure(4 * (B->n_pos + 1))) {
-      rc = GS_ENOMEM;
-      break;
-    }
-    Q.seeds = B->seeds.as<gs_seed_rec>();
-    Q.rows = B->rows.as<uint32_t>();
-    if (hipEventRecord(B->ev[3], st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (n_reads) hipLaunchKernelGGL(finalize_reads<true>, dim3(grid_for(n_reads, 256)), dim3(256), 0, st, Q);
-    if (hipEventRecord(B->ev[4], st) != hipSuccess) { rc = GS_EHIP; break; }
-    if (B->n_pos)
-      hipLaunchKernelGGL(locate_rows, dim3(grid_for(B->n_pos, 256)), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
-    if (hipEventRecord(B->ev[5], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) { rc = GS_EHIP; break; }
-  } while (0);
-  (void)hipHostFree(tot);
+  gs::PersistParams PP;
+  PP.X = S.X;
+  PP.codes = B->codes.as<uint4>();
+  PP.code_off = B->code_off.as<uint64_t>();
+  PP.lens = S.lens;
+  PP.task_off = S.task_off;
+  PP.items = B->items.as<uint64_t>();
+  PP.n_items = B->n_items;
+  PP.n_reads = S.n_reads;
+  PP.k = S.k;
+  PP.smem = S.smem;
+  PP.seg_tasks = B->seg_tasks;
+  PP.cand = S.cand;
+  PP.nF = S.nF;
+  PP.nR = S.nR;
+  PP.seg_meta = B->seg_meta.as<uint4>();
+  PP.queue = B->queue.as<uint32_t>();
+  PP.counters = B->counters.as<uint64_t>();
+  gs::JoinParams J;
+  J.long_reads = B->long_reads.as<uint32_t>();
+  J.item0 = B->item0.as<uint64_t>();
+  J.lens = S.lens;
+  J.task_off = S.task_off;
+  J.seg_meta = PP.seg_meta;
+  J.cand = S.cand;
+  J.nF = S.nF;
+  J.nR = S.nR;
+  J.n_long = B->n_long;
+  J.k = S.k;
+  J.smem = S.smem;
+  J.seg_tasks = B->seg_tasks;
+  int rc = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
+                                 B->codes.as<uint4>(), st);
+  if (!rc) rc = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, B->opts.wg, pgrid, st);
+  if (!rc) rc = gs::launch_join(J, st);
+  return rc;
+}
+
+static int enqueue_counts(gs_batch* B, const PostParams& Q, hipStream_t st) {
+  const uint64_t n = B->n_reads;
+  if (n) {
+    hipLaunchKernelGGL(finalize_reads<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, Q);
+    scan_counts(Q.nseeds, n, B->soff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
+    scan_counts(Q.npos, n, B->poff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>() + 1, st);
+    GS_HIP(hipMemcpyAsync(B->h_tot, B->tot.p, 16, hipMemcpyDeviceToHost, st));
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+static int enqueue_write(gs_batch* B, const PostParams& Q, hipStream_t st) {
+  if (B->n_reads) hipLaunchKernelGGL(finalize_reads<true>, dim3(grid_for(B->n_reads, 256)), dim3(256), 0, st, Q);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+static int enqueue_walk(gs_batch* B, const SeedParams& S, const PostParams& Q, hipStream_t st) {
+  if (B->n_pos) {
+    const unsigned R = B->opts.walk_rows ? B->opts.walk_rows : 1u;
+    const unsigned grid = grid_for((B->n_pos + R - 1) / R, 256);
+    if (R == 4)
+      hipLaunchKernelGGL(locate_rows<4>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+    else if (R == 2)
+      hipLaunchKernelGGL(locate_rows<2>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+    else
+      hipLaunchKernelGGL(locate_rows<1>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+  }
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+}
+
+static int persistent_grid_for(gs_batch* B, int kernel, unsigned* grid) {
+  *grid = 0;
+  if (kernel == 0 || !B->n_reads) return GS_OK;
+  return gs::persistent_grid(kernel == 2 ? 1 : 0, B->opts.wg, grid);
+}
+
+// Runs the whole device pipeline on an uploaded batch; results stay in HBM.
+extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_stats* stats) {
+  if (!B) return GS_EINVAL;
+  gs_index* X = B->X;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  hipStream_t st = X->stream;
+  const int kernel = kernel_override >= 0 ? kernel_override : B->opts.kernel;
+  if (kernel < 0 || kernel > 2) return GS_EINVAL;
+  unsigned pgrid = 0;
+  int rc = persistent_grid_for(B, kernel, &pgrid);
   if (rc) return rc;
+  SeedParams S;
+  PostParams Q;
+  fill_params(B, S, Q);
+  B->h_tot[0] = B->h_tot[1] = 0;
+  GS_HIP(hipEventRecord(B->ev[0], st));
+  rc = enqueue_extension(B, S, kernel, pgrid, st);
+  if (rc) return rc;
+  GS_HIP(hipEventRecord(B->ev[1], st));
+  rc = enqueue_counts(B, Q, st);
+  if (rc) return rc;
+  GS_HIP(hipEventRecord(B->ev[2], st));
+  GS_HIP(hipStreamSynchronize(st));
+  B->n_seeds = B->h_tot[0];
+  B->n_pos = B->h_tot[1];
+  const void *seeds0 = B->seeds.p, *rows0 = B->rows.p;
+  if (B->seeds.ensure(sizeof(gs_seed_rec) * (B->n_seeds + 1)) || B->rows.ensure(4 * (B->n_pos + 1)))
+    return GS_ENOMEM;
+  if (B->graph && (B->seeds.p != seeds0 || B->rows.p != rows0)) free_graph(B);  // it wrote the old buffers
+  fill_params(B, S, Q);
+  GS_HIP(hipEventRecord(B->ev[3], st));
+  if ((rc = enqueue_write(B, Q, st))) return rc;
+  GS_HIP(hipEventRecord(B->ev[4], st));
+  if ((rc = enqueue_walk(B, S, Q, st))) return rc;
+  GS_HIP(hipEventRecord(B->ev[5], st));
+  GS_HIP(hipStreamSynchronize(st));
   B->ran = true;
+  B->graph_fresh = false;
   if (stats) {
     float ms[5] = {0, 0, 0, 0, 0};
     (void)hipEventElapsedTime(&ms[0], B->ev[0], B->ev[1]);
@@ -834,22 +917,145 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
     (void)hipEventElapsedTime(&ms[3], B->ev[4], B->ev[5]);
     (void)hipEventElapsedTime(&ms[4], B->ev[0], B->ev[5]);
     memset(stats, 0, sizeof(*stats));
+    stats->t_h2d = B->t_h2d;
     stats->t_seed = ms[0] * 1e-3;
     stats->t_count = ms[1] * 1e-3;
     stats->t_write = ms[2] * 1e-3;
     stats->t_locate = ms[3] * 1e-3;
     stats->t_total = ms[4] * 1e-3;
-    stats->t_h2d = B->t_h2d;
-    stats->n_reads = n_reads;
+    stats->n_reads = B->n_reads;
     stats->n_seeds = B->n_seeds;
     stats->n_pos = B->n_pos;
-    if (kernel >= 1 && n_reads) {
+    if (kernel >= 1) {
       uint64_t c[2] = {0, 0};
       if (hipMemcpy(c, B->counters.p, 16, hipMemcpyDeviceToHost) == hipSuccess) {
         stats->n_table_reads = c[0];
         stats->n_occ_reads = c[1];
       }
     }
+  }
+  return GS_OK;
+}
+
+static void free_graph(gs_batch* B) {
+  if (B->gexec) (void)hipGraphExecDestroy(B->gexec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->gexec = nullptr;
+  B->graph = nullptr;
+  B->graph_fresh = false;
+}
+
+static void free_host_results(gs_batch* B) {
+  for (void* p : {(void*)B->hr_ns, (void*)B->hr_np, (void*)B->hr_soff, (void*)B->hr_poff, (void*)B->hr_seeds,
+                  (void*)B->hr_rows, (void*)B->hs_bases, (void*)B->hs_offs, (void*)B->hs_lens})
+    if (p) (void)hipHostFree(p);
+  B->hr_ns = B->hr_np = nullptr;
+  B->hr_soff = B->hr_poff = nullptr;
+  B->hr_seeds = nullptr;
+  B->hr_rows = nullptr;
+  B->hs_bases = nullptr;
+  B->hs_offs = nullptr;
+  B->hs_lens = nullptr;
+}
+
+// Captures upload -> extension -> counts -> records -> download as one graph.
+extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel, const char* bases, const uint64_t* offsets,
+                                       const uint32_t* lens) {
+  if (!B || !B->ran) return GS_EINVAL;
+  const int kern = kernel >= 0 ? kernel : B->opts.kernel;
+  if (kern < 0 || kern > 2) return GS_EINVAL;
+  if (B->n_reads && (!bases || !offsets || !lens)) return GS_EINVAL;
+  gs_index* X = B->X;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  hipStream_t st = X->stream;
+  free_graph(B);
+  free_host_results(B);
+  const uint64_t n = B->n_reads, nr1 = n + 1;
+  // pinned staging of the inputs and of the results, sized by the last run
+  if (hipHostMalloc((void**)&B->hs_bases, B->span + 1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hs_offs, 8 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hs_lens, 4 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hr_ns, 4 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hr_np, 4 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hr_soff, 8 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hr_poff, 8 * nr1, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&B->hr_seeds, sizeof(gs_seed_rec) * (B->n_seeds + 1), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&B->hr_rows, 4 * (B->n_pos + 1), hipHostMallocDefault) != hipSuccess) {
+    free_host_results(B);
+    return GS_ENOMEM;
+  }
+  if (B->span) memcpy(B->hs_bases, bases, B->span);
+  if (n) {
+    memcpy(B->hs_offs, offsets, 8 * n);
+    memcpy(B->hs_lens, lens, 4 * n);
+  }
+  unsigned pgrid = 0;
+  int rc = persistent_grid_for(B, kern, &pgrid);
+  if (rc) return rc;
+  SeedParams S;
+  PostParams Q;
+  fill_params(B, S, Q);
+  GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+  do {
+    if ((B->span && hipMemcpyAsync(B->bases.p, B->hs_bases, B->span, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (n && (hipMemcpyAsync(B->offs.p, B->hs_offs, 8 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(B->lens.p, B->hs_lens, 4 * n, hipMemcpyHostToDevice, st) != hipSuccess))) {
+      rc = GS_EHIP;
+      break;
+    }
+    if ((rc = enqueue_extension(B, S, kern, pgrid, st))) break;
+    if ((rc = enqueue_counts(B, Q, st))) break;
+    if ((rc = enqueue_write(B, Q, st))) break;
+    if ((rc = enqueue_walk(B, S, Q, st))) break;
+    if ((n && (hipMemcpyAsync(B->hr_ns, B->ns.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+               hipMemcpyAsync(B->hr_np, B->np.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+               hipMemcpyAsync(B->hr_soff, B->soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+               hipMemcpyAsync(B->hr_poff, B->poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+        (B->n_seeds && hipMemcpyAsync(B->hr_seeds, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds,
+                                      hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        (B->n_pos && hipMemcpyAsync(B->hr_rows, B->rows.p, 4 * B->n_pos, hipMemcpyDeviceToHost, st) != hipSuccess))
+      rc = GS_EHIP;
+  } while (0);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  if (rc || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : GS_EHIP;
+  }
+  B->graph = g;
+  if (hipGraphInstantiate(&B->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+    free_graph(B);
+    return GS_EHIP;
+  }
+  B->graph_seeds = B->n_seeds;
+  B->graph_pos = B->n_pos;
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_batch_replay(gs_batch* B, gs_batch_stats* stats) {
+  if (!B || !B->gexec) return GS_EINVAL;
+  gs_index* X = B->X;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  hipStream_t st = X->stream;
+  B->h_tot[0] = B->h_tot[1] = 0;
+  GS_HIP(hipEventRecord(B->ev[0], st));
+  GS_HIP(hipGraphLaunch(B->gexec, st));
+  GS_HIP(hipEventRecord(B->ev[5], st));
+  GS_HIP(hipStreamSynchronize(st));
+  // the totals the graph copied back must be the sizes it was captured with
+  if (B->n_reads && (B->h_tot[0] != B->graph_seeds || B->h_tot[1] != B->graph_pos)) return GS_ERANGE;
+  B->n_seeds = B->graph_seeds;
+  B->n_pos = B->graph_pos;
+  B->graph_fresh = true;
+  if (stats) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[5]);
+    memset(stats, 0, sizeof(*stats));
+    stats->t_total = ms * 1e-3;
+    stats->n_reads = B->n_reads;
+    stats->n_seeds = B->n_seeds;
+    stats->n_pos = B->n_pos;
   }
   return GS_OK;
 }
@@ -874,6 +1080,19 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
     gs_result_free(out);
     return GS_ENOMEM;
   }
+  if (B->graph_fresh) {  // the last replay already downloaded everything
+    memcpy(out->read_nseeds, B->hr_ns, 4 * n);
+    memcpy(out->read_npos, B->hr_np, 4 * n);
+    if (n) {
+      memcpy(out->read_seed_off, B->hr_soff, 8 * nr1);
+      memcpy(out->read_pos_off, B->hr_poff, 8 * nr1);
+    } else {
+      out->read_seed_off[0] = out->read_pos_off[0] = 0;
+    }
+    memcpy(out->seeds, B->hr_seeds, sizeof(gs_seed_rec) * B->n_seeds);
+    memcpy(out->pos, B->hr_rows, 4 * B->n_pos);
+    return GS_OK;
+  }
   hipEvent_t e0 = B->ev[5], e1 = B->ev[6];
   int rc = GS_OK;
   if (hipEventRecord(e0, st) != hipSuccess ||
@@ -893,6 +1112,16 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
   out->t_d2h = ms * 1e-3;
+  return GS_OK;
+}
+
+int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows) {
+  if (!B || kernel < 0 || kernel > 2 || (wg != 64 && wg != 128 && wg != 256) ||
+      (walk_rows != 1 && walk_rows != 2 && walk_rows != 4))
+    return GS_EINVAL;
+  B->opts.kernel = kernel;
+  B->opts.wg = (uint16_t)wg;
+  B->opts.walk_rows = (uint16_t)walk_rows;
   return GS_OK;
 }
 

@@ -167,8 +167,11 @@ typedef struct {
                        2: batched persistent kernel, 16-bit size records      */
   uint32_t seg_tasks; /* batched kernels: reads longer than 160 bases are cut
                        into work items of this many task ends, 1..112
-                       (0 = 64); the output does not depend on it             */
-  int reserved[2];
+                       (0 = 64)                                               */
+  uint16_t wg;        /* batched kernels: workgroup size 64, 128 or 256 (0 = 128) */
+  uint16_t walk_rows; /* suffix-array walk: rows per thread 1, 2 or 4 (0 = 1) */
+  int reserved;       /* 0. The output depends on none of the tuning fields
+                       (kernel, seg_tasks, wg, walk_rows)                     */
 } gs_opts;
 
 typedef struct {
@@ -220,6 +223,36 @@ GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* off
 GS_API int gs_batch_run(gs_batch* batch, int kernel, gs_batch_stats* stats); /* kernel -1: as created */
 GS_API int gs_batch_fetch(gs_batch* batch, gs_result* out);
 GS_API int gs_batch_destroy(gs_batch* batch);
+
+/* The whole pipeline as one HIP graph: upload of the reads (from pinned
+ * copies of bases/offsets/lens, which must be the batch's inputs), extension,
+ * counts and scans, records, suffix-array walk and the download of every
+ * result array into pinned host memory.  Output sizes come from the batch's
+ * previous gs_batch_run, so the graph has no host synchronisation inside.
+ * gs_batch_replay launches it and waits; gs_batch_fetch then returns the
+ * downloaded results.  Replays write the same results as gs_batch_run.
+ * No counterpart in the reference, which synchronises after every phase. */
+GS_API int gs_batch_capture(gs_batch* batch, int kernel, const char* bases, const uint64_t* offsets,
+                            const uint32_t* lens);
+GS_API int gs_batch_replay(gs_batch* batch, gs_batch_stats* stats);
+
+/* Autotuner: times the batched pipeline on the caller's reads (a sample of
+ * the workload) over its launch parameters: seg_tasks (if some reads are
+ * longer than 160 bases), size-record width x workgroup size, and suffix-array
+ * walk rows, one after another, each candidate the best of `reps` runs
+ * (0 = 3).  Returns base with the chosen tuning fields.  Every candidate must
+ * give the same seed and position counts (the output does not depend on the
+ * tuning fields).  No counterpart in the reference. */
+typedef struct {
+  gs_opts opts;           /* base with kernel, seg_tasks, wg, walk_rows chosen */
+  double seconds;         /* pipeline time with opts                           */
+  double default_seconds; /* pipeline time with base's fields (0 -> defaults)  */
+  int n_tried;
+  int pad;
+} gs_tune_result;
+
+GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
+                       uint64_t n_reads, const gs_opts* base, int reps, gs_tune_result* out);
 
 /* The -o text of a result (seed_gen.cu:3625-3648) written to file descriptor
  * fd, reads numbered from first_read_no. */
