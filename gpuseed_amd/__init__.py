@@ -66,6 +66,23 @@ class Opts(C.Structure):
                 ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("reserved", C.c_int)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("opts", Opts), ("budget_bytes", C.c_uint64), ("round_bases", C.c_uint64),
+                ("exclusive", C.c_int), ("reserved", C.c_int * 3)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
+                ("n_rounds", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("n_requests", C.c_uint64), ("n_rounds", C.c_uint64), ("n_reads", C.c_uint64),
+                ("n_bases", C.c_uint64), ("busy_seconds", C.c_double), ("device_seconds", C.c_double),
+                ("peak_round_bytes", C.c_uint64), ("elapsed", C.c_double)]
+
+
 class TuneResult(C.Structure):
     _fields_ = [("opts", Opts), ("seconds", C.c_double), ("default_seconds", C.c_double),
                 ("n_tried", C.c_int), ("pad", C.c_int)]
@@ -316,8 +333,7 @@ def _bind_batch(L):
     L.gs_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(BatchStats)]
     L.gs_batch_fetch.argtypes = [C.c_void_p, C.POINTER(Result)]
     L.gs_batch_destroy.argtypes = [C.c_void_p]
-    L.gs_batch_capture.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.POINTER(C.c_uint64),
-                                   C.POINTER(C.c_uint32)]
+    L.gs_batch_capture.argtypes = [C.c_void_p, C.c_int]
     L.gs_batch_replay.argtypes = [C.c_void_p, C.POINTER(BatchStats)]
     L.gs_autotune.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                               C.c_uint64, C.POINTER(Opts), C.c_int, C.POINTER(TuneResult)]
@@ -343,7 +359,7 @@ class Batch:
                  wg=0, walk_rows=0, opts=None):
         L = lib()
         _bind_batch(L)
-        buf, offs, lens = self._keep = _arrays(arrays)
+        buf, offs, lens = _arrays(arrays)
         o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
                                                     walk_rows)
         self.smem = bool(o.smem)
@@ -360,9 +376,7 @@ class Batch:
     def capture(self, kernel=-1):
         """Captures the whole pipeline (upload ... download) as a HIP graph;
         needs a run() first (it fixes the output sizes)."""
-        buf, offs, lens = self._keep
-        _check("gs_batch_capture", lib().gs_batch_capture(self.h, kernel, C.cast(buf.ctypes.data, C.c_char_p),
-                                                          _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32)))
+        _check("gs_batch_capture", lib().gs_batch_capture(self.h, kernel))
 
     def replay(self):
         s = BatchStats()
@@ -381,6 +395,82 @@ class Batch:
         if self.h:
             lib().gs_batch_destroy(self.h)
             self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _bind_server(L):
+    if getattr(L, "_server_bound", False):
+        return
+    L.gs_server_create.argtypes = [C.c_void_p, C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]
+    L.gs_server_submit.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
+                                   C.c_uint64, C.POINTER(C.c_uint64)]
+    L.gs_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(Result), C.POINTER(RequestInfo)]
+    L.gs_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+    L.gs_server_clock.argtypes = [C.c_void_p]
+    L.gs_server_clock.restype = C.c_double
+    L.gs_server_destroy.argtypes = [C.c_void_p]
+    L._server_bound = True
+
+
+class Server:
+    """Continuous-batching seeding server over a resident index (gs_server_*).
+    submit() returns a ticket; wait(ticket) returns (SeedOutput, RequestInfo)."""
+
+    def __init__(self, index, budget_bytes=0, round_bases=0, exclusive=False, opts=None, **kw):
+        L = lib()
+        _bind_server(L)
+        o = ServerOpts()
+        o.opts = opts if opts is not None else make_opts(**kw)
+        o.budget_bytes = budget_bytes
+        o.round_bases = round_bases
+        o.exclusive = 1 if exclusive else 0
+        self.h = C.c_void_p()
+        self._inputs = {}
+        _check("gs_server_create", L.gs_server_create(index.h, C.byref(o), C.byref(self.h)))
+
+    def submit(self, arrays):
+        buf, offs, lens = _arrays(arrays)
+        t = C.c_uint64()
+        _check("gs_server_submit", lib().gs_server_submit(self.h, C.cast(buf.ctypes.data, C.c_char_p),
+                                                          _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32),
+                                                          len(lens), C.byref(t)))
+        self._inputs[t.value] = (buf, offs, lens)  # alive until wait()
+        return t.value
+
+    def wait(self, ticket):
+        res, info = Result(), RequestInfo()
+        try:
+            _check("gs_server_wait", lib().gs_server_wait(self.h, ticket, C.byref(res), C.byref(info)))
+            return SeedOutput(res), info
+        finally:
+            lib().gs_result_free(C.byref(res))
+            self._inputs.pop(ticket, None)
+
+    def clock(self):
+        return lib().gs_server_clock(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        _check("gs_server_get_stats", lib().gs_server_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            rc = lib().gs_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._inputs.clear()
+            _check("gs_server_destroy", rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def __del__(self):
         try:

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+#include <vector>
+
 #include "gpuseed.h"
 #include "gs_device.h"
 
@@ -83,6 +86,32 @@ int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t
 int persistent_grid(int variant, int wg, unsigned* grid);
 int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st);
 int launch_join(const JoinParams& J, hipStream_t st);
+
+}  // namespace gs
+
+namespace gs {
+
+// The CPU side of a batch (see gs_batch_create): built on any thread, then
+// uploaded by batch_load.
+struct BatchPlan {
+  gs_opts opts{};
+  uint32_t seg_tasks = 64, maxlen = 0, next_batch_id = 0;
+  uint64_t n_reads = 0, span = 0, tasks = 0, code_chunks = 0;
+  std::vector<std::pair<const char*, uint64_t>> pieces;  // bytes uploaded back to back
+  std::vector<uint64_t> offs, task_off, code_off, items, item0;
+  std::vector<uint32_t> lens, rbatch, long_reads;
+};
+
+int plan_check(const gs_opts* o);
+void plan_reset(BatchPlan& P, const gs_opts& o);
+// appends reads [r0, r1) of one source as a group of their own
+int plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const uint32_t* lens, uint64_t r0,
+             uint64_t r1);
+void plan_finish(BatchPlan& P);
+uint64_t plan_device_bytes(const BatchPlan& P);  // device buffers the plan needs, outputs excluded
+gs_batch* batch_new(gs_index* X);
+int batch_load(gs_batch* B, const BatchPlan& P);
+void batch_free(gs_batch* B);
 
 }  // namespace gs
 

@@ -544,6 +544,10 @@ __global__ void __launch_bounds__(256) locate_rows(const DevIndex X, uint32_t* _
 
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+__global__ void zero_bytes(uint8_t* __restrict__ p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // ---- device buffers of one gs_seed call -------------------------------------
 struct DevBuf {
   void* p = nullptr;
@@ -553,7 +557,7 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = bytes + bytes / 4 + 256;
+    size_t want = bytes + (bytes / 4 < (256u << 20) ? bytes / 4 : (256u << 20)) + 256;  // headroom for reuse
     if (hipMalloc(&p, want) != hipSuccess) return GS_ENOMEM;
     cap = want;
     return GS_OK;
@@ -629,121 +633,178 @@ static void destroy_batch(gs_batch* B) {
   delete B;
 }
 
+// ---- host plan of a batch -----------------------------------------------------
+// Everything gs_batch_create derives on the CPU: the byte pieces to upload, the
+// offsets rebased onto them, task offsets, reference-batch ids, code chunks and
+// the persistent kernel's work items.  A plan can hold several groups of reads
+// (one per request of a server round); batch ids never span two groups.
+
+int gs::plan_check(const gs_opts* o) {
+  if (!o) return GS_EINVAL;
+  if (o->k < gs::kPrecalc + 1 || o->k > 1 << 20) return GS_EINVAL;
+  if (o->seg_tasks > gs::kSegTasksMax) return GS_EINVAL;
+  if (o->wg != 0 && o->wg != 64 && o->wg != 128 && o->wg != 256) return GS_EINVAL;
+  if (o->walk_rows != 0 && o->walk_rows != 1 && o->walk_rows != 2 && o->walk_rows != 4) return GS_EINVAL;
+  if (o->kernel < 0 || o->kernel > 2) return GS_EINVAL;
+  return GS_OK;
+}
+
+void gs::plan_reset(BatchPlan& P, const gs_opts& o) {
+  P = BatchPlan();
+  P.opts = o;
+  P.seg_tasks = o.seg_tasks ? o.seg_tasks : 64u;
+}
+
+int gs::plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const uint32_t* lens, uint64_t r0,
+                 uint64_t r1) {
+  if (r1 <= r0) return GS_OK;
+  const uint32_t k = (uint32_t)P.opts.k;
+  const uint32_t batch_bases = P.opts.batch_bases ? P.opts.batch_bases : 1000000u;
+  uint64_t lo = UINT64_MAX, hi = 0;
+  for (uint64_t i = r0; i < r1; ++i) {
+    if (offsets[i] < lo) lo = offsets[i];
+    if (offsets[i] + lens[i] > hi) hi = offsets[i] + lens[i];
+  }
+  if (hi < lo) lo = hi = 0;
+  if (P.n_reads + (r1 - r0) > 0xFFFFFFFFull / 2) return GS_ERANGE;
+  const uint64_t base = P.span;
+  if (hi > lo) P.pieces.push_back({bases + lo, hi - lo});
+  P.span += hi - lo;
+  uint64_t acc = 0;
+  uint32_t bid = P.next_batch_id;
+  for (uint64_t i = r0; i < r1; ++i) {
+    const uint32_t L = lens[i];
+    P.offs.push_back(offsets[i] - lo + base);
+    P.lens.push_back(L);
+    P.task_off.push_back(P.tasks);
+    P.tasks += L >= k ? L - k + 1 : 0;
+    P.rbatch.push_back(bid);
+    acc += L;
+    if (acc >= batch_bases) {  // a batch closes after the read that reaches the limit (2955-2991)
+      ++bid;
+      acc = 0;
+    }
+    if (L > P.maxlen) P.maxlen = L;
+    const uint64_t ri = P.n_reads++;
+    P.code_off.push_back(P.code_chunks);
+    P.code_chunks += 2 * ((L + 31) / 32);
+    const uint32_t nseg = gs::n_segments(L, k, P.seg_tasks);
+    if (nseg > 1 || (L > gs::kPersistLmax && L >= k)) {
+      P.long_reads.push_back((uint32_t)ri);
+      P.item0.push_back(P.items.size());
+    }
+    for (uint64_t g = 0; g < nseg; ++g) {
+      P.items.push_back(ri | (g << 33));
+      P.items.push_back(ri | (1ull << 32) | (g << 33));
+    }
+  }
+  P.next_batch_id = bid + 1;  // the next group starts a batch of its own
+  if (P.items.size() > 0xF0000000ull) return GS_ERANGE;
+  return GS_OK;
+}
+
+void gs::plan_finish(BatchPlan& P) {
+  P.task_off.push_back(P.tasks);
+  P.code_off.push_back(P.code_chunks);
+}
+
+uint64_t gs::plan_device_bytes(const BatchPlan& P) {
+  const uint64_t nr1 = P.n_reads + 1;
+  return (P.span + 32) + 16 * (P.code_chunks + 1) + 16 * (2 * P.tasks + 1) + 8 * P.items.size() +
+         (P.long_reads.empty() ? 0 : 16 * P.items.size()) + 12 * P.long_reads.size() + nr1 * (8 * 5 + 4 * 7) +
+         (P.maxlen > 256 ? 4 * (P.span + 16) : 0);
+}
+
+static int batch_events(gs_batch* B) {
+  for (auto& e : B->ev)
+    if (!e && hipEventCreate(&e) != hipSuccess) return GS_EHIP;
+  if (!B->h_tot && hipHostMalloc((void**)&B->h_tot, 16, hipHostMallocDefault) != hipSuccess) return GS_ENOMEM;
+  return GS_OK;
+}
+
+// Uploads a finished plan into B, reusing B's device buffers where they are
+// large enough.  B's results and any captured graph are dropped.
+int gs::batch_load(gs_batch* B, const BatchPlan& P) {
+  gs_index* X = B->X;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  int rc = batch_events(B);
+  if (rc) return rc;
+  free_graph(B);
+  B->opts = P.opts;
+  B->n_reads = P.n_reads;
+  B->span = P.span;
+  B->tasks = P.tasks;
+  B->maxlen = P.maxlen;
+  B->code_chunks = P.code_chunks;
+  B->n_items = P.items.size();
+  B->n_long = (uint32_t)P.long_reads.size();
+  B->seg_tasks = P.seg_tasks;
+  B->n_seeds = B->n_pos = 0;
+  B->ran = false;
+  const uint64_t n = P.n_reads, nr1 = n + 1;
+  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
+      B->items.ensure(8 * (B->n_items + 1)) || B->long_reads.ensure(4 * (B->n_long + 1)) ||
+      B->item0.ensure(8 * (B->n_long + 1)) || B->seg_meta.ensure(B->n_long ? 16 * B->n_items : 16) ||
+      B->bases.ensure(B->span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
+      B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
+      B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
+      B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
+      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
+      B->queue.ensure(16) || B->counters.ensure(32))
+    return GS_ENOMEM;
+  hipStream_t st = X->stream;
+  if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
+  uint64_t at = 0;
+  for (const auto& pc : P.pieces) {
+    if (hipMemcpyAsync(B->bases.as<char>() + at, pc.first, pc.second, hipMemcpyHostToDevice, st) != hipSuccess)
+      return GS_EHIP;
+    at += pc.second;
+  }
+  hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(32), 0, st, B->bases.as<uint8_t>() + B->span, 32u);
+  if (n &&
+      (hipMemcpyAsync(B->offs.p, P.offs.data(), 8 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+       hipMemcpyAsync(B->lens.p, P.lens.data(), 4 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+       hipMemcpyAsync(B->toff.p, P.task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
+       hipMemcpyAsync(B->batch.p, P.rbatch.data(), 4 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+       hipMemcpyAsync(B->code_off.p, P.code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
+       hipMemcpyAsync(B->items.p, P.items.data(), 8 * B->n_items, hipMemcpyHostToDevice, st) != hipSuccess ||
+       (B->n_long &&
+        (hipMemcpyAsync(B->long_reads.p, P.long_reads.data(), 4 * B->n_long, hipMemcpyHostToDevice, st) !=
+             hipSuccess ||
+         hipMemcpyAsync(B->item0.p, P.item0.data(), 8 * B->n_long, hipMemcpyHostToDevice, st) != hipSuccess))))
+    return GS_EHIP;
+  if (hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return GS_EHIP;
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
+  B->t_h2d = ms * 1e-3;
+  return GS_OK;
+}
+
+gs_batch* gs::batch_new(gs_index* X) {
+  gs_batch* B = new gs_batch();
+  B->X = X;
+  return B;
+}
+
+void gs::batch_free(gs_batch* B) { destroy_batch(B); }
+
 extern "C" GS_API int gs_batch_create(gs_index* X, const char* bases, const uint64_t* offsets,
                                       const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
                                       gs_batch** out) {
   if (!X || !opts || !out || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
   *out = nullptr;
-  if (opts->k < gs::kPrecalc + 1 || opts->k > 1 << 20) return GS_EINVAL;
-  if (opts->seg_tasks > gs::kSegTasksMax) return GS_EINVAL;
-  if (opts->wg != 0 && opts->wg != 64 && opts->wg != 128 && opts->wg != 256) return GS_EINVAL;
-  if (opts->walk_rows != 0 && opts->walk_rows != 1 && opts->walk_rows != 2 && opts->walk_rows != 4) return GS_EINVAL;
-  if (opts->kernel < 0 || opts->kernel > 2) return GS_EINVAL;
+  int rc = gs::plan_check(opts);
+  if (rc) return rc;
   if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
-  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  gs_batch* B = new gs_batch();
-  B->X = X;
-  B->opts = *opts;
-  B->n_reads = n_reads;
-  const uint32_t k = (uint32_t)opts->k;
-  const uint32_t batch_bases = opts->batch_bases ? opts->batch_bases : 1000000u;
-  // host-side layout: byte span, task offsets, the reference's batches (2955-2991)
-  std::vector<uint64_t> task_off(n_reads + 1);
-  std::vector<uint32_t> rbatch(n_reads ? n_reads : 1);
-  uint64_t acc = 0;
-  uint32_t bid = 0;
-  for (uint64_t i = 0; i < n_reads; ++i) {
-    const uint64_t end = offsets[i] + lens[i];
-    if (end > B->span) B->span = end;
-    task_off[i] = B->tasks;
-    B->tasks += lens[i] >= k ? lens[i] - k + 1 : 0;
-    rbatch[i] = bid;
-    acc += lens[i];
-    if (acc >= batch_bases) {  // a batch closes after the read that reaches the limit
-      ++bid;
-      acc = 0;
-    }
-    if (lens[i] > B->maxlen) B->maxlen = lens[i];
-  }
-  task_off[n_reads] = B->tasks;
-  const uint64_t nr1 = n_reads + 1;
-  // persistent kernel: 4-bit code chunks of both strands of every read, and
-  // its work items (reads longer than its LDS window become segments)
-  B->seg_tasks = opts->seg_tasks ? opts->seg_tasks : 64u;
-  std::vector<uint64_t> code_off(nr1);
-  std::vector<uint64_t> items;
-  std::vector<uint32_t> long_reads;
-  std::vector<uint64_t> item0;
-  items.reserve(2 * n_reads);
-  for (uint64_t i = 0; i < n_reads; ++i) {
-    code_off[i] = B->code_chunks;
-    B->code_chunks += 2 * ((lens[i] + 31) / 32);
-    const uint32_t nseg = gs::n_segments(lens[i], k, B->seg_tasks);
-    if (nseg > 1 || (lens[i] > gs::kPersistLmax && lens[i] >= k)) {
-      long_reads.push_back((uint32_t)i);
-      item0.push_back(items.size());
-    }
-    for (uint64_t g = 0; g < nseg; ++g) {
-      items.push_back(i | (g << 33));
-      items.push_back(i | (1ull << 32) | (g << 33));
-    }
-  }
-  code_off[n_reads] = B->code_chunks;
-  B->n_items = items.size();
-  B->n_long = (uint32_t)long_reads.size();
-  if (B->n_items > 0xF0000000ull) {
-    destroy_batch(B);
-    return GS_ERANGE;
-  }
-  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
-      B->items.ensure(8 * (B->n_items + 1)) || B->long_reads.ensure(4 * (B->n_long + 1)) ||
-      B->item0.ensure(8 * (B->n_long + 1)) || B->seg_meta.ensure(B->n_long ? 16 * B->n_items : 16)) {
-    destroy_batch(B);
-    return GS_ENOMEM;
-  }
-  for (auto& e : B->ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      destroy_batch(B);
-      return GS_EHIP;
-    }
-  if (hipHostMalloc((void**)&B->h_tot, 16, hipHostMallocDefault) != hipSuccess) {
-    destroy_batch(B);
-    return GS_ENOMEM;
-  }
-  if (B->bases.ensure(B->span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
-      B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
-      B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
-      B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
-      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
-      B->queue.ensure(16) || B->counters.ensure(32)) {
-    destroy_batch(B);
-    return GS_ENOMEM;
-  }
-  hipStream_t st = X->stream;
-  int rc = GS_OK;
-  if (hipEventRecord(B->ev[0], st) != hipSuccess ||
-      (B->span && hipMemcpyAsync(B->bases.p, bases, B->span, hipMemcpyHostToDevice, st) != hipSuccess) ||
-      hipMemsetAsync(B->bases.as<uint8_t>() + B->span, 0, 32, st) != hipSuccess ||
-      (n_reads &&
-       (hipMemcpyAsync(B->offs.p, offsets, 8 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->lens.p, lens, 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->toff.p, task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->batch.p, rbatch.data(), 4 * n_reads, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->code_off.p, code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(B->items.p, items.data(), 8 * B->n_items, hipMemcpyHostToDevice, st) != hipSuccess ||
-        (B->n_long &&
-         (hipMemcpyAsync(B->long_reads.p, long_reads.data(), 4 * B->n_long, hipMemcpyHostToDevice, st) !=
-              hipSuccess ||
-          hipMemcpyAsync(B->item0.p, item0.data(), 8 * B->n_long, hipMemcpyHostToDevice, st) != hipSuccess)))) ||
-      hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-    rc = GS_EHIP;
-  if (rc) {
+  gs::BatchPlan P;
+  gs::plan_reset(P, *opts);
+  if ((rc = gs::plan_add(P, bases, offsets, lens, 0, n_reads))) return rc;
+  gs::plan_finish(P);
+  gs_batch* B = gs::batch_new(X);
+  if ((rc = gs::batch_load(B, P))) {
     destroy_batch(B);
     return rc;
   }
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
-  B->t_h2d = ms * 1e-3;
   *out = B;
   return GS_OK;
 }
@@ -842,6 +903,9 @@ static int enqueue_counts(gs_batch* B, const PostParams& Q, hipStream_t st) {
     scan_counts(Q.nseeds, n, B->soff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
     scan_counts(Q.npos, n, B->poff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>() + 1, st);
     GS_HIP(hipMemcpyAsync(B->h_tot, B->tot.p, 16, hipMemcpyDeviceToHost, st));
+  } else {  // no reads: the offset arrays still hold their closing 0
+    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(32), 0, st, B->soff.as<uint8_t>(), 8u);
+    hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(32), 0, st, B->poff.as<uint8_t>(), 8u);
   }
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
@@ -959,12 +1023,10 @@ static void free_host_results(gs_batch* B) {
 }
 
 // Captures upload -> extension -> counts -> records -> download as one graph.
-extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel, const char* bases, const uint64_t* offsets,
-                                       const uint32_t* lens) {
+extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel) {
   if (!B || !B->ran) return GS_EINVAL;
   const int kern = kernel >= 0 ? kernel : B->opts.kernel;
   if (kern < 0 || kern > 2) return GS_EINVAL;
-  if (B->n_reads && (!bases || !offsets || !lens)) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   hipStream_t st = X->stream;
@@ -985,10 +1047,13 @@ extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel, const char* base
     free_host_results(B);
     return GS_ENOMEM;
   }
-  if (B->span) memcpy(B->hs_bases, bases, B->span);
-  if (n) {
-    memcpy(B->hs_offs, offsets, 8 * n);
-    memcpy(B->hs_lens, lens, 4 * n);
+  // the graph's upload nodes re-send the batch's resident inputs from these
+  if ((B->span && hipMemcpyAsync(B->hs_bases, B->bases.p, B->span, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      (n && (hipMemcpyAsync(B->hs_offs, B->offs.p, 8 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipMemcpyAsync(B->hs_lens, B->lens.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    free_host_results(B);
+    return GS_EHIP;
   }
   unsigned pgrid = 0;
   int rc = persistent_grid_for(B, kern, &pgrid);

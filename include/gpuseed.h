@@ -224,16 +224,16 @@ GS_API int gs_batch_run(gs_batch* batch, int kernel, gs_batch_stats* stats); /* 
 GS_API int gs_batch_fetch(gs_batch* batch, gs_result* out);
 GS_API int gs_batch_destroy(gs_batch* batch);
 
-/* The whole pipeline as one HIP graph: upload of the reads (from pinned
- * copies of bases/offsets/lens, which must be the batch's inputs), extension,
- * counts and scans, records, suffix-array walk and the download of every
- * result array into pinned host memory.  Output sizes come from the batch's
- * previous gs_batch_run, so the graph has no host synchronisation inside.
+/* The whole pipeline as one HIP graph: upload of the reads (from pinned host
+ * copies of the batch's inputs made here), extension, counts and scans,
+ * records, suffix-array walk and the download of every result array into
+ * pinned host memory.  Output sizes come from the batch's previous
+ * gs_batch_run, so the graph has no host synchronisation inside.
  * gs_batch_replay launches it and waits; gs_batch_fetch then returns the
  * downloaded results.  Replays write the same results as gs_batch_run.
- * No counterpart in the reference, which synchronises after every phase. */
-GS_API int gs_batch_capture(gs_batch* batch, int kernel, const char* bases, const uint64_t* offsets,
-                            const uint32_t* lens);
+ * kernel -1: as created.  No counterpart in the reference, which
+ * synchronises after every phase. */
+GS_API int gs_batch_capture(gs_batch* batch, int kernel);
 GS_API int gs_batch_replay(gs_batch* batch, gs_batch_stats* stats);
 
 /* Autotuner: times the batched pipeline on the caller's reads (a sample of
@@ -253,6 +253,55 @@ typedef struct {
 
 GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
                        uint64_t n_reads, const gs_opts* base, int reps, gs_tune_result* out);
+
+/* ---- serving ---------------------------------------------------------------
+ * A server keeps the index resident and seeds requests submitted from any
+ * thread with continuous batching: a planner thread packs the unscheduled
+ * reads of every waiting request into device rounds (each request gets a fair
+ * share of a round, whole 1,000,000-base reference batches at a time, then the
+ * round is filled in arrival order), bounded by round_bases and by half of
+ * budget_bytes of device memory for read and seed buffers; a device thread
+ * runs each round as one pipeline (one persistent-kernel launch over all its
+ * requests) while the planner prepares the next.  Requests arriving during a
+ * round join the next one.  A request's results are exactly those of gs_seed
+ * on the request alone.  The caller's read buffers must stay valid until
+ * gs_server_wait returns for that request.  No counterpart in the reference. */
+typedef struct gs_server gs_server;
+
+typedef struct {
+  gs_opts opts;          /* seeding options of every request; kernel 0 means 1
+                            unless exclusive                                  */
+  uint64_t budget_bytes; /* device memory of the rounds in flight (0 = 64 GiB) */
+  uint64_t round_bases;  /* bases per round at most (0 = 16,000,000)         */
+  int exclusive;         /* 1: a round holds one whole request, in arrival order
+                            (the per-request baseline)                        */
+  int reserved[3];
+} gs_server_opts;
+
+typedef struct {
+  double t_submit, t_start, t_done; /* seconds on the server clock; t_start =
+                                       start of the request's first round    */
+  uint64_t n_reads, n_seeds, n_pos;
+  uint32_t n_rounds, pad;
+} gs_request_info;
+
+typedef struct {
+  uint64_t n_requests, n_rounds, n_reads, n_bases;
+  double busy_seconds;       /* wall time of device rounds (upload to download) */
+  double device_seconds;     /* pipeline time of the rounds (HIP events)       */
+  uint64_t peak_round_bytes; /* largest round's device buffers, outputs excluded */
+  double elapsed;            /* server clock now                                */
+} gs_server_stats;
+
+GS_API int gs_server_create(gs_index* idx, const gs_server_opts* opts, gs_server** out);
+GS_API int gs_server_submit(gs_server* srv, const char* bases, const uint64_t* offsets, const uint32_t* lens,
+                            uint64_t n_reads, uint64_t* ticket);
+/* Blocks until the request is done; each ticket is waited for once. */
+GS_API int gs_server_wait(gs_server* srv, uint64_t ticket, gs_result* out, gs_request_info* info);
+GS_API int gs_server_get_stats(gs_server* srv, gs_server_stats* stats);
+GS_API double gs_server_clock(gs_server* srv);
+/* Finishes every submitted request, then stops the threads. */
+GS_API int gs_server_destroy(gs_server* srv);
 
 /* The -o text of a result (seed_gen.cu:3625-3648) written to file descriptor
  * fd, reads numbered from first_read_no. */

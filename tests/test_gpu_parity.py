@@ -199,7 +199,9 @@ def test_repeated_calls_are_identical(g, config1):
 
 def test_empty_batch_and_short_reads(g, config1):
     with g.Index(config1["prefix"]) as X:
-        out = g.seed(X, [], k=20, smem=True)
-        assert out.n_reads == 0 and len(out.seeds) == 0
+        for kernel in (0, 1):
+            out = g.seed(X, [], k=20, smem=True, kernel=kernel)
+            assert out.n_reads == 0 and len(out.seeds) == 0
+            assert list(out.read_seed_off) == [0] and list(out.read_pos_off) == [0]
         out = g.seed(X, [b"", b"ACGT", b"N" * 30], k=20, smem=False)
         assert out.as_lists() == [[], [], []]
