@@ -84,6 +84,106 @@ def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
                 bit_exact=bool(same))
 
 
+def make_requests(pool, n, max_reads, seed):
+    """n requests of log-uniform size in [1, max_reads] reads, each a run of
+    consecutive reads of the pool (views of its bytes, offsets rebased)."""
+    import numpy as np
+    buf, offs, lens = pool
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(0.0, np.log(max_reads), n)).round().astype(np.int64).clip(1, max_reads)
+    out = []
+    for m in sizes:
+        a = int(rng.integers(0, len(lens) - m + 1))
+        lo, hi = int(offs[a]), int(offs[a + m - 1] + lens[a + m - 1])
+        out.append((buf[lo:hi], offs[a:a + m] - np.uint64(lo), lens[a:a + m]))
+    return out
+
+
+def run_load(g, X, requests, rate, opts, exclusive, seed, label):
+    """Poisson arrivals at `rate` requests/s (0: all at once) into a server;
+    returns latencies (server clock: submit to results complete) and stats."""
+    import queue
+    import threading
+
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(requests)) if rate > 0 else np.zeros(len(requests))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lat, infos, errors = [], [], []
+    tickets = queue.Queue()
+    with g.Server(X, opts=opts, exclusive=exclusive) as S:
+        def waiter():
+            while True:
+                t = tickets.get()
+                if t is None:
+                    return
+                try:
+                    _, info = S.wait(t)
+                    infos.append(info)
+                except Exception as e:  # reported below
+                    errors.append(e)
+        th = [threading.Thread(target=waiter) for _ in range(8)]
+        for t in th:
+            t.start()
+        t0 = S.clock()
+        for r, at in zip(requests, arrivals):
+            d = t0 + at - S.clock()
+            if d > 0:
+                time.sleep(d)
+            tickets.put(S.submit(r))
+        for _ in th:
+            tickets.put(None)
+        for t in th:
+            t.join()
+        st = S.stats()
+    if errors:
+        raise errors[0]
+    lat = np.array([i.t_done - i.t_submit for i in infos])
+    reads = sum(int(i.n_reads) for i in infos)
+    span = max(i.t_done for i in infos) - min(i.t_submit for i in infos)
+    out = dict(requests=len(infos), reads=reads, seconds=span, reads_per_s=reads / span,
+               requests_per_s=len(infos) / span, p50_ms=float(np.percentile(lat, 50) * 1e3),
+               p99_ms=float(np.percentile(lat, 99) * 1e3), max_ms=float(lat.max() * 1e3),
+               rounds=int(st.n_rounds), device_busy=st.busy_seconds / max(span, 1e-9),
+               round_ms={"load": st.t_load / max(st.n_rounds, 1) * 1e3, "run": st.t_run / max(st.n_rounds, 1) * 1e3,
+                         "pipeline": st.device_seconds / max(st.n_rounds, 1) * 1e3,
+                         "fetch": st.t_fetch / max(st.n_rounds, 1) * 1e3,
+                         "split": st.t_split / max(st.n_rounds, 1) * 1e3})
+    log("%s: %d requests, %.0f reads/s, p50 %.1f ms, p99 %.1f ms, %d rounds; per round ms %s" %
+        (label, out["requests"], out["reads_per_s"], out["p50_ms"], out["p99_ms"], out["rounds"],
+         " ".join("%s %.1f" % kv for kv in out["round_ms"].items())))
+    return out
+
+
+def serve_bench(g, X, ref, k, max_occ, args):
+    """Configuration 5: requests of 1-200,000 reads (log-uniform) of the
+    configuration-4 length mix against the resident index, Poisson arrivals
+    at fractions of the measured peak; the batched server against a server
+    that runs each request alone with the plain kernel."""
+    rp, qp = g.preset(5)
+    qp.n_reads = args.serve_pool
+    log("serving: generating a pool of %d configuration-4 reads" % qp.n_reads)
+    pool = g.synth_reads(ref, qp)[:3]
+    opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel)
+    plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
+    # peak: a burst the server drains at full speed
+    burst = make_requests(pool, args.serve_burst, 200000, seed=50)
+    peak = run_load(g, X, burst, 0, opts, False, 1, "peak (burst of %d)" % len(burst))
+    levels = []
+    for i, frac in enumerate(args.serve_levels):
+        rate = frac * peak["requests_per_s"]
+        n = max(20, int(rate * args.serve_seconds))
+        reqs = make_requests(pool, n, 200000, seed=100 + i)
+        b = run_load(g, X, reqs, rate, opts, False, 200 + i, "batched at %.0f%% (%.1f req/s)" % (frac * 100, rate))
+        nb = min(n, args.serve_plain_requests)
+        p = run_load(g, X, reqs[:nb], rate, plain, True, 200 + i,
+                     "plain per-request at %.0f%%" % (frac * 100))
+        levels.append(dict(load=frac, rate_requests_per_s=rate, batched=b, plain=p))
+    return dict(peak=peak, levels=levels,
+                workload="requests of 1-200,000 reads (log-uniform), 70%% 100-250 bp / 25%% 1-5 kbp / 5%% 5-20 kbp, "
+                         "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
+
+
 def timed_steps(batches, warmup, kernel):
     for b in range(warmup):
         s = batches[b][1].run(kernel)
@@ -119,6 +219,15 @@ def main():
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement (configuration 5)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving measurement")
+    ap.add_argument("--serve-levels", type=float, nargs="*", default=[0.1, 0.5, 0.8, 0.9],
+                    help="offered loads as fractions of the measured peak")
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="arrival window per load level")
+    ap.add_argument("--serve-pool", type=int, default=600000, help="reads in the request pool")
+    ap.add_argument("--serve-burst", type=int, default=60, help="requests of the peak burst")
+    ap.add_argument("--serve-plain-requests", type=int, default=25,
+                    help="requests per load level for the plain per-request server")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this project has exactly one MI355X, "
@@ -158,8 +267,15 @@ def main():
     log("index: %.1fs total; suffix sort %.1fs in %d rounds, occ %.1fs, files %.1fs, table %.2fs, "
         "%.2f GB on device" % (t_build, info.t_sa, info.sa_rounds, info.t_repack, info.t_write,
                                info.t_table, info.device_bytes / 1e9))
+    if args.serve_only:
+        serving = serve_bench(g, X, ref, k, max_occ, args)
+        X.close()
+        shutil.rmtree(tmpdir, ignore_errors=True)
+        print(json.dumps({"metric": "serving (configuration 5 workload)", "serving": serving}), flush=True)
+        return
     log("generating %d reads" % qp.n_reads)
     buf, offs, lens, _, _ = g.synth_reads(ref, qp)
+    serve_ref = ref if (args.config == 3 and not args.no_serve and not args.profile_steps) else None
     del ref
     batches = []
     for b in range(n_batches):
@@ -207,6 +323,10 @@ def main():
         cpu["threads"] = threads
         log("cpu baseline: %.0f reads/s on %d threads; parity %d/%d reads differ"
             % (cpu["reads_per_s"], threads, cpu["mismatches"], cpu["reads"]))
+    serving = None
+    if serve_ref is not None:
+        serving = serve_bench(g, X, serve_ref, k, max_occ, args)
+        del serve_ref
     X.close()
     shutil.rmtree(tmpdir, ignore_errors=True)
 
@@ -268,6 +388,11 @@ def main():
                       "write": sum(s.t_write for s in stats) / len(stats) * 1e3,
                       "locate": sum(s.t_locate for s in stats) / len(stats) * 1e3},
     }
+    if serving:
+        out["serving"] = serving
+        l80 = [lv for lv in serving["levels"] if lv["load"] == 0.8]
+        if l80:
+            out["p99_ms_at_80pct_load"] = l80[0]["batched"]["p99_ms"]
     out.update(extra)
     print(json.dumps(out), flush=True)
 

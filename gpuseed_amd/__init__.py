@@ -68,7 +68,7 @@ class Opts(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("opts", Opts), ("budget_bytes", C.c_uint64), ("round_bases", C.c_uint64),
-                ("exclusive", C.c_int), ("reserved", C.c_int * 3)]
+                ("exclusive", C.c_int), ("workers", C.c_int), ("reserved", C.c_int * 2)]
 
 
 class RequestInfo(C.Structure):
@@ -80,6 +80,7 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("n_requests", C.c_uint64), ("n_rounds", C.c_uint64), ("n_reads", C.c_uint64),
                 ("n_bases", C.c_uint64), ("busy_seconds", C.c_double), ("device_seconds", C.c_double),
+                ("t_load", C.c_double), ("t_run", C.c_double), ("t_fetch", C.c_double), ("t_split", C.c_double),
                 ("peak_round_bytes", C.c_uint64), ("elapsed", C.c_double)]
 
 
@@ -421,7 +422,7 @@ class Server:
     """Continuous-batching seeding server over a resident index (gs_server_*).
     submit() returns a ticket; wait(ticket) returns (SeedOutput, RequestInfo)."""
 
-    def __init__(self, index, budget_bytes=0, round_bases=0, exclusive=False, opts=None, **kw):
+    def __init__(self, index, budget_bytes=0, round_bases=0, exclusive=False, workers=0, opts=None, **kw):
         L = lib()
         _bind_server(L)
         o = ServerOpts()
@@ -429,6 +430,7 @@ class Server:
         o.budget_bytes = budget_bytes
         o.round_bases = round_bases
         o.exclusive = 1 if exclusive else 0
+        o.workers = workers
         self.h = C.c_void_p()
         self._inputs = {}
         _check("gs_server_create", L.gs_server_create(index.h, C.byref(o), C.byref(self.h)))

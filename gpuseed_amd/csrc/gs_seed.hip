@@ -594,6 +594,7 @@ void launch_seed(const SeedParams& P, hipStream_t st) {
 
 struct gs_batch {
   gs_index* X = nullptr;
+  hipStream_t stream = nullptr;  // a batch's own stream: batches of one index may run concurrently
   gs_opts opts;
   uint64_t n_reads = 0, span = 0, tasks = 0;
   uint32_t maxlen = 0;
@@ -630,6 +631,7 @@ static void destroy_batch(gs_batch* B) {
   if (B->h_tot) (void)hipHostFree(B->h_tot);
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
   delete B;
 }
 
@@ -703,9 +705,26 @@ int gs::plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const
   return GS_OK;
 }
 
+// Work items in dispatch order: whole-read streams (up to ~150 task ends each)
+// before the segments of long reads (at most seg_tasks each), so that the
+// last items a persistent kernel hands out are the small ones.
 void gs::plan_finish(BatchPlan& P) {
   P.task_off.push_back(P.tasks);
   P.code_off.push_back(P.code_chunks);
+  if (P.long_reads.empty() || P.long_reads.size() == P.n_reads) return;
+  std::vector<uint8_t> is_long(P.n_reads, 0);
+  for (uint32_t r : P.long_reads) is_long[r] = 1;
+  std::vector<uint64_t> items;
+  items.reserve(P.items.size());
+  for (uint64_t v : P.items)
+    if (!is_long[(uint32_t)v]) items.push_back(v);
+  for (size_t i = 0; i < P.long_reads.size(); ++i) {
+    const uint64_t a = P.item0[i];
+    const uint64_t n = 2ull * gs::n_segments(P.lens[P.long_reads[i]], (uint32_t)P.opts.k, P.seg_tasks);
+    P.item0[i] = items.size();
+    items.insert(items.end(), P.items.begin() + a, P.items.begin() + a + n);
+  }
+  P.items.swap(items);
 }
 
 uint64_t gs::plan_device_bytes(const BatchPlan& P) {
@@ -716,6 +735,7 @@ uint64_t gs::plan_device_bytes(const BatchPlan& P) {
 }
 
 static int batch_events(gs_batch* B) {
+  if (!B->stream && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return GS_EHIP;
   for (auto& e : B->ev)
     if (!e && hipEventCreate(&e) != hipSuccess) return GS_EHIP;
   if (!B->h_tot && hipHostMalloc((void**)&B->h_tot, 16, hipHostMallocDefault) != hipSuccess) return GS_ENOMEM;
@@ -752,7 +772,7 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
       B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
       B->queue.ensure(16) || B->counters.ensure(32))
     return GS_ENOMEM;
-  hipStream_t st = X->stream;
+  hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
   uint64_t at = 0;
   for (const auto& pc : P.pieces) {
@@ -940,7 +960,7 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
   if (!B) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  hipStream_t st = X->stream;
+  hipStream_t st = B->stream;
   const int kernel = kernel_override >= 0 ? kernel_override : B->opts.kernel;
   if (kernel < 0 || kernel > 2) return GS_EINVAL;
   unsigned pgrid = 0;
@@ -1029,7 +1049,7 @@ extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel) {
   if (kern < 0 || kern > 2) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  hipStream_t st = X->stream;
+  hipStream_t st = B->stream;
   free_graph(B);
   free_host_results(B);
   const uint64_t n = B->n_reads, nr1 = n + 1;
@@ -1102,7 +1122,7 @@ extern "C" GS_API int gs_batch_replay(gs_batch* B, gs_batch_stats* stats) {
   if (!B || !B->gexec) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  hipStream_t st = X->stream;
+  hipStream_t st = B->stream;
   B->h_tot[0] = B->h_tot[1] = 0;
   GS_HIP(hipEventRecord(B->ev[0], st));
   GS_HIP(hipGraphLaunch(B->gexec, st));
@@ -1129,7 +1149,7 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   if (!B || !out || !B->ran) return GS_EINVAL;
   memset(out, 0, sizeof(*out));
   if (hipSetDevice(B->X->device) != hipSuccess) return GS_EHIP;
-  hipStream_t st = B->X->stream;
+  hipStream_t st = B->stream;
   const uint64_t n = B->n_reads, nr1 = n + 1;
   out->n_reads = n;
   out->n_seeds = B->n_seeds;

@@ -5,19 +5,21 @@
 // the read that brings it to batch_bases, seed_gen.cu:2955-2991), so a chunk
 // of a request seeds exactly as the request would alone: SMEM seeds depend on
 // the read only, MEM row skips never leave a batch (plan_add starts a new
-// batch id for every chunk).  Two threads:
-//  * the planner builds the next round on the CPU while the device works:
+// batch id for every chunk).  Threads:
+//  * a planner builds the next round on the CPU while the device works:
 //    every request with unscheduled reads first gets a fair share of the
 //    round (round_bases / active requests, at least one reference batch),
-//    then the round is filled in arrival order up to round_bases and to half
-//    the device budget (two rounds can hold device memory at once: the one
-//    running and the one being loaded);
-//  * the device thread uploads a round into a reused batch, runs the whole
-//    pipeline (one persistent-kernel launch for the reads of every request in
-//    the round), downloads the results and appends each chunk's share to its
-//    request.  A request is done when its last chunk is.
-// Requests arriving while a round runs join the next one; a large request is
-// spread over several rounds, interleaved with the requests behind it.
+//    then the round is filled in arrival order up to round_bases and to the
+//    device budget divided among the rounds that can hold memory at once;
+//  * `workers` device threads (default 2), each with its own batch and HIP
+//    stream, take rounds in turn: upload, the whole pipeline (one
+//    persistent-kernel launch over the reads of every request in the round),
+//    download, then hand each chunk's share to its request.  Two workers keep
+//    two rounds on the GPU at once, so the waves of the next round fill the
+//    compute units that the tail of the previous one leaves idle.
+// Requests arriving while rounds run join the next one; a large request is
+// spread over several rounds, interleaved with the requests behind it, and
+// its chunks are put back in read order when it is collected.
 #include <math.h>
 #include <string.h>
 
@@ -46,9 +48,15 @@ struct Request {
   size_t next_cut = 0;         // first cut not yet scheduled
   uint64_t next_read = 0;      // first read not yet scheduled
   uint64_t done_reads = 0;
-  std::vector<uint32_t> nseeds, npos;
-  std::vector<gs_seed_rec> seeds;
-  std::vector<uint32_t> pos;
+  struct Piece {  // results of one chunk, written by the worker that runs it
+    std::vector<uint32_t> nseeds, npos;
+    std::vector<gs_seed_rec> seeds;
+    std::vector<uint32_t> pos;
+  };
+  // by chunk ordinal, i.e. in read order; each piece is its own allocation so
+  // that a worker filling one never races with the planner adding another
+  std::vector<std::shared_ptr<Piece>> pieces;
+  uint64_t n_seeds = 0, n_pos = 0;
   double t_submit = 0, t_start = -1, t_done = -1;
   uint32_t rounds = 0;
   uint32_t inflight = 0;   // chunks planned and not yet processed
@@ -61,6 +69,7 @@ struct Chunk {
   Request* q;
   uint64_t r0, r1;  // reads of the request
   uint64_t at;      // first read in the round
+  std::shared_ptr<Request::Piece> piece;
 };
 
 struct Round {
@@ -83,8 +92,10 @@ struct gs_server {
   uint64_t next_id = 1;
   bool stop = false;
   int fatal = GS_OK;
-  std::thread planner, device;
-  gs_batch* B = nullptr;
+  std::thread planner;
+  std::vector<std::thread> devices;
+  std::vector<gs_batch*> batches;  // one per device thread
+  int running = 0;                 // rounds on the device
   gs_server_stats st{};
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -117,7 +128,7 @@ std::unique_ptr<Round> plan_round(gs_server* S) {
   auto R = std::make_unique<Round>();
   gs::plan_reset(R->plan, S->o.opts);
   const uint64_t cap = S->o.round_bases;
-  const uint64_t mem_cap = S->o.budget_bytes / 2;
+  const uint64_t mem_cap = S->o.budget_bytes / (uint64_t)(S->o.workers + 1);
   uint64_t bases = 0, est = 0;
   // device bytes of a chunk, an upper bound of what plan_device_bytes adds:
   // bytes, codes, two candidate slots per task, work items and per-read arrays
@@ -133,7 +144,9 @@ std::unique_ptr<Round> plan_round(gs_server* S) {
     if (gs::plan_add(R->plan, q->bases, q->offs, q->lens, q->next_read, end) != GS_OK) return false;
     bases += b;
     est += e;
-    R->chunks.push_back({q, q->next_read, end, at});
+    auto piece = std::make_shared<Request::Piece>();
+    q->pieces.push_back(piece);
+    R->chunks.push_back({q, q->next_read, end, at, piece});
     ++q->inflight;
     advance(q, end);
     return true;
@@ -197,44 +210,57 @@ void planner_main(gs_server* S) {
   S->cv_dev.notify_all();
 }
 
-void device_main(gs_server* S) {
+void device_main(gs_server* S, gs_batch* B) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
     S->cv_dev.wait(lk, [&] { return S->ready || (S->stop && S->active.empty()); });
     if (!S->ready) break;
     std::unique_ptr<Round> R = std::move(S->ready);
+    ++S->running;
     S->cv_plan.notify_all();
     const double t_round = S->now();
-    for (const Chunk& c : R->chunks) {
-      if (c.q->t_start < 0) c.q->t_start = t_round;
-      ++c.q->rounds;
+    std::vector<char> skip(R->chunks.size());  // requests that already failed keep nothing more
+    for (size_t i = 0; i < R->chunks.size(); ++i) {
+      Request* q = R->chunks[i].q;
+      if (q->t_start < 0) q->t_start = t_round;
+      ++q->rounds;
+      skip[i] = q->done;
     }
     lk.unlock();
-    // device work without the lock: the planner builds the next round meanwhile
-    int rc = gs::batch_load(S->B, R->plan);
+    // device work without the lock: the planner and the other workers go on
+    int rc = gs::batch_load(B, R->plan);
+    const double t1 = S->now();
     gs_batch_stats bs;
     memset(&bs, 0, sizeof bs);
-    if (!rc) rc = gs_batch_run(S->B, -1, &bs);
+    if (!rc) rc = gs_batch_run(B, -1, &bs);
+    const double t2 = S->now();
     gs_result res;
     memset(&res, 0, sizeof res);
-    if (!rc) rc = gs_batch_fetch(S->B, &res);
+    if (!rc) rc = gs_batch_fetch(B, &res);
+    const double t3 = S->now();
     if (!rc) {
-      for (const Chunk& c : R->chunks) {  // append each chunk's results to its request
-        Request* q = c.q;
-        if (q->done) continue;  // failed earlier: nothing more is kept
+      for (size_t i = 0; i < R->chunks.size(); ++i) {  // each chunk's results into its piece
+        const Chunk& c = R->chunks[i];
+        if (skip[i]) continue;
+        Request::Piece& pc = *c.piece;
         const uint64_t a = c.at, z = c.at + (c.r1 - c.r0);
-        q->nseeds.insert(q->nseeds.end(), res.read_nseeds + a, res.read_nseeds + z);
-        q->npos.insert(q->npos.end(), res.read_npos + a, res.read_npos + z);
-        q->seeds.insert(q->seeds.end(), res.seeds + res.read_seed_off[a], res.seeds + res.read_seed_off[z]);
-        q->pos.insert(q->pos.end(), res.pos + res.read_pos_off[a], res.pos + res.read_pos_off[z]);
+        pc.nseeds.assign(res.read_nseeds + a, res.read_nseeds + z);
+        pc.npos.assign(res.read_npos + a, res.read_npos + z);
+        pc.seeds.assign(res.seeds + res.read_seed_off[a], res.seeds + res.read_seed_off[z]);
+        pc.pos.assign(res.pos + res.read_pos_off[a], res.pos + res.read_pos_off[z]);
       }
     }
     gs_result_free(&res);
     const double t_end = S->now();
     lk.lock();
+    --S->running;
     S->st.n_rounds++;
     S->st.busy_seconds += t_end - t_round;
     S->st.device_seconds += bs.t_total;
+    S->st.t_load += t1 - t_round;
+    S->st.t_run += t2 - t1;
+    S->st.t_fetch += t3 - t2;
+    S->st.t_split += t_end - t3;
     S->st.n_reads += R->plan.n_reads;
     S->st.n_bases += R->plan.span;
     if (R->device_bytes > S->st.peak_round_bytes) S->st.peak_round_bytes = R->device_bytes;
@@ -246,6 +272,8 @@ void device_main(gs_server* S) {
           finish_request(S, q, rc);
         } else {
           q->done_reads += c.r1 - c.r0;
+          q->n_seeds += c.piece->seeds.size();
+          q->n_pos += c.piece->pos.size();
           if (q->done_reads == q->n) finish_request(S, q, GS_OK);
         }
       }
@@ -268,10 +296,12 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (!S->o.budget_bytes) S->o.budget_bytes = 64ull << 30;
   if (!S->o.round_bases) S->o.round_bases = 16000000ull;
   if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 1;
+  if (S->o.workers <= 0) S->o.workers = S->o.exclusive ? 1 : 2;
+  if (S->o.workers > 8) S->o.workers = 8;
   S->t0 = Clock::now();
-  S->B = gs::batch_new(X);
+  for (int w = 0; w < S->o.workers; ++w) S->batches.push_back(gs::batch_new(X));
   S->planner = std::thread(planner_main, S);
-  S->device = std::thread(device_main, S);
+  for (int w = 0; w < S->o.workers; ++w) S->devices.emplace_back(device_main, S, S->batches[w]);
   *out = S;
   return GS_OK;
 }
@@ -340,8 +370,8 @@ extern "C" GS_API int gs_server_wait(gs_server* S, uint64_t ticket, gs_result* o
     info->t_start = q->t_start;
     info->t_done = q->t_done;
     info->n_reads = q->n;
-    info->n_seeds = q->seeds.size();
-    info->n_pos = q->pos.size();
+    info->n_seeds = q->n_seeds;
+    info->n_pos = q->n_pos;
     info->n_rounds = q->rounds;
   }
   if (q->rc) {
@@ -354,8 +384,8 @@ extern "C" GS_API int gs_server_wait(gs_server* S, uint64_t ticket, gs_result* o
   const uint64_t n = q->n, nr1 = n + 1;
   int rc = GS_OK;
   out->n_reads = n;
-  out->n_seeds = q->seeds.size();
-  out->n_pos = q->pos.size();
+  out->n_seeds = q->n_seeds;
+  out->n_pos = q->n_pos;
   out->read_nseeds = (uint32_t*)malloc(4 * nr1);
   out->read_npos = (uint32_t*)malloc(4 * nr1);
   out->read_seed_off = (uint64_t*)malloc(8 * nr1);
@@ -367,19 +397,26 @@ extern "C" GS_API int gs_server_wait(gs_server* S, uint64_t ticket, gs_result* o
     gs_result_free(out);
     rc = GS_ENOMEM;
   } else {
-    uint64_t so = 0, po = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-      out->read_nseeds[i] = q->nseeds[i];
-      out->read_npos[i] = q->npos[i];
-      out->read_seed_off[i] = so;
-      out->read_pos_off[i] = po;
-      so += q->nseeds[i];
-      po += q->npos[i];
+    uint64_t so = 0, po = 0, i = 0;
+    for (const auto& pp : q->pieces) {  // chunks in read order
+      const Request::Piece& pc = *pp;
+      for (size_t j = 0; j < pc.nseeds.size(); ++j, ++i) {
+        out->read_nseeds[i] = pc.nseeds[j];
+        out->read_npos[i] = pc.npos[j];
+        out->read_seed_off[i] = so;
+        out->read_pos_off[i] = po;
+        so += pc.nseeds[j];
+        po += pc.npos[j];
+      }
+      memcpy(out->seeds + (so - (pc.seeds.size())), pc.seeds.data(), sizeof(gs_seed_rec) * pc.seeds.size());
+      memcpy(out->pos + (po - pc.pos.size()), pc.pos.data(), 4 * pc.pos.size());
     }
     out->read_seed_off[n] = so;
     out->read_pos_off[n] = po;
-    memcpy(out->seeds, q->seeds.data(), sizeof(gs_seed_rec) * out->n_seeds);
-    memcpy(out->pos, q->pos.data(), 4 * out->n_pos);
+    if (i != n || so != out->n_seeds || po != out->n_pos) {
+      gs_result_free(out);
+      rc = GS_EHIP;  // pieces do not add up: must not happen
+    }
   }
   lk.lock();
   release_request(q);
@@ -405,9 +442,9 @@ extern "C" GS_API int gs_server_destroy(gs_server* S) {
     S->cv_dev.notify_all();
   }
   S->planner.join();
-  S->device.join();
+  for (auto& t : S->devices) t.join();
   for (auto& kv : S->reqs) delete kv.second;  // submitted but never collected
-  gs::batch_free(S->B);
+  for (gs_batch* B : S->batches) gs::batch_free(B);
   const int rc = S->fatal;
   delete S;
   return rc;

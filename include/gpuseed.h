@@ -259,11 +259,12 @@ GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64_t* offsets
  * thread with continuous batching: a planner thread packs the unscheduled
  * reads of every waiting request into device rounds (each request gets a fair
  * share of a round, whole 1,000,000-base reference batches at a time, then the
- * round is filled in arrival order), bounded by round_bases and by half of
- * budget_bytes of device memory for read and seed buffers; a device thread
- * runs each round as one pipeline (one persistent-kernel launch over all its
- * requests) while the planner prepares the next.  Requests arriving during a
- * round join the next one.  A request's results are exactly those of gs_seed
+ * round is filled in arrival order), bounded by round_bases and by a share of
+ * budget_bytes of device memory for read and seed buffers (the budget over
+ * workers + 1 rounds); device threads run rounds as one pipeline each (one
+ * persistent-kernel launch over all its requests), two rounds on the GPU at
+ * once by default, while the planner prepares the next.  Requests arriving
+ * during a round join the next one.  A request's results are exactly those of gs_seed
  * on the request alone.  The caller's read buffers must stay valid until
  * gs_server_wait returns for that request.  No counterpart in the reference. */
 typedef struct gs_server gs_server;
@@ -275,7 +276,9 @@ typedef struct {
   uint64_t round_bases;  /* bases per round at most (0 = 16,000,000)         */
   int exclusive;         /* 1: a round holds one whole request, in arrival order
                             (the per-request baseline)                        */
-  int reserved[3];
+  int workers;           /* device threads, each with its own stream and round
+                            buffers (0 = 2, or 1 when exclusive; at most 8)   */
+  int reserved[2];
 } gs_server_opts;
 
 typedef struct {
@@ -289,6 +292,7 @@ typedef struct {
   uint64_t n_requests, n_rounds, n_reads, n_bases;
   double busy_seconds;       /* wall time of device rounds (upload to download) */
   double device_seconds;     /* pipeline time of the rounds (HIP events)       */
+  double t_load, t_run, t_fetch, t_split; /* wall time of the round stages, summed */
   uint64_t peak_round_bytes; /* largest round's device buffers, outputs excluded */
   double elapsed;            /* server clock now                                */
 } gs_server_stats;
