@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="reads timed on the CPU (default 1,000,000; configuration 4: 20,000)")
     ap.add_argument("--seg-tasks", type=int, default=0, help="task ends per work item of long reads")
+    ap.add_argument("--kmer-len", type=int, default=16,
+                    help="interval table length: 13 is the reference's table, 14..16 add a longer one")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
@@ -267,6 +269,10 @@ def main():
     log("index: %.1fs total; suffix sort %.1fs in %d rounds, occ %.1fs, files %.1fs, table %.2fs, "
         "%.2f GB on device" % (t_build, info.t_sa, info.sa_rounds, info.t_repack, info.t_write,
                                info.t_table, info.device_bytes / 1e9))
+    if args.kmer_len > 13:
+        X.set_kmer_len(args.kmer_len)
+        info = X.info()
+        log("%d-mer interval table in %.2fs, %.2f GB on device" % (info.kmer_len, info.t_ext, info.device_bytes / 1e9))
     if args.serve_only:
         serving = serve_bench(g, X, ref, k, max_occ, args)
         X.close()
@@ -372,7 +378,7 @@ def main():
                                   "%d-%d bp (mean %.0f)" % (int(lens.min()), int(lens.max()), float(lens.mean())),
                                   rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
-                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7,
+                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
                    "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {

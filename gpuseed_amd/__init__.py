@@ -57,7 +57,7 @@ class IndexInfo(C.Structure):
                 ("L2", C.c_uint32 * 5), ("n_blocks", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("t_read", C.c_double), ("t_upload", C.c_double), ("t_repack", C.c_double),
                 ("t_table", C.c_double), ("t_sa", C.c_double), ("t_write", C.c_double),
-                ("sa_rounds", C.c_int), ("pad", C.c_int)]
+                ("sa_rounds", C.c_int), ("kmer_len", C.c_int), ("t_ext", C.c_double)]
 
 
 class Opts(C.Structure):
@@ -135,6 +135,8 @@ def lib():
         L.gs_index_get_info.argtypes = [C.c_void_p, C.POINTER(IndexInfo)]
         L.gs_index_read_occ.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_read_kmer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
+        L.gs_index_read_kext.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
+        L.gs_index_set_kmer_len.argtypes = [C.c_void_p, C.c_int]
         L.gs_seed.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint64, C.POINTER(Opts),
                               C.POINTER(Result)]
         L.gs_result_free.argtypes = [C.POINTER(Result)]
@@ -238,6 +240,15 @@ class Index:
     def read_occ(self, first, count):
         out = np.empty((count, 4), dtype=np.uint32)
         _check("gs_index_read_occ", lib().gs_index_read_occ(self.h, first, count, _ptr(out, C.c_uint32)))
+        return out
+
+    def set_kmer_len(self, length):
+        """Adds (14..16) or drops (13) the extended interval table."""
+        _check("gs_index_set_kmer_len", lib().gs_index_set_kmer_len(self.h, length))
+
+    def read_kext(self, first, count):
+        out = np.empty((count, 2), dtype=np.uint32)
+        _check("gs_index_read_kext", lib().gs_index_read_kext(self.h, first, count, _ptr(out, C.c_uint32)))
         return out
 
     def read_kmer(self, first, count):

@@ -376,6 +376,7 @@ extern "C" GS_API int gs_index_build(const uint8_t* codes, uint64_t n, uint32_t 
     (void)hipFree(X->occ);
     (void)hipFree(X->sa);
     (void)hipFree(X->kmer);
+    (void)hipFree(X->kext);
     if (X->stream) (void)hipStreamDestroy(X->stream);
     delete X;
     return rc;

@@ -36,8 +36,9 @@ __device__ __forceinline__ uint32_t rank_in_entry(uint4 e, uint32_t bit) {
 struct DevIndex {
   const uint4* occ;      // [n_blocks * 4]
   const uint2* kmer;     // 4^13 (l, u) intervals
+  const uint2* kext;     // optional 4^ext_len intervals (ext_len 14..16), else null
   const uint32_t* sa;    // sampled suffix array, sa[0] = 0xFFFFFFFF
-  uint32_t primary, seq_len, sa_intv, pad;
+  uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)
   uint32_t L2[5];
 };

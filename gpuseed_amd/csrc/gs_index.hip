@@ -6,6 +6,8 @@
 #include <string.h>
 #include <sys/time.h>
 
+#include <chrono>
+
 #include "gs_internal.h"
 #include "host/gs_host.h"
 
@@ -29,11 +31,12 @@ DevIndex gs_index::dev() const {
   DevIndex d;
   d.occ = occ;
   d.kmer = kmer;
+  d.kext = kext;
   d.sa = sa;
   d.primary = primary;
   d.seq_len = seq_len;
   d.sa_intv = sa_intv;
-  d.pad = 0;
+  d.ext_len = kext ? ext_len : 0;
   d.sa_div = gs_reciprocal64(sa_intv);
   for (int c = 0; c < 5; ++c) d.L2[c] = L2[c];
   return d;
@@ -109,8 +112,8 @@ __global__ void repack_occ96(const uint32_t* __restrict__ bwt, uint32_t n, uint6
 // after i characters depends on the low 2i bits of p; character i is applied
 // next; an empty interval is carried unchanged (the break at 2666-2668).
 __global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uint2* __restrict__ next,
-                           uint32_t level, uint64_t count) {
-  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+                           uint32_t level, uint64_t first, uint64_t count) {
+  const uint64_t t = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (t >= count) return;
   const uint64_t q = t & ((1ull << (2 * level)) - 1);
   const uint32_t c = (uint32_t)(t >> (2 * level));
@@ -125,6 +128,17 @@ __global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uin
 }
 
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// One table level in launches of at most 2^30 work-items (a dispatch's
+// work-item count is a 32-bit quantity; level 16 has 2^32 entries).
+void launch_level(const DevIndex& D, const uint2* prev, uint2* next, uint32_t level, uint64_t cnt,
+                  hipStream_t st) {
+  const uint64_t step = 1ull << 30;
+  for (uint64_t first = 0; first < cnt; first += step) {
+    const uint64_t n = cnt - first < step ? cnt - first : step;
+    hipLaunchKernelGGL(kmer_level, dim3(grid_for(n, 256)), dim3(256), 0, st, D, prev, next, level, first, cnt);
+  }
+}
 
 }  // namespace
 
@@ -143,8 +157,7 @@ int gs_build_kmer_table(gs_index* X) {
   for (uint32_t lev = 0; lev < (uint32_t)gs::kPrecalc; ++lev) {
     const uint64_t cnt = 1ull << (2 * (lev + 1));
     uint2* next = lev + 1 == (uint32_t)gs::kPrecalc ? X->kmer : ((lev & 1) ? b : a);
-    hipLaunchKernelGGL(kmer_level, dim3(grid_for(cnt, 256)), dim3(256), 0, X->stream, D, prev, next,
-                       lev, cnt);
+    launch_level(D, prev, next, lev, cnt, X->stream);
     prev = next;
   }
   hipError_t e = hipStreamSynchronize(X->stream);
@@ -153,11 +166,62 @@ int gs_build_kmer_table(gs_index* X) {
   return e == hipSuccess ? GS_OK : GS_EHIP;
 }
 
+// The extended table: the intervals of all 4^len patterns (len 14..16),
+// entry p = sum of code_j * 4^j in backward-search order, empty patterns
+// stored as l > u.  Built from the 13-mer table one level at a time with the
+// same kernel; seeding then answers the first len bases of a task with one
+// lookup and goes back to the 13-mer table only when that pattern is absent
+// or its window holds a code above 3.
+extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
+  if (!X || len < gs::kPrecalc || len > 16) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  if (X->kext) {
+    GS_HIP(hipStreamSynchronize(X->stream));
+    (void)hipFree(X->kext);
+    X->device_bytes -= (1ull << (2 * X->ext_len)) * sizeof(uint2);
+    X->kext = nullptr;
+    X->ext_len = 0;
+  }
+  if (len == gs::kPrecalc) return GS_OK;
+  const double t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  uint2* levels[4] = {nullptr, nullptr, nullptr, nullptr};  // levels 14..len
+  int rc = GS_OK;
+  const DevIndex D = X->dev();
+  const uint2* prev = X->kmer;
+  for (int lev = gs::kPrecalc; lev < len && !rc; ++lev) {
+    const uint64_t cnt = 1ull << (2 * (lev + 1));
+    uint2*& next = levels[lev + 1 - 14];
+    if (hipMalloc(&next, cnt * sizeof(uint2)) != hipSuccess) {
+      rc = GS_ENOMEM;
+      break;
+    }
+    launch_level(D, prev, next, (uint32_t)lev, cnt, X->stream);
+    if (hipGetLastError() != hipSuccess) rc = GS_EHIP;
+    prev = next;
+  }
+  if (!rc && hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
+  for (int lev = 14; lev <= 16; ++lev) {
+    uint2* a = levels[lev - 14];
+    if (!a) continue;
+    if (lev == len && !rc) {
+      X->kext = a;
+      X->ext_len = (uint32_t)len;
+      X->device_bytes += (1ull << (2 * len)) * sizeof(uint2);
+    } else {
+      (void)hipFree(a);
+    }
+  }
+  const double t1 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  X->t_ext = t1 - t0;
+  return rc;
+}
+
 static void free_index(gs_index* X) {
   if (!X) return;
   (void)hipFree(X->occ);
   (void)hipFree(X->sa);
   (void)hipFree(X->kmer);
+  (void)hipFree(X->kext);
   if (X->stream) (void)hipStreamDestroy(X->stream);
   delete X;
 }
@@ -281,6 +345,8 @@ extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) 
   info->t_sa = X->t_sa;
   info->t_write = X->t_write;
   info->sa_rounds = X->sa_rounds;
+  info->kmer_len = X->kext ? (int)X->ext_len : gs::kPrecalc;
+  info->t_ext = X->t_ext;
   return GS_OK;
 }
 
@@ -290,6 +356,13 @@ extern "C" GS_API int gs_index_read_occ(const gs_index* X, uint64_t first, uint6
   if (!X || !out4 || first + count > (X->n_blocks + 1) * 4) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   GS_HIP(hipMemcpy(out4, X->occ + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_index_read_kext(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out2) {
+  if (!X || !out2 || !X->kext || first + count > (1ull << (2 * X->ext_len))) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  GS_HIP(hipMemcpy(out2, X->kext + first, count * sizeof(uint2), hipMemcpyDeviceToHost));
   return GS_OK;
 }
 

@@ -10,8 +10,9 @@
 //    pack_codes, so a refill is a few 16-byte loads and LDS stores;
 //  * a lane's LDS slot holds its stream's codes (4 bits per base) and its
 //    size record (one AT per position), indexed directly in LDS;
-//  * the loop body is one memory operation (a 13-mer table entry or one or
-//    two occ entries) followed by a short processing step; the rare
+//  * the loop body is one memory operation (an interval-table entry, from
+//    the extended table when the index has one, or one or two occ entries)
+//    followed by a short processing step; the rare
 //    transitions (windows with a code > 3, N-transparent positions, reaching
 //    the read start) are resolved inside the processing step.
 #include "gs_internal.h"
@@ -46,6 +47,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   uint32_t t0 = 0, first_b = 0, flags = 0;
   uint64_t item = 0;
   int phase = 0;
+  bool ext_miss = false;  // the task's extended-table pattern is absent: use the 13-mer path
   uint4* out = nullptr;
   const uint4* gcodes = nullptr;  // the stream's codes in device memory (segments only)
   uint64_t n_table = 0, n_occ = 0;
@@ -77,6 +79,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   // next task with a clean 13-base window, or end of the stream
   auto next_task = [&]() {
     phase = 0;
+    ext_miss = false;
     do {
       if (!advance()) {
         end_stream();
@@ -114,7 +117,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     while (true) {
       if (x < 0) {
         finish_task();
-        next_task();
+        if (P.smem)
+          end_stream();  // later tasks of the item end at the read start too and are filtered out
+        else
+          next_task();
         return;
       }
       const uint32_t c = code(x);
@@ -209,8 +215,23 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const uint4* a2 = nullptr;
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
+    uint32_t tlen = gs::kPrecalc, tidx = p;  // table lookup: pattern length and entry
     if (op == 1) {
-      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);
+      const uint2* tbl = P.X.kmer;
+      if (P.X.ext_len && !ext_miss && e >= (int32_t)P.X.ext_len) {
+        uint32_t q = p, ok = 1;
+        for (uint32_t j = gs::kPrecalc; j < P.X.ext_len; ++j) {
+          const uint32_t c = code(e - 1 - (int32_t)j);
+          ok &= c <= 3 ? 1u : 0u;
+          q |= (c & 3u) << (2 * j);
+        }
+        if (ok) {
+          tlen = P.X.ext_len;
+          tidx = q;
+          tbl = P.X.kext;
+        }
+      }
+      a1 = reinterpret_cast<const uint4*>(tbl) + (tidx >> 1);
       ++n_table;
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
@@ -229,18 +250,21 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     if (a2) d2 = *a2;
 
     if (op == 1) {
-      const uint32_t il = (p & 1u) ? d1.z : d1.x, iu = (p & 1u) ? d1.w : d1.y;
+      const uint32_t il = (tidx & 1u) ? d1.z : d1.x, iu = (tidx & 1u) ? d1.w : d1.y;
       if (il > iu) {
-        next_task();
+        if (tlen > gs::kPrecalc)
+          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
+        else
+          next_task();
       } else {
         l = il;
         u = iu;
         s = u - l + 1u;
-        b = (uint32_t)e - gs::kPrecalc;
+        b = (uint32_t)e - tlen;
         if (prune_at((int32_t)b, s)) {
           next_task();
         } else {
-          x = e - gs::kPrecalc - 1;
+          x = (int32_t)b - 1;
           phase = 1;
           prepare_x();
         }

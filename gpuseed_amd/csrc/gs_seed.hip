@@ -26,7 +26,9 @@
 //    per-position size record.
 //  * SMEM filtering happens in the lane: tasks finish in decreasing end order
 //    and left ends never increase, so a task is a survivor exactly when its
-//    left end is smaller than the previous finished task's (991-1018).
+//    left end is smaller than the previous finished task's (991-1018).  For
+//    the same reason an SMEM stream ends when a task reaches the read start:
+//    every later task would end there too and be filtered out.
 #include <stdlib.h>
 #include <string.h>
 
@@ -111,12 +113,14 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
       nbad += c > 3;
       p |= (c & 3u) << (2 * j);
     }
-  int phase = 0;  // 0: task e waits for its 13-mer lookup; 1: task e is extending
+  int phase = 0;  // 0: task e waits for its table lookup; 1: task e is extending
+  bool ext_miss = false;  // the task's extended-table pattern is absent: use the 13-mer path
   int32_t x = 0;
   uint32_t l = 0, u = 0, b = 0, s = 0, last_b = 0xFFFFFFFFu, nout = 0;
   uint4* outF = P.cand + (live ? 2 * P.task_off[r] : 0);
 
   auto next_task = [&]() -> bool {
+    ext_miss = false;
     if ((uint32_t)(e - 1) < k) return false;
     const uint32_t cn = V(e - 14), co = V(e - 1);
     nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
@@ -157,8 +161,11 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
       }
       if (x < 0) {
         finish();
+        // SMEM: once a task reaches the read start, every later task does too
+        // (its pattern is a substring of this one's), so none can pass the
+        // filter b < last_b = 0: the stream is done
         phase = 0;
-        if (!next_task()) done = true;
+        if (P.smem || !next_task()) done = true;
         continue;
       }
       c = V(x);
@@ -181,8 +188,23 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
     const uint4* a2 = nullptr;
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
+    uint32_t tlen = gs::kPrecalc, tidx = p;  // table lookup: pattern length and entry
     if (op == 1) {
-      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);
+      const uint2* tbl = P.X.kmer;
+      if (P.X.ext_len && !ext_miss && e >= (int32_t)P.X.ext_len) {
+        uint32_t q = p, ok = 1;
+        for (uint32_t j = gs::kPrecalc; j < P.X.ext_len; ++j) {
+          const uint32_t cj = V(e - 1 - (int32_t)j);
+          ok &= cj <= 3 ? 1u : 0u;
+          q |= (cj & 3u) << (2 * j);
+        }
+        if (ok) {
+          tlen = P.X.ext_len;
+          tidx = q;
+          tbl = P.X.kext;
+        }
+      }
+      a1 = reinterpret_cast<const uint4*>(tbl) + (tidx >> 1);
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
       have1 = r1 != 0xFFFFFFFFu;
@@ -199,18 +221,21 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
     if (a2) d2 = *a2;
 
     if (op == 1) {
-      const uint32_t il = (p & 1) ? d1.z : d1.x, iu = (p & 1) ? d1.w : d1.y;
+      const uint32_t il = (tidx & 1) ? d1.z : d1.x, iu = (tidx & 1) ? d1.w : d1.y;
       if (il > iu) {
-        if (!next_task()) done = true;
+        if (tlen > gs::kPrecalc)
+          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
+        else if (!next_task())
+          done = true;
       } else {
         l = il;
         u = iu;
         s = u - l + 1;
-        b = (uint32_t)e - gs::kPrecalc;
+        b = (uint32_t)e - tlen;
         if (prune_at((int32_t)b, s)) {
           if (!next_task()) done = true;
         } else {
-          x = e - gs::kPrecalc - 1;
+          x = (int32_t)b - 1;
           phase = 1;
         }
       }

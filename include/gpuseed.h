@@ -130,7 +130,8 @@ typedef struct {
   double t_read, t_upload, t_repack, t_table; /* seconds                      */
   double t_sa, t_write; /* gs_index_build: suffix sorting, file writing      */
   int sa_rounds;        /* gs_index_build: prefix-doubling rounds            */
-  int pad;
+  int kmer_len;         /* longest interval table on the device (13..16)     */
+  double t_ext;         /* gs_index_set_kmer_len: table build, seconds       */
 } gs_index_info;
 
 GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
@@ -140,6 +141,17 @@ GS_API int gs_index_get_info(const gs_index* idx, gs_index_info* info);
  * (4 u32 each: count, 96-bit bitmap) and 13-mer table entries (2 u32: l, u). */
 GS_API int gs_index_read_occ(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
 GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
+
+/* Adds (len 14..16) or drops (len 13) an extended interval table: the SA
+ * interval of every pattern of len bases, 8 * 4^len bytes of HBM (2.1 GB,
+ * 8.6 GB, 34.4 GB).  Seeding then answers a task's first len bases with one
+ * lookup where the reference takes the 13-mer table plus len - 13 backward
+ * steps (seed_gen.cu:2132-2174); it falls back to those steps when the
+ * pattern does not occur or its window holds a code above 3, so the output
+ * is unchanged.  No counterpart in the reference. */
+GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
+/* Entries [first, first+count) of the extended table (tests). */
+GS_API int gs_index_read_kext(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
 
 /* Builds the index of codes[0..n) (values 0..3) on the device: suffix array
  * by prefix doubling, then the same device structures gs_index_open makes.
