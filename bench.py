@@ -46,6 +46,28 @@ def cpu_model():
     return "unknown"
 
 
+def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len):
+    """Memory operations per read of the device algorithm, counted by the
+    oracle on the first `sample` reads: 13-mer table lookups, extended-table
+    lookups (8 bytes each) and occ entries (16 bytes each), with distance-1
+    pruning and the SMEM read-start stop; and the same count for the
+    reference's procedure (13-mer table, distance-1 pruning only)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    buf, offs, lens = arrays
+    n = min(sample, len(lens))
+    end = int(offs[n - 1] + lens[n - 1])
+    sub = (buf[:end], offs[:n], lens[:n])
+    ox = ob.Index(prefix)
+    c = ob.seed_arrays(ox, *sub, k, True, max_occ=max_occ, threads=threads, smem_stop=1,
+                       ext_len=ext_len if ext_len > 13 else 0)[4]
+    ox.close()
+    table = (c["w_tasks"] + c["w_ext"]) / n
+    occ = c["w_blocks96"] / n
+    return dict(reads=n, table_per_read=table, occ_per_read=occ, requests_per_read=table + occ,
+                bytes_per_read=8.0 * table + 16.0 * occ)
+
+
 def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
     """The oracle (C restatement of the reference path, distance-1 pruning,
     13-mer table built before timing) on the first `sample` reads of a batch;
@@ -214,6 +236,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="reads timed on the CPU (default 1,000,000; configuration 4: 20,000)")
     ap.add_argument("--seg-tasks", type=int, default=0, help="task ends per work item of long reads")
+    ap.add_argument("--work-sample", type=int, default=200000, help="reads whose work the oracle counts")
     ap.add_argument("--kmer-len", type=int, default=16,
                     help="interval table length: 13 is the reference's table, 14..16 add a longer one")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -329,6 +352,11 @@ def main():
         cpu["threads"] = threads
         log("cpu baseline: %.0f reads/s on %d threads; parity %d/%d reads differ"
             % (cpu["reads_per_s"], threads, cpu["mismatches"], cpu["reads"]))
+    wm = None
+    if not args.no_cpu:
+        wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len)
+        log("work model: %.1f table + %.1f occ requests per read (%.0f bytes)" %
+            (wm["table_per_read"], wm["occ_per_read"], wm["bytes_per_read"]))
     serving = None
     if serve_ref is not None:
         serving = serve_bench(g, X, serve_ref, k, max_occ, args)
@@ -339,25 +367,28 @@ def main():
     roofline = None
     extra = {}
     if cpu:
-        c = cpu["counters"]
-        n = cpu["reads"]
-        # extension kernel, algorithmic: one 8-byte table entry per task with a
-        # clean 13-base window, one 16-byte occ entry per distinct 96-base block
-        # touched by each backward step (exact distance-1 pruning)
-        req_per_read = (c["w_tasks"] + c["w_blocks96"]) / n
+        extra["parity"] = {"reads_checked": cpu["reads"], "mismatches": cpu["mismatches"],
+                           "bit_exact": cpu["bit_exact"]}
+    if wm:
+        # extension kernel, algorithmic: the device algorithm's table entries
+        # (8 bytes) and distinct 96-base occ entries per backward step (16
+        # bytes), counted by the oracle; W for the reference's procedure
+        # (13-mer table, distance-1 pruning) is reported beside it
+        req_per_read = wm["requests_per_read"]
         act = sum(st.n_table_reads + st.n_occ_reads for st in stats) / (len(stats) * args.batch)
-        bytes_per_read = (8.0 * c["w_tasks"] + 16.0 * c["w_blocks96"]) / n
-        achieved = bytes_per_read * args.batch / seed_s
+        achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
                     "frac": achieved / 8e12, "traffic": None,
                     "kernel": "seed_persistent", "kernel_ms": seed_s * 1e3,
+                    "bytes_per_read": wm["bytes_per_read"], "work_sample_reads": wm["reads"],
                     "random_access": {"requests_per_read": req_per_read,
                                       "issued_requests_per_read": act,
                                       "achieved_requests_per_s": req_per_read * args.batch / seed_s,
                                       "measured_ceiling_requests_per_s": r_max,
                                       "frac": req_per_read * args.batch / seed_s / r_max}}
-        extra["parity"] = {"reads_checked": n, "mismatches": cpu["mismatches"],
-                           "bit_exact": cpu["bit_exact"]}
+        if cpu:
+            c = cpu["counters"]
+            roofline["reference_procedure_requests_per_read"] = (c["w_tasks"] + c["w_blocks96"]) / cpu["reads"]
     mean_ms = elapsed / args.steps * 1e3
     out = {
         "metric": "SMEM reads/s (150bp,k=19,3.1Gbp) batched vs plain, bit-exact; p99 ms",

@@ -304,7 +304,7 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext;
 } work_t;
 
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
@@ -322,8 +322,13 @@ static void count_blocks(const or_index* X, uint32_t l, uint32_t u, work_t* w) {
   w->blocks96 += (!s1) + (!s2) - ((!s1 && !s2 && i1 / 96 == i2 / 96) ? 1 : 0);
 }
 
+/* One strand's candidates (find_seed_intervals_gpu's forward procedure,
+ * 2087-2264; the reverse strand runs it on reverse-complement codes).  The
+ * output is the same whatever prune, smem_stop and ext_len say; they only
+ * change which steps run and are counted (see or_opts). */
 static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t L, uint32_t k,
-                                  int prune, cand_t* out, uint32_t* A, work_t* w) {
+                                  int prune, int smem_stop, uint32_t ext_len, cand_t* out, uint32_t* A,
+                                  work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
@@ -339,23 +344,46 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       p |= c << (j << 1);
     }
     if (bad) continue;
-    ++w->tasks;
-    uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
-    uint32_t l = (uint32_t)iv, u = (uint32_t)(iv >> 32);
-    if (l > u) continue; /* 2145 */
-    uint32_t b = e - PRECALC;
-    int pruned = 0;
+    uint32_t l = 0, u = 0, b = 0;
+    int have = 0;
+    if (ext_len > PRECALC && e >= ext_len) {
+      int clean = 1;
+      for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
+      if (clean) { /* one lookup answers the first ext_len bases when they occur */
+        ++w->ext;
+        uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
+        l = (uint32_t)iv;
+        u = (uint32_t)(iv >> 32);
+        for (uint32_t j = PRECALC; j < ext_len && l <= u; ++j) step(X, &l, &u, v[e - 1 - j]);
+        if (l <= u) {
+          have = 1;
+          b = e - ext_len;
+        }
+      }
+    }
+    if (!have) {
+      ++w->tasks;
+      uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
+      l = (uint32_t)iv;
+      u = (uint32_t)(iv >> 32);
+      if (l > u) continue; /* 2145 */
+      b = e - PRECALC;
+    }
+    int pruned = 0, reached = 1;
     if (prune) {
       uint32_t s = u - l + 1;
       if (A[b] == s) pruned = 1;
       A[b] = s;
     }
-    for (int64_t x = (int64_t)e - PRECALC - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
+    for (int64_t x = (int64_t)b - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
       uint32_t c = v[x], l2 = l, u2 = u;
       if (c <= 3) count_blocks(X, l, u, w);
       step(X, &l2, &u2, c);
       ++w->steps;
-      if (l2 > u2) break; /* 2246 */
+      if (l2 > u2) { /* 2246 */
+        reached = 0;
+        break;
+      }
       l = l2;
       u = u2;
       b = (uint32_t)x; /* 2252-2255 */
@@ -367,6 +395,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     }
     if (pruned) continue;
     if (e - b >= k) out[n++] = (cand_t){b, e, l, u}; /* 2259-2264 */
+    if (smem_stop && reached) break; /* later tasks end at 0 too and fail the SMEM filter */
   }
   return n;
 }
@@ -421,12 +450,14 @@ static void phase_candidates(void* vctx, uint64_t i, int tid) {
   Ls->F = tmp;
   Ls->R = tmp + ntask;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_fwd(s[q]);
-  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, tmp + ntask, A, &C->work[tid]);
+  const int stop = C->o->smem && C->o->smem_stop;
+  const uint32_t ext = (uint32_t)C->o->ext_len;
+  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, tmp + ntask, A, &C->work[tid]);
   /* forward: decreasing end -> slot order (increasing end) */
   for (uint32_t q = 0; q < nf; ++q) Ls->F[q] = tmp[ntask + nf - 1 - q];
   Ls->nF = nf;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_rev(s[L - 1 - q]);
-  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, Ls->R, A, &C->work[tid]);
+  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, Ls->R, A, &C->work[tid]);
   for (uint32_t q = 0; q < nr; ++q) { /* virtual [b,e) -> reverse (L-e, L-b) with strand bits */
     cand_t c = Ls->R[q];
     Ls->R[q] = (cand_t){(L - c.e) | 0x80000000u, (L - c.b) | 0x80000000u, c.l, c.u};
@@ -709,6 +740,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
   res->read_pos_off[n] = res->n_pos;
   for (int t = 0; t < threads; ++t) {
     res->w_tasks += work[t].tasks;
+    res->w_ext += work[t].ext;
     res->w_steps += work[t].steps;
     res->w_blocks64 += work[t].blocks64;
     res->w_blocks96 += work[t].blocks96;

@@ -61,6 +61,15 @@ typedef struct {
                        interval is larger get no positions                        */
   int threads;      /* 0: all online CPUs                                        */
   int use_table;    /* 1: precompute the 13-mer table like 2887-2895            */
+  /* Work model of the device kernels (output unchanged, counters only):
+   * smem_stop: in SMEM mode a strand's tasks stop once one reaches the read
+   *            start (every later task would end there and be filtered out);
+   * ext_len:   14..16: a task with a clean ext_len-base window first looks
+   *            up the interval of that pattern (one lookup, counted in
+   *            w_ext) and extends from there, or falls back to the 13-mer
+   *            table when the pattern does not occur; 0 or 13: off. */
+  int smem_stop;
+  int ext_len;
 } or_opts;
 
 typedef struct {
@@ -89,6 +98,7 @@ typedef struct {
   uint64_t w_sa;          /* SA samples read                                  */
   uint64_t w_steps_unpruned; /* steps with no pruning at all                 */
   uint64_t n_short;       /* reads shorter than k (no tasks)                  */
+  uint64_t w_ext;         /* extended-table lookups (opts.ext_len)            */
 } or_result;
 
 /* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line

@@ -80,3 +80,30 @@ def test_seeds_unchanged_by_the_extended_table(g, tmp_path, length):
                 got = g.seed(X, reads, k=k, smem=smem, kernel=kernel).as_lists()
                 bad = [i for i in range(len(reads)) if got[i] != want[i]]
                 assert not bad, "read %d" % bad[0]
+
+
+@pytest.mark.parametrize("length", [13, 16])
+@pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
+def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k):
+    """The batched kernel's own memory-operation counters equal the oracle's
+    count of the same algorithm (distance-1 pruning, the SMEM read-start stop,
+    the extended table): 8-byte table lookups and 16-byte occ entries, for
+    reads held whole in the kernel's LDS window."""
+    rng = random.Random(7 + length)
+    text = repetitive_text(rng, 30000, n_units=8, unit_len=200, copies=6, div=0.02, tandem=6)
+    reads = sample_reads(rng, text, 600, 60, 160, sub=0.01, indel=0.002, n_frac=0.1)
+    prefix = build_index(tmp_path, text, 7)
+    ox = ob.Index(prefix)
+    arrays = g.pack_reads(reads)
+    _, _, _, _, c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1,
+                                   ext_len=length if length > 13 else 0)
+    with g.Index(prefix) as X:
+        if length > 13:
+            X.set_kmer_len(length)
+        b = g.Batch(X, arrays, k=k, smem=smem, kernel=1)
+        st = b.run()
+        b.close()
+    say("T=%d smem=%d: kernel %d table %d occ, oracle %d table %d occ" %
+        (length, smem, st.n_table_reads, st.n_occ_reads, c["w_tasks"] + c["w_ext"], c["w_blocks96"]))
+    assert st.n_table_reads == c["w_tasks"] + c["w_ext"]
+    assert st.n_occ_reads == c["w_blocks96"]

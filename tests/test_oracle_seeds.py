@@ -161,3 +161,28 @@ def test_mem_lookahead_crosses_reads_within_a_batch(built, tmp_path):
     assert as_tuples(got) == brute.mem(bi, [rev, rev], 20)
     assert got[0][0][4] == [2000]
     assert got[1][0][4] == [bi.sa[got[1][0][2] + 1]]
+
+
+def test_work_model_options_do_not_change_output(built, tmp_path):
+    """The device kernels' shortcuts (a strand stops once a task reaches the
+    read start in SMEM mode; an extended interval table answers the first
+    14..16 bases) change only the counted work, never the seeds."""
+    import gpuseed_amd
+    rp, qp = gpuseed_amd.preset(1)
+    ref = gpuseed_amd.synth_reference(rp)
+    qp.n_reads = 2000
+    buf, offs, lens, _, _ = gpuseed_amd.synth_reads(ref, qp)
+    prefix = os.path.join(str(tmp_path), "c1")
+    gpuseed_amd.build_index_host(ref, 7, prefix)
+    ox = ob.Index(prefix)
+    for smem, k in ((True, 19), (False, 20)):
+        base = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4)
+        for stop, ext in ((1, 0), (0, 16), (1, 14), (1, 16)):
+            got = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=stop, ext_len=ext)
+            for a, b in zip(base[:4], got[:4]):
+                assert (a == b).all(), (smem, stop, ext)
+            cb, cg = base[4], got[4]
+            if stop and smem:
+                assert cg["w_blocks96"] < cb["w_blocks96"]
+            if ext:
+                assert cg["w_ext"] > 0 and cg["w_steps"] < cb["w_steps"]
