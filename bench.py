@@ -237,6 +237,7 @@ def main():
                     help="reads timed on the CPU (default 1,000,000; configuration 4: 20,000)")
     ap.add_argument("--seg-tasks", type=int, default=0, help="task ends per work item of long reads")
     ap.add_argument("--work-sample", type=int, default=200000, help="reads whose work the oracle counts")
+    ap.add_argument("--reuse", action="store_true", help="one resident batch re-run every step")
     ap.add_argument("--kmer-len", type=int, default=16,
                     help="interval table length: 13 is the reference's table, 14..16 add a longer one")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -268,7 +269,7 @@ def main():
     rp, qp = g.preset(args.config)
     if args.ref_length:
         rp.length = args.ref_length
-    reuse = args.config == 4  # one resident batch, re-run every step
+    reuse = args.config == 4 or args.reuse  # one resident batch, re-run every step
     if not args.batch:
         args.batch = qp.n_reads if reuse else 1000000
     if not args.cpu_sample:

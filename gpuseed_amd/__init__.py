@@ -63,7 +63,8 @@ class IndexInfo(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
                 ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("seg_tasks", C.c_uint32),
-                ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("reserved", C.c_int)]
+                ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("split_tasks", C.c_uint16),
+                ("split_work", C.c_uint16)]
 
 
 class ServerOpts(C.Structure):
@@ -300,8 +301,13 @@ def pack_reads(reads):
     return np.frombuffer(b"".join(reads) or b"\0", dtype=np.uint8), offs, lens
 
 
-def make_opts(k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0, wg=0, walk_rows=0):
-    return Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks, wg, walk_rows, 0)
+NO_SPLIT = 0xFFFF
+
+
+def make_opts(k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0, wg=0, walk_rows=0,
+              split_tasks=0, split_work=0):
+    return Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks, wg, walk_rows, split_tasks,
+                split_work)
 
 
 def _arrays(arrays):
@@ -311,11 +317,11 @@ def _arrays(arrays):
 
 
 def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
-         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, opts=None):
+         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None):
     """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
     buf, offs, lens = _arrays(arrays if arrays is not None else pack_reads(reads))
     o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                walk_rows)
+                                                walk_rows, split_tasks, split_work)
     smem = bool(o.smem)
     res = Result()
     _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
@@ -368,12 +374,12 @@ class Batch:
     """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
 
     def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0,
-                 wg=0, walk_rows=0, opts=None):
+                 wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None):
         L = lib()
         _bind_batch(L)
         buf, offs, lens = _arrays(arrays)
         o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                    walk_rows)
+                                                    walk_rows, split_tasks, split_work)
         self.smem = bool(o.smem)
         self.h = C.c_void_p()
         _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),

@@ -54,22 +54,44 @@ struct PersistParams {
   uint32_t* nF;
   uint32_t* nR;
   uint4* seg_meta;           // per item of a segmented read: {emitted, first b, last b, flags}
+  uint32_t* seg_link;        // per item of a segmented read: first piece split off it, or ~0
   uint32_t* queue;           // work-queue counter
   uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads
+  // work splitting (dyn_cap 0: off)
+  uint4* dyn_items;          // pieces {read, strand, first task, tasks}
+  uint4* dyn_meta;           // per piece, as seg_meta
+  uint32_t* dyn_link;        // per piece: the next piece in task order, or ~0
+  uint32_t* dyn_ready;       // per piece: == epoch once its entry is written (cleared per launch)
+  uint32_t* dyn_ctl;         // [0] pieces reserved, [64] next piece to claim, [128] items running
+  uint4* wmeta;              // per whole-read stream (2 * read + strand) that was split
+  uint32_t* wlink;
+  uint32_t dyn_cap, split_work, split_tasks, epoch;
+  uint32_t split_below, pad2[3];  // split only while fewer items than this run (set at launch)
 };
 
-// Joins the segments of long reads into the per-read candidate layout.
+// Joins pieces (segments of long reads, split-off pieces) into the per-read
+// candidate layout.
 struct JoinParams {
   const uint32_t* long_reads;   // reads cut into segments
   const uint64_t* item0;        // per long read: index of its first item
   const uint32_t* lens;
   const uint64_t* task_off;
   const uint4* seg_meta;
+  const uint32_t* seg_link;
+  const uint4* dyn_items;
+  const uint4* dyn_meta;
+  const uint32_t* dyn_link;
+  const uint4* wmeta;
+  const uint32_t* wlink;
   uint4* cand;
   uint32_t* nF;
   uint32_t* nR;
   uint32_t n_long, k, smem, seg_tasks;
+  uint32_t n_reads, dyn_cap, split_tasks, pad;
 };
+
+constexpr uint32_t kSplitStream = 0xFFFFFFFFu;  // nF/nR of a whole-read stream left for join_whole
+constexpr uint32_t kSplitItemsPerLane = 4;      // work splitting only in batches this small
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
 constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start

@@ -2,25 +2,51 @@
 //
 // Same stream procedure and output as seed_streams in gs_seed.hip (the plain
 // one-stream-per-lane kernel), restructured for throughput on gfx950:
-//  * a fixed grid that fills the chip; every lane pulls (read, strand)
-//    streams from a device work queue (one atomic per wave refill, leader
-//    lane + ballot) and starts the next stream as soon as its current one
-//    ends, so no lane idles while its wave finishes a long stream;
+//  * a fixed grid that fills the chip; every lane pulls work items from a
+//    device queue (one atomic per wave refill, leader lane + ballot) and
+//    starts the next one as soon as its current one ends;
 //  * reads are pre-packed to 4-bit codes of both virtual strands by
 //    pack_codes, so a refill is a few 16-byte loads and LDS stores;
-//  * a lane's LDS slot holds its stream's codes (4 bits per base) and its
-//    size record (one AT per position), indexed directly in LDS;
+//  * a lane's LDS slot holds its item's codes (4 bits per base) and its size
+//    record (one AT per position), indexed directly in LDS;
 //  * the loop body is one memory operation (an interval-table entry, from
 //    the extended table when the index has one, or one or two occ entries)
-//    followed by a short processing step; the rare
-//    transitions (windows with a code > 3, N-transparent positions, reaching
-//    the read start) are resolved inside the processing step.
+//    followed by a short processing step; the rare transitions (windows with
+//    a code > 3, N-transparent positions, reaching the read start) are
+//    resolved inside the processing step;
+//  * work splitting: the cost of a stream is heavy-tailed (in young repeat
+//    families interval sizes shrink at every step, so distance-1 pruning
+//    never stops a task and each task walks far).  Once the main queue is
+//    exhausted and fewer items run than half the lanes, an item that has used
+//    split_work memory operations since it started or last split, and still
+//    has at least 2 * split_tasks task ends to do, hands half of them to an
+//    overflow queue, which idle lanes take (a piece starts without size
+//    records, so splitting while every lane is busy would only add work); the join kernels put the pieces back in task order and settle the
+//    SMEM filter at their boundaries, as for the segments of long reads.
+//    A split keeps the upper half of the remaining task ends in the lane
+//    (with its size records) and pushes the lower half as one piece, which
+//    may split again; pieces are chained in task order.  Lanes never wait
+//    for one another: a piece is written by the lane that reserved it in the
+//    same iteration, a lane that claimed a piece whose entry is not visible
+//    yet checks again next iteration, and a lane leaves once the main queue
+//    is exhausted, no piece is left to claim and no item is running anywhere
+//    (so no piece can appear any more).
 #include "gs_internal.h"
 
 namespace {
 
 __device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
   return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
+}
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename AT, int LMAX, int BS>
@@ -39,36 +65,57 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   const uint32_t L2_0 = P.X.L2[0], L2_1 = P.X.L2[1], L2_2 = P.X.L2[2], L2_3 = P.X.L2[3];
   const uint32_t primary = P.X.primary;
   const uint64_t n_items = P.n_items;
+  // three control words, 256 bytes apart (separate cache lines and channels)
+  uint32_t* const reserved = P.dyn_ctl;          // pieces reserved by splitting lanes
+  uint32_t* const next_piece = P.dyn_ctl + 64;
+  uint32_t* const running = P.dyn_ctl + 128;     // items being processed
 
-  bool have = false, finished = false;
+  bool have = false, finished = false, main_done = false;
+  int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
   uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0, p = 0, nbad = 0;
   uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
   int32_t e = 0, x = 0, e_min = 0, w0 = 0;
-  uint32_t t0 = 0, first_b = 0, flags = 0;
+  uint32_t t0 = 0, first_b = 0, flags = 0, work = 0;
+  uint32_t pending = kNone;                                // claimed piece not visible yet
+  uint32_t started = 0, ended = 0;                          // items this iteration (for `running`)
+  uint32_t poll_wait = 0;                                   // wave-uniform: iterations to the next poll
+  uint32_t backoff = 1;                                     // wave-uniform: sleeps while all lanes wait
+  uint32_t link = kNone;                                   // first piece split off this item
   uint64_t item = 0;
   int phase = 0;
   bool ext_miss = false;  // the task's extended-table pattern is absent: use the 13-mer path
   uint4* out = nullptr;
-  const uint4* gcodes = nullptr;  // the stream's codes in device memory (segments only)
+  const uint4* gcodes = nullptr;  // the stream's codes in device memory
   uint64_t n_table = 0, n_occ = 0;
 
   // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
-  // a segment's extension may run left of it and then reads device memory
-  auto code = [&](int32_t i) -> uint32_t {
+  // an extension may run left of it and then reads device memory
+  auto code = [&](int32_t i) __attribute__((always_inline)) -> uint32_t {
     const int32_t j = i - w0;
     if (j >= 0) return (lds[sbase + (j >> 1)] >> ((j & 1) << 2)) & 15u;
     const uint32_t w = reinterpret_cast<const uint32_t*>(gcodes + (i >> 5))[(i >> 3) & 3];
     return (w >> ((i & 7) << 2)) & 15u;
   };
 
-  auto end_stream = [&]() {
-    if (gcodes)
-      P.seg_meta[item] = make_uint4(nout, first_b, last_b, flags);
-    else
+  auto end_stream = [&]() __attribute__((always_inline)) {
+    const uint4 meta = make_uint4(nout, first_b, last_b, flags);
+    if (kind == 2) {
+      P.dyn_meta[item] = meta;
+      P.dyn_link[item] = link;
+    } else if (kind == 1) {
+      P.seg_meta[item] = meta;
+      if (P.dyn_cap) P.seg_link[item] = link;  // allocated only with splitting on
+    } else if (link != kNone) {  // a whole read's stream that was split: joined later
+      P.wmeta[2 * r + strand] = meta;
+      P.wlink[2 * r + strand] = link;
+      (strand ? P.nR : P.nF)[r] = gs::kSplitStream;
+    } else {
       (strand ? P.nR : P.nF)[r] = nout;
+    }
     have = false;
+    ++ended;  // folded into `running` once per wave and iteration
   };
-  auto advance = [&]() -> bool {
+  auto advance = [&]() __attribute__((always_inline)) -> bool {
     if (e - 1 < e_min) return false;
     const uint32_t cn = code(e - 14), co = code(e - 1);
     nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
@@ -76,10 +123,37 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     --e;
     return true;
   };
-  // next task with a clean 13-base window, or end of the stream
-  auto next_task = [&]() {
+  // Hands the lower half of the task ends still to do (e-1 .. e_min) to the
+  // overflow queue as one piece and keeps the upper half, whose size records
+  // this lane holds; the piece is linked right after this item's own tasks
+  // (ahead of pieces split off earlier, which cover later task ends).  False
+  // if the queue is full (then the item goes on whole).
+  auto split = [&]() __attribute__((always_inline)) -> bool {
+    const uint32_t n_rem = (uint32_t)(e - e_min);
+    const uint32_t n_push = n_rem / 2;
+    if (__hip_atomic_load(reserved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= P.dyn_cap) return false;
+    const uint32_t id = atomicAdd(reserved, 1u);
+    if (id >= P.dyn_cap) return false;  // reserved past the end: nothing to write, nobody claims it
+    const uint32_t t_first = L - (uint32_t)(e_min + (int32_t)n_push - 1);  // task index of end e_min + n_push - 1
+    P.dyn_items[id] = make_uint4(r, strand, t_first, n_push);
+    P.dyn_link[id] = link;  // initial value: the piece's own successor
+    __threadfence();
+    store_release(P.dyn_ready + id, P.epoch);
+    link = id;
+    e_min += (int32_t)n_push;
+    work = 0;
+    return true;
+  };
+  // next task with a clean 13-base window, or end of the item
+  auto next_task = [&]() __attribute__((always_inline)) {
     phase = 0;
     ext_miss = false;
+    // split only in the tail: the main queue is exhausted and fewer items run
+    // than half the lanes, so pieces find idle lanes instead of adding work
+    if (P.dyn_cap && work > P.split_work && e - e_min >= 2 * (int32_t)P.split_tasks &&
+        __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
+        __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
+      split();
     do {
       if (!advance()) {
         end_stream();
@@ -89,7 +163,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   };
   // size record at pos (positions left of the window keep none: pruning is
   // only an early exit, so a missing record never changes the output)
-  auto prune_at = [&](int32_t pos, uint32_t size) -> bool {
+  auto prune_at = [&](int32_t pos, uint32_t size) __attribute__((always_inline)) -> bool {
     const int32_t j = pos - w0;
     if (j < 0) return false;
     const uint32_t sv = size >= SAT ? SAT : size;
@@ -97,7 +171,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     A[j] = (AT)sv;
     return false;
   };
-  auto finish_task = [&]() {
+  auto finish_task = [&]() __attribute__((always_inline)) {
     const bool emit = (uint32_t)e - b >= k && (!P.smem || b < last_b);
     if (emit) {
       if (strand == 0)
@@ -106,14 +180,14 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         out[ntask + t0 + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
       ++nout;
     }
-    if (!(flags & 2u)) {  // the segment's first finished task
+    if (!(flags & 2u)) {  // the item's first finished task
       first_b = b;
       flags = 2u | (emit ? 1u : 0u);
     }
     last_b = b;
   };
   // phase 1: settle x on the next position with a base code
-  auto prepare_x = [&]() {
+  auto prepare_x = [&]() __attribute__((always_inline)) {
     while (true) {
       if (x < 0) {
         finish_task();
@@ -136,31 +210,19 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       --x;
     }
   };
-  auto start_stream = [&](uint64_t it) {
-    item = it;
-    const uint64_t v = P.items[it];
-    r = (uint32_t)v;
-    strand = (uint32_t)(v >> 32) & 1u;
-    const uint32_t seg = (uint32_t)(v >> 33);
-    L = P.lens[r];
-    if (L < k) {
-      (strand ? P.nR : P.nF)[r] = 0;
-      return;
-    }
+  // Loads task ends e_hi .. e_hi - tcount + 1 of stream (r, strand) into the
+  // lane: the LDS code window [w0, top) covers every task's 13-base start.
+  auto start_item = [&](uint32_t tstart, uint32_t tcount, uint32_t link0) __attribute__((always_inline)) {
     ntask = L - k + 1;
     const uint32_t nch = (L + 31) >> 5;
     const uint4* src = P.codes + P.code_off[r] + strand * nch;
-    uint32_t top, tcount;
-    if (L <= (uint32_t)LMAX) {  // the whole stream in the window
-      gcodes = nullptr;
-      t0 = 0;
-      tcount = ntask;
+    uint32_t top;
+    t0 = tstart;
+    gcodes = src;
+    if (L <= (uint32_t)LMAX) {  // the whole stream fits the window
       top = nch << 5;
       w0 = 0;
-    } else {                    // one segment of task ends
-      gcodes = src;
-      t0 = seg * P.seg_tasks;
-      tcount = min(P.seg_tasks, ntask - t0);
+    } else {
       top = ((L - t0) + 31u) & ~31u;
       w0 = top > (uint32_t)LMAX ? (int32_t)(top - LMAX) : 0;
     }
@@ -174,6 +236,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     last_b = 0xFFFFFFFFu;
     first_b = 0xFFFFFFFFu;
     flags = 0;
+    work = 0;
+    link = link0;  // set before the item can end (a piece whose windows all hold an N ends here)
     e = (int32_t)(L - t0);
     e_min = e - (int32_t)tcount + 1;
     p = 0;
@@ -184,13 +248,106 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       p |= (c & 3u) << (2 * j);
     }
     have = true;
+    ++started;
     phase = 0;
-    if (nbad != 0) next_task();
+    ext_miss = false;
+    if (nbad != 0) next_task();  // no work done yet, so this never splits
+  };
+  auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
+    item = it;
+    const uint64_t v = P.items[it];
+    r = (uint32_t)v;
+    strand = (uint32_t)(v >> 32) & 1u;
+    const uint32_t seg = (uint32_t)(v >> 33);
+    L = P.lens[r];
+    if (L < k) {
+      (strand ? P.nR : P.nF)[r] = 0;
+      return;
+    }
+    if (L <= (uint32_t)LMAX) {
+      kind = 0;
+      start_item(0, L - k + 1, kNone);
+    } else {
+      kind = 1;
+      const uint32_t ts = seg * P.seg_tasks;
+      start_item(ts, min(P.seg_tasks, L - k + 1 - ts), kNone);
+    }
+  };
+  auto start_piece = [&](uint32_t id) __attribute__((always_inline)) {
+    const uint4 d = P.dyn_items[id];
+    const uint32_t lk = P.dyn_link[id];
+    item = id;
+    kind = 2;
+    r = d.x;
+    strand = d.y;
+    L = P.lens[r];
+    start_item(d.z, d.w, lk);  // pieces split off this one go in front of its successor
+  };
+  // items started minus items ended in this iteration, one atomic per wave;
+  // a split pushes its pieces (and fences) before its item ends
+  auto fold_running = [&]() __attribute__((always_inline)) {
+    if (__ballot(started != ended) == 0) {
+      started = ended = 0;
+      return;
+    }
+    int d = (int)started - (int)ended;
+    started = ended = 0;
+    for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+    if (lane == 0 && d) atomicAdd(running, (uint32_t)d);
+  };
+  // overflow queue, polled by one lane per wave: claims pieces for the
+  // wave's idle lanes with one CAS, or finds that none can come any more.
+  // Relaxed loads only (an early exit is always safe: a lane that splits
+  // its item takes its own pieces if nobody else does).
+  auto poll_pieces = [&]() __attribute__((always_inline)) {
+    const bool idle = !have && !finished && main_done && pending == kNone;
+    const uint64_t idlemask = __ballot(idle);
+    if (!idlemask) return;
+    if (poll_wait) {
+      --poll_wait;
+      return;
+    }
+    const int leader = __ffsll((unsigned long long)idlemask) - 1;
+    uint32_t got = 0, first = 0, none = 0;
+    if (lane == leader) {
+      const uint32_t nx = __hip_atomic_load(next_piece, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t rs = __hip_atomic_load(reserved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t lim = rs < P.dyn_cap ? rs : P.dyn_cap;
+      if (nx < lim) {
+        const uint32_t want = min((uint32_t)__popcll(idlemask), lim - nx);
+        if (atomicCAS(next_piece, nx, nx + want) == nx) {
+          got = want;
+          first = nx;
+        }
+      } else if (__hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        none = 1;  // nothing to claim and no item anywhere that could still split
+      }
+    }
+    got = __shfl(got, leader);
+    first = __shfl(first, leader);
+    none = __shfl(none, leader);
+    if (idle) {
+      const uint32_t rank = (uint32_t)__popcll(idlemask & ((1ull << lane) - 1ull));
+      if (rank < got)
+        pending = first + rank;
+      else if (none)
+        finished = true;
+    }
+    poll_wait = got ? 0u : 256u;  // waves with busy lanes poll rarely
+  };
+  // a claimed piece starts once its entry is visible
+  auto take_pending = [&]() __attribute__((always_inline)) {
+    if (pending == kNone || have) return;
+    if (__hip_atomic_load(P.dyn_ready + pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != P.epoch) return;
+    __threadfence();
+    const uint32_t id = pending;
+    pending = kNone;
+    start_piece(id);
   };
 
   while (true) {
-    // lanes without a stream take the next items of the queue
-    const bool need = !have && !finished;
+    // lanes without an item take the next one of the main queue
+    const bool need = !have && !finished && !main_done;
     const uint64_t needmask = __ballot(need);
     if (needmask) {
       const int leader = __ffsll((unsigned long long)needmask) - 1;
@@ -199,17 +356,29 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       base = __shfl(base, leader);
       if (need) {
         const uint64_t it = (uint64_t)base + __popcll(needmask & ((1ull << lane) - 1ull));
-        if (it >= n_items)
-          finished = true;
-        else
-          start_stream(it);
+        if (it >= n_items) {
+          main_done = true;
+          if (!P.dyn_cap) finished = true;
+        } else {
+          start_main(it);
+        }
       }
+    }
+    if (P.dyn_cap) {
+      poll_pieces();
+      take_pending();
     }
     const int op = have ? (phase == 0 ? 1 : 2) : 0;
     if (__ballot(op != 0) == 0) {
+      if (P.dyn_cap) fold_running();
       if (__ballot(!finished) == 0) break;
+      // the whole wave waits for pieces: back off (about 3 us per sleep, up to 16)
+      for (uint32_t i = 0; i < backoff; ++i) __builtin_amdgcn_s_sleep(127);
+      backoff = backoff < 16 ? 2 * backoff : 16;
+      poll_wait = 0;
       continue;
     }
+    backoff = 1;
     // one memory operation per lane
     const uint4* a1 = nullptr;
     const uint4* a2 = nullptr;
@@ -233,6 +402,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       }
       a1 = reinterpret_cast<const uint4*>(tbl) + (tidx >> 1);
       ++n_table;
+      ++work;
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
       have1 = r1 != 0xFFFFFFFFu;
@@ -243,6 +413,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       a1 = P.X.occ + (size_t)b2 * 4 + cx;
       if (have1 && b1 != b2) a2 = P.X.occ + (size_t)b1 * 4 + cx;
       n_occ += a2 ? 2 : 1;
+      work += a2 ? 2 : 1;
     }
     uint4 d1 = make_uint4(0, 0, 0, 0);
     if (a1) d1 = *a1;
@@ -290,6 +461,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         }
       }
     }
+    if (P.dyn_cap) fold_running();
   }
   if (P.counters) {
     // wave totals, one atomic per wave
@@ -302,6 +474,73 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)n_occ);
     }
   }
+}
+
+// One piece of a stream (its task ends from t0 on, `m` = {candidates,
+// first finished b, last finished b, flags}) moved next to the stream's
+// earlier pieces; K candidates so far, LB = left end of the last finished
+// task so far.  A piece filters its own tasks in the lane, but its first
+// finished task had no predecessor there: it survives iff its left end is
+// smaller than LB (finish_task, 991-1018).  Candidates only move toward the
+// region's end they are packed against, in an order that reads every source
+// before it can be overwritten.
+__device__ __forceinline__ void join_piece(uint4* out, uint32_t ntask, uint32_t strand, uint32_t t0, uint4 m,
+                                           uint32_t smem, uint32_t& K, uint32_t& LB) {
+  const uint32_t drop = (smem && (m.w & 1u) && !(m.y < LB)) ? 1u : 0u;
+  for (uint32_t i = drop; i < m.x; ++i) {
+    if (strand == 0)
+      out[ntask - 1 - K - (i - drop)] = out[ntask - 1 - t0 - i];
+    else
+      out[ntask + K + (i - drop)] = out[ntask + t0 + i];
+  }
+  K += m.x - drop;
+  if (m.w & 2u) LB = m.z;
+}
+
+// the pieces linked after an item, in task order (at most dyn_cap of them)
+__device__ __forceinline__ void join_chain(const gs::JoinParams& J, uint4* out, uint32_t ntask, uint32_t strand,
+                                           uint32_t link, uint32_t& K, uint32_t& LB) {
+  for (uint32_t guard = 0; link != kNone && guard < J.dyn_cap; ++guard) {
+    join_piece(out, ntask, strand, J.dyn_items[link].z, J.dyn_meta[link], J.smem, K, LB);
+    link = J.dyn_link[link];
+  }
+}
+
+// Streams of long reads (one thread per stream): segments in order, each
+// followed by the chain of pieces split off it.
+__global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (t >= 2ull * J.n_long) return;
+  const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
+  const uint32_t L = J.lens[r];
+  const uint32_t ntask = L - J.k + 1;
+  const uint32_t S = J.seg_tasks;
+  const uint32_t nseg = (ntask + S - 1) / S;
+  uint4* out = J.cand + 2 * J.task_off[r];
+  const uint64_t i0 = J.item0[t >> 1] + strand;
+  uint32_t K = 0, LB = 0xFFFFFFFFu;
+  for (uint32_t g = 0; g < nseg; ++g) {
+    const uint64_t it = i0 + 2 * (uint64_t)g;
+    join_piece(out, ntask, strand, g * S, J.seg_meta[it], J.smem, K, LB);
+    if (J.dyn_cap) join_chain(J, out, ntask, strand, J.seg_link[it], K, LB);
+  }
+  (strand ? J.nR : J.nF)[r] = K;
+}
+
+// Whole-read streams that were split (one thread per stream of the batch;
+// the others were final already).
+__global__ void __launch_bounds__(256) join_whole(gs::JoinParams J) {
+  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (t >= 2ull * J.n_reads) return;
+  const uint32_t r = (uint32_t)(t >> 1), strand = (uint32_t)(t & 1);
+  uint32_t* const n_out = strand ? J.nR : J.nF;
+  if (n_out[r] != gs::kSplitStream) return;
+  const uint32_t ntask = J.lens[r] - J.k + 1;
+  uint4* out = J.cand + 2 * J.task_off[r];
+  uint32_t K = 0, LB = 0xFFFFFFFFu;
+  join_piece(out, ntask, strand, 0, J.wmeta[t], J.smem, K, LB);
+  join_chain(J, out, ntask, strand, J.wlink[t], K, LB);
+  n_out[r] = K;
 }
 
 // 4-bit codes of both virtual strands: forward v[i] = code of byte i, reverse
@@ -354,41 +593,6 @@ __global__ void __launch_bounds__(256) pack_codes_long(const uint8_t* __restrict
   }
 }
 
-// Joins the segments of a long read's stream (one thread per stream): moves
-// each segment's candidates next to the previous segment's and settles the
-// SMEM filter at segment boundaries.  A segment filters its own tasks in the
-// lane, but its first finished task has no predecessor there: it survives iff
-// its left end is smaller than the last finished task's of earlier segments
-// (finish_task of the persistent kernel, 991-1018).  Candidates only move
-// toward the read's end of the region, in an order that reads every source
-// before it can be overwritten.
-__global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
-  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
-  if (t >= 2ull * J.n_long) return;
-  const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
-  const uint32_t L = J.lens[r];
-  const uint32_t ntask = L - J.k + 1;
-  const uint32_t S = J.seg_tasks;
-  const uint32_t nseg = (ntask + S - 1) / S;
-  uint4* out = J.cand + 2 * J.task_off[r];
-  const uint4* meta = J.seg_meta + J.item0[t >> 1] + strand;
-  uint32_t K = 0, LB = 0xFFFFFFFFu;
-  for (uint32_t g = 0; g < nseg; ++g) {
-    const uint4 m = meta[2 * (uint64_t)g];
-    const uint32_t t0 = g * S;
-    const uint32_t drop = (J.smem && (m.w & 1u) && !(m.y < LB)) ? 1u : 0u;
-    for (uint32_t i = drop; i < m.x; ++i) {
-      if (strand == 0)
-        out[ntask - 1 - K - (i - drop)] = out[ntask - 1 - t0 - i];
-      else
-        out[ntask + K + (i - drop)] = out[ntask + t0 + i];
-    }
-    K += m.x - drop;
-    if (m.w & 2u) LB = m.z;
-  }
-  (strand ? J.nR : J.nF)[r] = K;
-}
-
 __global__ void zero_words(uint32_t* __restrict__ p, uint32_t n) {
   const uint32_t i = blockIdx.x * 64 + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -408,7 +612,8 @@ int grid_t(unsigned* grid) {
 }
 
 template <typename AT, int LMAX, int BS>
-int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
+int launch_t(gs::PersistParams P, unsigned grid, hipStream_t st) {
+  P.split_below = grid * (uint32_t)BS / 2;  // half the lanes of the launch
   hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, P);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
@@ -430,9 +635,14 @@ int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t
 }
 
 int launch_join(const JoinParams& J, hipStream_t st) {
-  if (!J.n_long) return GS_OK;
-  const uint64_t threads = 2ull * J.n_long;
-  hipLaunchKernelGGL(join_segments, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, J);
+  if (J.n_long) {
+    const uint64_t threads = 2ull * J.n_long;
+    hipLaunchKernelGGL(join_segments, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, J);
+  }
+  if (J.dyn_cap && J.n_reads) {
+    const uint64_t threads = 2ull * J.n_reads;
+    hipLaunchKernelGGL(join_whole, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, J);
+  }
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
@@ -453,6 +663,10 @@ int persistent_grid(int variant, int wg, unsigned* grid) {
 int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st) {
   if (!grid) return GS_EINVAL;
   hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, P.queue, 1u);
+  if (P.dyn_cap) {  // an empty overflow queue: no piece reserved, claimed or written
+    hipLaunchKernelGGL(zero_words, dim3(3), dim3(64), 0, st, P.dyn_ctl, 192u);
+    hipLaunchKernelGGL(zero_words, dim3((P.dyn_cap + 63) / 64), dim3(64), 0, st, P.dyn_ready, P.dyn_cap);
+  }
   if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 4u);
   constexpr int L = kPersistLmax;
   switch (variant * 1000 + (wg ? wg : 128)) {

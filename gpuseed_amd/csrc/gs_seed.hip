@@ -32,6 +32,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gs_internal.h"
@@ -626,8 +627,9 @@ struct gs_batch {
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
   DevBuf codes, code_off, queue, counters;  // persistent kernel
   DevBuf items, seg_meta, long_reads, item0;
+  DevBuf seg_link, dyn_items, dyn_meta, dyn_link, dyn_ready, dyn_ctl, wmeta, wlink;  // work splitting
   uint64_t code_chunks = 0, n_items = 0;
-  uint32_t n_long = 0, seg_tasks = 64;
+  uint32_t n_long = 0, seg_tasks = 64, dyn_cap = 0;
   uint64_t n_seeds = 0, n_pos = 0;
   bool ran = false;
   hipEvent_t ev[7] = {};
@@ -673,6 +675,8 @@ int gs::plan_check(const gs_opts* o) {
   if (o->wg != 0 && o->wg != 64 && o->wg != 128 && o->wg != 256) return GS_EINVAL;
   if (o->walk_rows != 0 && o->walk_rows != 1 && o->walk_rows != 2 && o->walk_rows != 4) return GS_EINVAL;
   if (o->kernel < 0 || o->kernel > 2) return GS_EINVAL;
+  if (o->split_tasks > 64 && o->split_tasks != 0xFFFF) return GS_EINVAL;
+  if (o->split_work == 0xFFFF) return GS_EINVAL;
   return GS_OK;
 }
 
@@ -787,6 +791,19 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   B->n_seeds = B->n_pos = 0;
   B->ran = false;
   const uint64_t n = P.n_reads, nr1 = n + 1;
+  // Work splitting pays where the tail of a launch is a large share of it:
+  // batches of at most kSplitItemsPerLane work items per lane of the
+  // persistent grid (in larger batches the tail is amortised and keeping
+  // the overflow queue live costs more than it saves).
+  bool split_on = P.opts.split_tasks != 0xFFFF && P.opts.kernel != 0;
+  if (split_on) {
+    unsigned grid = 0;
+    const int wg = P.opts.wg ? P.opts.wg : 128;
+    if (gs::persistent_grid(P.opts.kernel == 2 ? 1 : 0, wg, &grid) != GS_OK || !grid ||
+        B->n_items > (uint64_t)gs::kSplitItemsPerLane * grid * (uint64_t)wg)
+      split_on = false;
+  }
+  B->dyn_cap = split_on ? (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, B->n_items)) : 0u;
   if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
       B->items.ensure(8 * (B->n_items + 1)) || B->long_reads.ensure(4 * (B->n_long + 1)) ||
       B->item0.ensure(8 * (B->n_long + 1)) || B->seg_meta.ensure(B->n_long ? 16 * B->n_items : 16) ||
@@ -795,7 +812,12 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
       B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
       B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
       B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
-      B->queue.ensure(16) || B->counters.ensure(32))
+      B->queue.ensure(16) || B->counters.ensure(32) ||
+      B->seg_link.ensure(B->n_long ? 4 * B->n_items : 16) ||
+      (B->dyn_cap &&
+       (B->dyn_items.ensure(16ull * B->dyn_cap) || B->dyn_meta.ensure(16ull * B->dyn_cap) ||
+        B->dyn_link.ensure(4ull * B->dyn_cap) || B->dyn_ready.ensure(4ull * B->dyn_cap) ||
+        B->dyn_ctl.ensure(1024) || B->wmeta.ensure(32 * nr1) || B->wlink.ensure(8 * nr1))))
     return GS_ENOMEM;
   hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
@@ -919,8 +941,20 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   PP.nF = S.nF;
   PP.nR = S.nR;
   PP.seg_meta = B->seg_meta.as<uint4>();
+  PP.seg_link = B->seg_link.as<uint32_t>();
   PP.queue = B->queue.as<uint32_t>();
   PP.counters = B->counters.as<uint64_t>();
+  PP.dyn_items = B->dyn_items.as<uint4>();
+  PP.dyn_meta = B->dyn_meta.as<uint4>();
+  PP.dyn_link = B->dyn_link.as<uint32_t>();
+  PP.dyn_ready = B->dyn_ready.as<uint32_t>();
+  PP.dyn_ctl = B->dyn_ctl.as<uint32_t>();
+  PP.wmeta = B->wmeta.as<uint4>();
+  PP.wlink = B->wlink.as<uint32_t>();
+  PP.dyn_cap = B->dyn_cap;
+  PP.split_tasks = B->opts.split_tasks ? B->opts.split_tasks : 8u;
+  PP.split_work = 64u * (B->opts.split_work ? B->opts.split_work : 32u);
+  PP.epoch = 1;  // dyn_ready is cleared by launch_persistent before every launch
   gs::JoinParams J;
   J.long_reads = B->long_reads.as<uint32_t>();
   J.item0 = B->item0.as<uint64_t>();
@@ -934,6 +968,16 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   J.k = S.k;
   J.smem = S.smem;
   J.seg_tasks = B->seg_tasks;
+  J.seg_link = PP.seg_link;
+  J.dyn_items = PP.dyn_items;
+  J.dyn_meta = PP.dyn_meta;
+  J.dyn_link = PP.dyn_link;
+  J.wmeta = PP.wmeta;
+  J.wlink = PP.wlink;
+  J.n_reads = S.n_reads;
+  J.dyn_cap = PP.dyn_cap;
+  J.split_tasks = PP.split_tasks;
+  J.pad = 0;
   int rc = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
                                  B->codes.as<uint4>(), st);
   if (!rc) rc = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, B->opts.wg, pgrid, st);

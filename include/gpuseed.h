@@ -182,8 +182,14 @@ typedef struct {
                        (0 = 64)                                               */
   uint16_t wg;        /* batched kernels: workgroup size 64, 128 or 256 (0 = 128) */
   uint16_t walk_rows; /* suffix-array walk: rows per thread 1, 2 or 4 (0 = 1) */
-  int reserved;       /* 0. The output depends on none of the tuning fields
-                       (kernel, seg_tasks, wg, walk_rows)                     */
+  uint16_t split_tasks; /* batched kernels, batches of at most 4 work items per
+                       lane: in the tail of a launch an item that has used
+                       split_work memory operations hands half of its task
+                       ends to an idle lane while at least 2 * split_tasks
+                       remain, 1..64 (0 = 8; 0xFFFF: no splitting)           */
+  uint16_t split_work;  /* in units of 64 memory operations, 1..0xFFFE (0 = 32).
+                       The output depends on none of the tuning fields
+                       (kernel, seg_tasks, wg, walk_rows, split_*)           */
 } gs_opts;
 
 typedef struct {

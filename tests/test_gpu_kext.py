@@ -100,7 +100,7 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k):
     with g.Index(prefix) as X:
         if length > 13:
             X.set_kmer_len(length)
-        b = g.Batch(X, arrays, k=k, smem=smem, kernel=1)
+        b = g.Batch(X, arrays, k=k, smem=smem, kernel=1, split_tasks=g.NO_SPLIT)
         st = b.run()
         b.close()
     say("T=%d smem=%d: kernel %d table %d occ, oracle %d table %d occ" %

@@ -89,7 +89,8 @@ def test_config4_mix_batched_equals_plain_and_oracle(g, mix, smem):
         plain = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=0)
         batched = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1)
         b16 = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=2, seg_tasks=40)
-    for out in (batched, b16):
+        nosplit = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1, split_tasks=g.NO_SPLIT)
+    for out in (batched, b16, nosplit):
         assert np.array_equal(out.seeds, plain.seeds)
         assert np.array_equal(out.pos, plain.pos)
         assert np.array_equal(out.read_seed_off, plain.read_seed_off)

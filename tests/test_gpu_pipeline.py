@@ -71,9 +71,24 @@ def test_tuning_fields_do_not_change_output(g, case, smem, k):
         assert not bad, bad[:3]
 
 
+@pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
+def test_work_splitting_does_not_change_output(g, case, smem, k):
+    """Items hand their remaining task ends to idle lanes after split_work
+    memory operations; very small budgets make nearly every stream split."""
+    with g.Index(case["prefix"]) as X:
+        ref = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=0, max_occ=50)
+        for kernel, split_tasks, split_work in ((1, 1, 1), (1, 3, 1), (2, 8, 2), (1, 64, 1), (1, 8, 32),
+                                                (2, 5, 4)):
+            say("smem=%d kernel=%d split_tasks=%d split_work=%d" % (smem, kernel, split_tasks, split_work))
+            out = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=kernel, max_occ=50,
+                         split_tasks=split_tasks, split_work=split_work, seg_tasks=48)
+            assert same(out, ref), (kernel, split_tasks, split_work,
+                                    first_diff(out, ref, case["arrays"][2]))
+
+
 def test_bad_tuning_fields_are_refused(g, case):
     with g.Index(case["prefix"]) as X:
-        for kw in (dict(wg=96), dict(walk_rows=3), dict(kernel=3)):
+        for kw in (dict(wg=96), dict(walk_rows=3), dict(kernel=3), dict(split_tasks=65), dict(split_work=0xFFFF)):
             with pytest.raises(g.GsError):
                 g.seed(X, arrays=case["arrays"], k=19, smem=True, **kw)
 
