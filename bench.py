@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the MI355X seeder on one GPU.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--config 3] ...
+    python bench.py [--gpus 1] [--steps 4] [--warmup 1] [--config 3] ...
 
 Workload (default --config 3): a 3.1 Gbp synthetic reference (41 % GC, 45 %
 interspersed repeats, 3 % tandem repeats, 5 % segmental duplications) indexed
@@ -46,7 +46,7 @@ def cpu_model():
     return "unknown"
 
 
-def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len):
+def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True):
     """Memory operations per read of the device algorithm, counted by the
     oracle on the first `sample` reads: 13-mer table lookups, extended-table
     lookups (8 bytes each) and occ entries (16 bytes each), with distance-1
@@ -59,7 +59,7 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len):
     end = int(offs[n - 1] + lens[n - 1])
     sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
-    c = ob.seed_arrays(ox, *sub, k, True, max_occ=max_occ, threads=threads, smem_stop=1,
+    c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
                        ext_len=ext_len if ext_len > 13 else 0)[4]
     ox.close()
     table = (c["w_tasks"] + c["w_ext"]) / n
@@ -68,7 +68,7 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len):
                 bytes_per_read=8.0 * table + 16.0 * occ)
 
 
-def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
+def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ, smem=True):
     """The oracle (C restatement of the reference path, distance-1 pruning,
     13-mer table built before timing) on the first `sample` reads of a batch;
     its output is also compared bit for bit with the GPU's for those reads."""
@@ -82,10 +82,10 @@ def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ):
     sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
     t = time.time()
-    ob.seed_arrays(ox, buf[:int(lens[0])], offs[:1], lens[:1], k, True, max_occ=max_occ, threads=threads)
+    ob.seed_arrays(ox, buf[:int(lens[0])], offs[:1], lens[:1], k, smem, max_occ=max_occ, threads=threads)
     log("oracle: index loaded, 13-mer table built in %.1fs" % (time.time() - t))
     t = time.time()
-    seeds, pos, so, po, counters = ob.seed_arrays(ox, *sub, k, True, max_occ=max_occ, threads=threads)
+    seeds, pos, so, po, counters = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads)
     dt = time.time() - t
     log("oracle: %d reads in %.2fs on %d threads" % (n, dt, threads))
     ox.close()
@@ -121,7 +121,7 @@ def make_requests(pool, n, max_reads, seed):
     return out
 
 
-def run_load(g, X, requests, rate, opts, exclusive, seed, label):
+def run_load(g, X, requests, rate, opts, exclusive, seed, label, growth=0):
     """Poisson arrivals at `rate` requests/s (0: all at once) into a server;
     returns latencies (server clock: submit to results complete) and stats."""
     import queue
@@ -133,7 +133,7 @@ def run_load(g, X, requests, rate, opts, exclusive, seed, label):
     arrivals = np.cumsum(gaps) - gaps[0]
     lat, infos, errors = [], [], []
     tickets = queue.Queue()
-    with g.Server(X, opts=opts, exclusive=exclusive) as S:
+    with g.Server(X, opts=opts, exclusive=exclusive, round_growth=growth) as S:
         def waiter():
             while True:
                 t = tickets.get()
@@ -190,20 +190,42 @@ def serve_bench(g, X, ref, k, max_occ, args):
     plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
     # peak: a burst the server drains at full speed
     burst = make_requests(pool, args.serve_burst, 200000, seed=50)
-    peak = run_load(g, X, burst, 0, opts, False, 1, "peak (burst of %d)" % len(burst))
+    peak = run_load(g, X, burst, 0, opts, False, 1, "peak (burst of %d)" % len(burst), args.serve_growth)
     levels = []
     for i, frac in enumerate(args.serve_levels):
         rate = frac * peak["requests_per_s"]
         n = max(20, int(rate * args.serve_seconds))
         reqs = make_requests(pool, n, 200000, seed=100 + i)
-        b = run_load(g, X, reqs, rate, opts, False, 200 + i, "batched at %.0f%% (%.1f req/s)" % (frac * 100, rate))
+        b = run_load(g, X, reqs, rate, opts, False, 200 + i, "batched at %.0f%% (%.1f req/s)" % (frac * 100, rate),
+                     args.serve_growth)
         nb = min(n, args.serve_plain_requests)
-        p = run_load(g, X, reqs[:nb], rate, plain, True, 200 + i,
-                     "plain per-request at %.0f%%" % (frac * 100))
+        p = None if not nb else run_load(g, X, reqs[:nb], rate, plain, True, 200 + i,
+                                         "plain per-request at %.0f%%" % (frac * 100))
         levels.append(dict(load=frac, rate_requests_per_s=rate, batched=b, plain=p))
     return dict(peak=peak, levels=levels,
                 workload="requests of 1-200,000 reads (log-uniform), 70%% 100-250 bp / 25%% 1-5 kbp / 5%% 5-20 kbp, "
                          "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
+
+
+def profiled_traffic(config, kernel, kmer_len, reads):
+    """HBM bytes per launch of the extension kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled to `reads`
+    per launch, if it was measured on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "seed_persistent_config%d.txt" % config)
+    if not os.path.exists(path):
+        return None, None
+    kv = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            kv[parts[0]] = float(parts[1])
+    if kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len:
+        return None, None
+    per_launch = (kv["fetch_kib"] * kv["fetch_factor"] + kv["write_kib"] * kv["write_factor"]) * 1024.0
+    src = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE at %d reads per launch (%s), scaled by the stream-copy "
+           "calibration%s" % (kv["reads_per_launch"], os.path.relpath(path, ROOT),
+                              "" if kv["reads_per_launch"] == reads else " and to %d reads per launch" % reads))
+    return per_launch * reads / kv["reads_per_launch"], src
 
 
 def timed_steps(batches, warmup, kernel):
@@ -225,11 +247,12 @@ def timed_steps(batches, warmup, kernel):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4])
     ap.add_argument("--batch", type=int, default=0,
-                    help="reads per step (default 1,000,000; configuration 4: its 2,000,000 reads)")
+                    help="reads per step (default 2,500,000, so that the 4 default steps seed configuration 3's "
+                         "10,000,000 reads; configuration 4: its 2,000,000 reads)")
     ap.add_argument("--k", type=int, default=0, help="default: 20 for config 1, 19 otherwise")
     ap.add_argument("--max-occ", type=int, default=-1, help="default: 0 for config 1, 500 otherwise")
     ap.add_argument("--ref-length", type=int, default=0, help="override the reference length")
@@ -238,8 +261,9 @@ def main():
     ap.add_argument("--seg-tasks", type=int, default=0, help="task ends per work item of long reads")
     ap.add_argument("--work-sample", type=int, default=200000, help="reads whose work the oracle counts")
     ap.add_argument("--reuse", action="store_true", help="one resident batch re-run every step")
-    ap.add_argument("--kmer-len", type=int, default=16,
-                    help="interval table length: 13 is the reference's table, 14..16 add a longer one")
+    ap.add_argument("--kmer-len", type=int, default=0,
+                    help="interval table length: 13 is the reference's table, 14..16 add a longer one "
+                         "(default 16; configuration 1, a 1 Mbp reference: 13)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
@@ -253,7 +277,9 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=600000, help="reads in the request pool")
     ap.add_argument("--serve-burst", type=int, default=60, help="requests of the peak burst")
     ap.add_argument("--serve-plain-requests", type=int, default=25,
-                    help="requests per load level for the plain per-request server")
+                    help="requests per load level for the plain per-request server (0: none)")
+    ap.add_argument("--serve-growth", type=int, default=0,
+                    help="rounds grow up to this many times 16 Mbases under backlog (0: fixed rounds)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this project has exactly one MI355X, "
@@ -263,7 +289,9 @@ def main():
     import numpy as np
 
     import gpuseed_amd as g
-    smem = True
+    smem = args.config != 1  # configuration 1 is the reference's MEM mode
+    if not args.kmer_len:
+        args.kmer_len = 13 if args.config == 1 else 16
     k = args.k or (20 if args.config == 1 else 19)
     max_occ = args.max_occ if args.max_occ >= 0 else (0 if args.config == 1 else 500)
     rp, qp = g.preset(args.config)
@@ -271,7 +299,7 @@ def main():
         rp.length = args.ref_length
     reuse = args.config == 4 or args.reuse  # one resident batch, re-run every step
     if not args.batch:
-        args.batch = qp.n_reads if reuse else 1000000
+        args.batch = qp.n_reads if (reuse or args.config == 1) else 2500000
     if not args.cpu_sample:
         args.cpu_sample = 20000 if args.config == 4 else 1000000
     n_batches = 1 if reuse else args.warmup + args.steps
@@ -323,7 +351,8 @@ def main():
     value = reads / elapsed
     log("batched kernel %d: %d reads in %.3f s -> %.0f reads/s" % (args.kernel, reads, elapsed, value))
     if args.profile_steps:
-        print(json.dumps({"metric": "profile run", "value": value}))
+        print(json.dumps({"metric": "profile run", "value": value, "reads_per_step": args.batch,
+                          "seed_kernel_ms_hip_events": [s.t_seed * 1e3 for s in stats]}), flush=True)
         return
     plain = None
     if not args.no_plain:
@@ -349,13 +378,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(prefix, first_arrays, gpu_out, args.cpu_sample, threads, k, max_occ)
+        cpu = cpu_baseline(prefix, first_arrays, gpu_out, args.cpu_sample, threads, k, max_occ, smem)
         cpu["threads"] = threads
         log("cpu baseline: %.0f reads/s on %d threads; parity %d/%d reads differ"
             % (cpu["reads_per_s"], threads, cpu["mismatches"], cpu["reads"]))
     wm = None
     if not args.no_cpu:
-        wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len)
+        wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem)
         log("work model: %.1f table + %.1f occ requests per read (%.0f bytes)" %
             (wm["table_per_read"], wm["occ_per_read"], wm["bytes_per_read"]))
     serving = None
@@ -377,9 +406,10 @@ def main():
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
         act = sum(st.n_table_reads + st.n_occ_reads for st in stats) / (len(stats) * args.batch)
+        traffic, traffic_src = profiled_traffic(args.config, args.kernel, info.kmer_len, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
-                    "frac": achieved / 8e12, "traffic": None,
+                    "frac": achieved / 8e12, "traffic": traffic, "traffic_source": traffic_src,
                     "kernel": "seed_persistent", "kernel_ms": seed_s * 1e3,
                     "bytes_per_read": wm["bytes_per_read"], "work_sample_reads": wm["reads"],
                     "random_access": {"requests_per_read": req_per_read,
@@ -404,13 +434,13 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "config %d: SMEM -k %d, %s reads, %.2f Gbp synthetic reference, "
+        "config": {"workload": "config %d: %s -k %d, %s reads, %.2f Gbp synthetic reference, "
                                "positions for intervals <= %d, batched persistent kernel"
-                               % (args.config, k, ("%d bp" % int(lens[0])) if lens.min() == lens.max() else
+                               % (args.config, "SMEM" if smem else "MEM", k, ("%d bp" % int(lens[0])) if lens.min() == lens.max() else
                                   "%d-%d bp (mean %.0f)" % (int(lens.min()), int(lens.max()), float(lens.mean())),
                                   rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
-                   "mode": "SMEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
+                   "mode": "SMEM" if smem else "MEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
                    "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {

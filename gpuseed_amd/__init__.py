@@ -69,7 +69,8 @@ class Opts(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("opts", Opts), ("budget_bytes", C.c_uint64), ("round_bases", C.c_uint64),
-                ("exclusive", C.c_int), ("workers", C.c_int), ("reserved", C.c_int * 2)]
+                ("exclusive", C.c_int), ("workers", C.c_int), ("round_growth", C.c_int),
+                ("reserved", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -439,7 +440,8 @@ class Server:
     """Continuous-batching seeding server over a resident index (gs_server_*).
     submit() returns a ticket; wait(ticket) returns (SeedOutput, RequestInfo)."""
 
-    def __init__(self, index, budget_bytes=0, round_bases=0, exclusive=False, workers=0, opts=None, **kw):
+    def __init__(self, index, budget_bytes=0, round_bases=0, exclusive=False, workers=0, round_growth=0,
+                 opts=None, **kw):
         L = lib()
         _bind_server(L)
         o = ServerOpts()
@@ -448,6 +450,7 @@ class Server:
         o.round_bases = round_bases
         o.exclusive = 1 if exclusive else 0
         o.workers = workers
+        o.round_growth = round_growth
         self.h = C.c_void_p()
         self._inputs = {}
         _check("gs_server_create", L.gs_server_create(index.h, C.byref(o), C.byref(self.h)))

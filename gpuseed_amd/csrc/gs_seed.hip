@@ -794,9 +794,10 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   // Work splitting pays where the tail of a launch is a large share of it:
   // batches of at most kSplitItemsPerLane work items per lane of the
   // persistent grid (in larger batches the tail is amortised and keeping
-  // the overflow queue live costs more than it saves).
+  // the overflow queue live costs more than it saves).  An explicit
+  // split_tasks turns it on whatever the batch size.
   bool split_on = P.opts.split_tasks != 0xFFFF && P.opts.kernel != 0;
-  if (split_on) {
+  if (split_on && P.opts.split_tasks == 0) {
     unsigned grid = 0;
     const int wg = P.opts.wg ? P.opts.wg : 128;
     if (gs::persistent_grid(P.opts.kernel == 2 ? 1 : 0, wg, &grid) != GS_OK || !grid ||

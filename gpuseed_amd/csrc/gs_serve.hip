@@ -47,6 +47,7 @@ struct Request {
   std::vector<uint64_t> cuts;  // read index after each reference batch; last = n
   size_t next_cut = 0;         // first cut not yet scheduled
   uint64_t next_read = 0;      // first read not yet scheduled
+  uint64_t sched_bases = 0;    // bases of the reads before next_read
   uint64_t done_reads = 0;
   struct Piece {  // results of one chunk, written by the worker that runs it
     std::vector<uint32_t> nseeds, npos;
@@ -127,7 +128,15 @@ void advance(Request* q, uint64_t end) {
 std::unique_ptr<Round> plan_round(gs_server* S) {
   auto R = std::make_unique<Round>();
   gs::plan_reset(R->plan, S->o.opts);
-  const uint64_t cap = S->o.round_bases;
+  // rounds may grow with the backlog, up to round_growth x round_bases: a
+  // launch has a fixed cost (its tail), so larger rounds raise throughput
+  // under load, at the price of longer waits for the requests that arrive
+  // behind them (fixed-size rounds are the default)
+  uint64_t backlog = 0;
+  for (Request* q : S->active) backlog += q->n_bases - q->sched_bases;
+  const uint64_t grown = backlog / (uint64_t)S->o.workers;
+  const uint64_t cap_max = S->o.round_bases * (uint64_t)S->o.round_growth;
+  const uint64_t cap = grown < S->o.round_bases ? S->o.round_bases : (grown > cap_max ? cap_max : grown);
   const uint64_t mem_cap = S->o.budget_bytes / (uint64_t)(S->o.workers + 1);
   uint64_t bases = 0, est = 0;
   // device bytes of a chunk, an upper bound of what plan_device_bytes adds:
@@ -144,6 +153,7 @@ std::unique_ptr<Round> plan_round(gs_server* S) {
     if (gs::plan_add(R->plan, q->bases, q->offs, q->lens, q->next_read, end) != GS_OK) return false;
     bases += b;
     est += e;
+    q->sched_bases += b;
     auto piece = std::make_shared<Request::Piece>();
     q->pieces.push_back(piece);
     R->chunks.push_back({q, q->next_read, end, at, piece});
@@ -298,6 +308,8 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 1;
   if (S->o.workers <= 0) S->o.workers = S->o.exclusive ? 1 : 2;
   if (S->o.workers > 8) S->o.workers = 8;
+  if (S->o.round_growth <= 0) S->o.round_growth = 1;
+  if (S->o.round_growth > 64) S->o.round_growth = 64;
   S->t0 = Clock::now();
   for (int w = 0; w < S->o.workers; ++w) S->batches.push_back(gs::batch_new(X));
   S->planner = std::thread(planner_main, S);

@@ -15,7 +15,7 @@
  *                      (default), 2 batched with 16-bit size records
  *   --max-occ N        seeds with more than N occurrences get no positions
  *                      (default 0: every occurrence, as the reference)
- *   --device-bases N   bases uploaded per device batch (default 32,000,000);
+ *   --device-bases N   bases uploaded per device batch (default 256,000,000);
  *                      device batches hold whole reference batches, so MEM
  *                      output does not depend on this value
  *   --device N         HIP device ordinal (default 0)
@@ -24,6 +24,12 @@
  *                      more positions than its fixed output buffer holds
  *                      (seed_gen.cu:3553-3558); off by default, the library
  *                      has no such limit
+ *   --kmer-len N       13 (default): the reference's 13-mer table only;
+ *                      14..16: also an N-mer interval table (8 * 4^N bytes
+ *                      of HBM) that answers a task's first N bases with one
+ *                      lookup; the output is the same
+ *   --autotune         pick the batched kernel's launch parameters by timing
+ *                      them on the first device batch (same output)
  *
  * Exit status 0 on success, 1 on any error (the reference's exit(1) and
  * exit(EXIT_FAILURE) cases: missing or inconsistent index files, output
@@ -340,13 +346,15 @@ int main(int argc, char** argv) {
   memset(&o, 0, sizeof o);
   o.k = 20;
   o.kernel = 1;
-  int print = 0, device = 0, reference_capacity = 0, c, li;
-  uint64_t device_bases = 32000000ull;
+  int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, c, li;
+  uint64_t device_bases = 256000000ull;
   static const struct option longopts[] = {{"kernel", required_argument, 0, 1},
                                            {"max-occ", required_argument, 0, 2},
                                            {"device-bases", required_argument, 0, 3},
                                            {"device", required_argument, 0, 4},
                                            {"reference-capacity", no_argument, 0, 5},
+                                           {"kmer-len", required_argument, 0, 6},
+                                           {"autotune", no_argument, 0, 7},
                                            {0, 0, 0, 0}};
   while ((c = getopt_long(argc, argv, "k:so", longopts, &li)) >= 0) {
     switch (c) {
@@ -358,12 +366,15 @@ int main(int argc, char** argv) {
       case 3: device_bases = strtoull(optarg, NULL, 10); break;
       case 4: device = atoi(optarg); break;
       case 5: reference_capacity = 1; break;
+      case 6: kmer_len = atoi(optarg); break;
+      case 7: autotune = 1; break;
       default: break; /* like the reference: getopt's message, then go on */
     }
   }
   if (optind + 2 > argc) {
     fprintf(stderr, "Usage: seed_gen [-k INT] [-s] [-o] [--kernel N] [--max-occ N] [--device-bases N] "
-                    "[--device N] [--reference-capacity] <index-prefix> <reads.fa>\n");
+                    "[--device N] [--reference-capacity] [--kmer-len 13..16] [--autotune] "
+                    "<index-prefix> <reads.fa>\n");
     return 1;
   }
   if (o.k < 14) {
@@ -372,6 +383,10 @@ int main(int argc, char** argv) {
   }
   if (o.kernel < 0 || o.kernel > 2) {
     fprintf(stderr, "seed_gen: --kernel must be 0, 1 or 2\n");
+    return 1;
+  }
+  if (kmer_len < 13 || kmer_len > 16) {
+    fprintf(stderr, "seed_gen: --kmer-len must be 13, 14, 15 or 16\n");
     return 1;
   }
   if (device_bases < 1) device_bases = 1;
@@ -397,6 +412,16 @@ int main(int argc, char** argv) {
           info.sa_intv, info.L2[0], info.L2[1], info.L2[2], info.L2[3], info.L2[4]);
   fprintf(stderr, "Index on the GPU in %.3f seconds (read %.3f, upload %.3f, repack %.3f, 13-mer table %.3f), %.2f GB\n",
           now() - t0, info.t_read, info.t_upload, info.t_repack, info.t_table, info.device_bytes / 1e9);
+  if (kmer_len > 13) {
+    rc = gs_index_set_kmer_len(idx, kmer_len);
+    if (rc) {
+      fprintf(stderr, "seed_gen: building the %d-mer table failed: %s (%d)\n", kmer_len, errname(rc), rc);
+      return 1;
+    }
+    gs_index_get_info(idx, &info);
+    fprintf(stderr, "%d-mer interval table in %.3f seconds, %.2f GB on the GPU\n", kmer_len, info.t_ext,
+            info.device_bytes / 1e9);
+  }
 
   fifo q_in, q_out;
   fifo_init(&q_in, 2);
@@ -418,6 +443,22 @@ int main(int argc, char** argv) {
       continue;
     }
     double t1 = now();
+    if (autotune && o.kernel != 0) { /* launch parameters from the first device batch */
+      gs_tune_result tr;
+      rc = gs_autotune(idx, ck->bases, ck->offs, ck->lens, ck->n, &o, 0, &tr);
+      if (rc) {
+        fprintf(stderr, "seed_gen: autotuning failed: %s (%d)\n", errname(rc), rc);
+        seed_rc = rc;
+        chunk_free(ck);
+        continue;
+      }
+      o = tr.opts;
+      fprintf(stderr, "autotune: kernel %d, wg %u, seg_tasks %u, walk_rows %u: %.3f s against %.3f s (%d tried)\n",
+              o.kernel, (unsigned)o.wg, (unsigned)o.seg_tasks, (unsigned)o.walk_rows, tr.seconds,
+              tr.default_seconds, tr.n_tried);
+      autotune = 0;
+      t1 = now();
+    }
     rc = gs_seed(idx, ck->bases, ck->offs, ck->lens, ck->n, &o, &ck->res);
     if (rc) {
       fprintf(stderr, "seed_gen: seeding failed: %s (%d)\n", errname(rc), rc);

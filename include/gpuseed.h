@@ -182,11 +182,12 @@ typedef struct {
                        (0 = 64)                                               */
   uint16_t wg;        /* batched kernels: workgroup size 64, 128 or 256 (0 = 128) */
   uint16_t walk_rows; /* suffix-array walk: rows per thread 1, 2 or 4 (0 = 1) */
-  uint16_t split_tasks; /* batched kernels, batches of at most 4 work items per
-                       lane: in the tail of a launch an item that has used
-                       split_work memory operations hands half of its task
-                       ends to an idle lane while at least 2 * split_tasks
-                       remain, 1..64 (0 = 8; 0xFFFF: no splitting)           */
+  uint16_t split_tasks; /* batched kernels: in the tail of a launch an item
+                       that has used split_work memory operations hands half
+                       of its task ends to an idle lane while at least
+                       2 * split_tasks remain, 1..64; 0 = 8, and only in
+                       batches of at most 4 work items per lane of the
+                       persistent grid; 0xFFFF: no splitting                 */
   uint16_t split_work;  /* in units of 64 memory operations, 1..0xFFFE (0 = 32).
                        The output depends on none of the tuning fields
                        (kernel, seg_tasks, wg, walk_rows, split_*)           */
@@ -296,7 +297,11 @@ typedef struct {
                             (the per-request baseline)                        */
   int workers;           /* device threads, each with its own stream and round
                             buffers (0 = 2, or 1 when exclusive; at most 8)   */
-  int reserved[2];
+  int round_growth;      /* rounds grow to backlog / workers bases, at most
+                            round_growth x round_bases, at most 64 (0 or 1:
+                            fixed size): more throughput under load, longer
+                            waits for small requests                          */
+  int reserved;
 } gs_server_opts;
 
 typedef struct {

@@ -58,13 +58,15 @@ def same(a, b):
             np.array_equal(a.read_seed_off, b.read_seed_off) and np.array_equal(a.read_pos_off, b.read_pos_off))
 
 
-@pytest.mark.parametrize("smem,k,round_bases", [(True, 19, 2000000), (False, 20, 1500000), (True, 19, 0)])
-def test_each_result_is_the_same_alone_and_in_the_mix(g, world, smem, k, round_bases):
-    reqs = make_requests(world, 40, 2000, seed=7 + k + round_bases % 7)
+@pytest.mark.parametrize("smem,k,round_bases,growth", [(True, 19, 2000000, 0), (False, 20, 1500000, 0),
+                                                       (True, 19, 0, 0), (True, 19, 1000000, 8),
+                                                       (False, 20, 1000000, 4)])
+def test_each_result_is_the_same_alone_and_in_the_mix(g, world, smem, k, round_bases, growth):
+    reqs = make_requests(world, 40, 2000, seed=7 + k + round_bases % 7 + growth)
     with g.Index(world["prefix"]) as X:
         alone = [g.seed(X, arrays=r, k=k, smem=smem, kernel=0, max_occ=200) for r in reqs]
         say("alone done: %d requests, %d reads" % (len(reqs), sum(len(r[2]) for r in reqs)))
-        with g.Server(X, round_bases=round_bases, k=k, smem=smem, max_occ=200) as S:
+        with g.Server(X, round_bases=round_bases, round_growth=growth, k=k, smem=smem, max_occ=200) as S:
             tickets = [None] * len(reqs)
 
             def submit(idx):

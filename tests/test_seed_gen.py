@@ -193,6 +193,8 @@ def test_cli_argument_errors(built, c1):
     ["-s", "-k", "19", "--kernel", "0"],
     ["-k", "20", "--device-bases", "300000"],       # one reference batch per device batch
     ["-s", "-k", "20", "--max-occ", "5", "--device-bases", "2500000"],
+    ["-s", "-k", "19", "--kmer-len", "15", "--autotune"],   # extended table, tuned launch
+    ["-k", "20", "--kmer-len", "14", "--device-bases", "400000"],
 ])
 def test_cli_text_matches_oracle(built, c1, tmp_path, flags):
     say("seed_gen", " ".join(flags))
@@ -203,8 +205,10 @@ def test_cli_text_matches_oracle(built, c1, tmp_path, flags):
     assert p.returncode == 0, p.stderr.decode()[-2000:]
     oflags, i = [], 0
     while i < len(flags):           # the oracle takes -k, -s and -m; device options are dropped
-        if flags[i] in ("--device-bases", "--kernel"):
+        if flags[i] in ("--device-bases", "--kernel", "--kmer-len"):
             i += 2
+        elif flags[i] == "--autotune":
+            i += 1
         elif flags[i] == "--max-occ":
             oflags += ["-m", flags[i + 1]]
             i += 2
