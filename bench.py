@@ -268,6 +268,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
+    ap.add_argument("--no-graph", action="store_true", help="skip the host-to-host (HIP graph) measurement")
     ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement (configuration 5)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement")
@@ -362,6 +363,30 @@ def main():
         log("plain kernel: %.0f reads/s" % plain["reads_per_s"])
         # leave the batched results in place for the parity check below
         batches[args.warmup][1].run(args.kernel)
+    e2e = None
+    if not args.no_graph:
+        # the whole pipeline of one batch, host memory to host memory: the
+        # drop-in call (gs_seed: upload, pipeline, download, one call each
+        # time) against replays of the batch's captured HIP graph (upload from
+        # pinned copies, pipeline, download into pinned memory, one launch);
+        # the graph's results are the ones the parity check below reads
+        fa, fb = batches[args.warmup]
+        eager = []
+        for _ in range(2):
+            t = time.perf_counter()
+            g.seed(X, arrays=fa, k=k, smem=smem, max_occ=max_occ, kernel=args.kernel, seg_tasks=args.seg_tasks)
+            eager.append(time.perf_counter() - t)
+        fb.capture(args.kernel)
+        fb.replay()
+        graph = []
+        for _ in range(3):
+            t = time.perf_counter()
+            fb.replay()
+            graph.append(time.perf_counter() - t)
+        e2e = {"reads": args.batch, "drop_in_call_ms": min(eager) * 1e3, "graph_replay_ms": min(graph) * 1e3,
+               "graph_reads_per_s": args.batch / min(graph)}
+        log("one batch host to host: gs_seed %.1f ms, graph replay %.1f ms (%.0f reads/s)" %
+            (e2e["drop_in_call_ms"], e2e["graph_replay_ms"], e2e["graph_reads_per_s"]))
 
     seed_s = sum(s.t_seed for s in stats) / len(stats)
     first_arrays, first_batch = batches[args.warmup]
@@ -451,6 +476,7 @@ def main():
         "index_build_s": t_build,
         "plain": None if not plain else {"reads_per_s": plain["reads_per_s"], "seed_kernel_ms": plain["seed_ms"],
                                          "batched_over_plain": value / plain["reads_per_s"]},
+        "host_to_host": e2e,
         "kernel_ms": {"seed": seed_s * 1e3,
                       "count": sum(s.t_count for s in stats) / len(stats) * 1e3,
                       "write": sum(s.t_write for s in stats) / len(stats) * 1e3,

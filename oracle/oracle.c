@@ -327,8 +327,8 @@ static void count_blocks(const or_index* X, uint32_t l, uint32_t u, work_t* w) {
  * output is the same whatever prune, smem_stop and ext_len say; they only
  * change which steps run and are counted (see or_opts). */
 static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t L, uint32_t k,
-                                  int prune, int smem_stop, uint32_t ext_len, cand_t* out, uint32_t* A,
-                                  work_t* w) {
+                                  int prune, int smem_stop, uint32_t ext_len, uint32_t sat, cand_t* out,
+                                  uint32_t* A, work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
@@ -372,7 +372,8 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     int pruned = 0, reached = 1;
     if (prune) {
       uint32_t s = u - l + 1;
-      if (A[b] == s) pruned = 1;
+      if (sat && s >= sat) s = sat; /* a saturated record never prunes */
+      if (A[b] == s && s != sat) pruned = 1;
       A[b] = s;
     }
     for (int64_t x = (int64_t)b - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
@@ -389,7 +390,8 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       b = (uint32_t)x; /* 2252-2255 */
       if (prune) {
         uint32_t s = u - l + 1;
-        if (A[b] == s) pruned = 1;
+        if (sat && s >= sat) s = sat;
+        if (A[b] == s && s != sat) pruned = 1;
         A[b] = s;
       }
     }
@@ -452,12 +454,13 @@ static void phase_candidates(void* vctx, uint64_t i, int tid) {
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_fwd(s[q]);
   const int stop = C->o->smem && C->o->smem_stop;
   const uint32_t ext = (uint32_t)C->o->ext_len;
-  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, tmp + ntask, A, &C->work[tid]);
+  const uint32_t sat = C->o->prune_sat;
+  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, tmp + ntask, A, &C->work[tid]);
   /* forward: decreasing end -> slot order (increasing end) */
   for (uint32_t q = 0; q < nf; ++q) Ls->F[q] = tmp[ntask + nf - 1 - q];
   Ls->nF = nf;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_rev(s[L - 1 - q]);
-  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, Ls->R, A, &C->work[tid]);
+  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, Ls->R, A, &C->work[tid]);
   for (uint32_t q = 0; q < nr; ++q) { /* virtual [b,e) -> reverse (L-e, L-b) with strand bits */
     cand_t c = Ls->R[q];
     Ls->R[q] = (cand_t){(L - c.e) | 0x80000000u, (L - c.b) | 0x80000000u, c.l, c.u};

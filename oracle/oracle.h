@@ -70,6 +70,10 @@ typedef struct {
    *            table when the pattern does not occur; 0 or 13: off. */
   int smem_stop;
   int ext_len;
+  /* prune_sat: size records saturate at this value and a saturated record
+   * never prunes, like the device's 8-bit (255) and 16-bit (65535) records;
+   * 0: exact records. */
+  uint32_t prune_sat;
 } or_opts;
 
 typedef struct {

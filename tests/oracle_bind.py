@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 class OrOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("prune", C.c_int),
                 ("max_occ", C.c_uint32), ("threads", C.c_int), ("use_table", C.c_int),
-                ("smem_stop", C.c_int), ("ext_len", C.c_int)]
+                ("smem_stop", C.c_int), ("ext_len", C.c_int), ("prune_sat", C.c_uint32)]
 
 
 class OrSeed(C.Structure):
@@ -129,10 +129,10 @@ def pack_reads(reads):
     return buf, offs, lens
 
 
-def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem_stop=0, ext_len=0):
+def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem_stop=0, ext_len=0, prune_sat=0):
     """Returns (per-read list of (qbeg, qend, l, u, [positions])), counters dict)."""
     buf, offs, lens = pack_reads(reads)
-    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len)
+    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len, prune_sat)
     res = OrResult()
     rc = lib().or_seed_reads(index.h, buf if buf else b"\0",
                              offs.ctypes.data_as(C.POINTER(C.c_uint64)),
@@ -160,13 +160,13 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem
 
 
 def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, use_table=1, smem_stop=0,
-                ext_len=0):
+                ext_len=0, prune_sat=0):
     """Flat numpy copies of the oracle's result: (seeds[n,6] u32, pos u32,
     read_seed_off u64[n+1], read_pos_off u64[n+1]) plus the work counters."""
     buf = np.ascontiguousarray(buf, dtype=np.uint8)
     offs = np.ascontiguousarray(offs, dtype=np.uint64)
     lens = np.ascontiguousarray(lens, dtype=np.uint32)
-    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len)
+    o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len, prune_sat)
     res = OrResult()
     rc = lib().or_seed_reads(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                              offs.ctypes.data_as(C.POINTER(C.c_uint64)),

@@ -84,9 +84,11 @@ def test_seeds_unchanged_by_the_extended_table(g, tmp_path, length):
 
 @pytest.mark.parametrize("length", [13, 16])
 @pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
-def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k):
+@pytest.mark.parametrize("kernel,sat", [(1, 255), (2, 65535)])
+def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k, kernel, sat):
     """The batched kernel's own memory-operation counters equal the oracle's
-    count of the same algorithm (distance-1 pruning, the SMEM read-start stop,
+    count of the same algorithm (distance-1 pruning with size records that
+    saturate like the kernel's 8- or 16-bit ones, the SMEM read-start stop,
     the extended table): 8-byte table lookups and 16-byte occ entries, for
     reads held whole in the kernel's LDS window."""
     rng = random.Random(7 + length)
@@ -96,11 +98,11 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k):
     ox = ob.Index(prefix)
     arrays = g.pack_reads(reads)
     _, _, _, _, c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1,
-                                   ext_len=length if length > 13 else 0)
+                                   ext_len=length if length > 13 else 0, prune_sat=sat)
     with g.Index(prefix) as X:
         if length > 13:
             X.set_kmer_len(length)
-        b = g.Batch(X, arrays, k=k, smem=smem, kernel=1, split_tasks=g.NO_SPLIT)
+        b = g.Batch(X, arrays, k=k, smem=smem, kernel=kernel, split_tasks=g.NO_SPLIT)
         st = b.run()
         b.close()
     say("T=%d smem=%d: kernel %d table %d occ, oracle %d table %d occ" %

@@ -186,3 +186,17 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
                 assert cg["w_blocks96"] < cb["w_blocks96"]
             if ext:
                 assert cg["w_ext"] > 0 and cg["w_steps"] < cb["w_steps"]
+    # saturating size records only lose pruning: same seeds, at least as
+    # much work, and more work the lower the saturation (repeats of the
+    # 1 Mbp reference reach interval sizes of 10, so 4 and 8 lose prunes)
+    for smem, k in ((True, 19), (False, 20)):
+        exact = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1)
+        prev = exact[4]["w_blocks96"]
+        for sat in (255, 8, 4, 2):
+            got = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, prune_sat=sat)
+            for a, b in zip(exact[:4], got[:4]):
+                assert (a == b).all(), (smem, sat)
+            assert got[4]["w_blocks96"] >= prev, (smem, sat)
+            prev = got[4]["w_blocks96"]
+        assert prev > exact[4]["w_blocks96"]
+    ox.close()
