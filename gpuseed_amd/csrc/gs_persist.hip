@@ -186,27 +186,18 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     }
     last_b = b;
   };
-  // phase 1: settle x on the next position with a base code
-  auto prepare_x = [&]() __attribute__((always_inline)) {
+  // phase 1: settle x on the next position with a base code.  Returns 0 if
+  // the task goes on, 1 if it was pruned, 3 if it reached the read start.
+  auto prepare_x = [&]() __attribute__((always_inline)) -> int {
     while (true) {
-      if (x < 0) {
-        finish_task();
-        if (P.smem)
-          end_stream();  // later tasks of the item end at the read start too and are filtered out
-        else
-          next_task();
-        return;
-      }
+      if (x < 0) return 3;
       const uint32_t c = code(x);
       if (c <= 3) {
         cx = c;
-        return;
+        return 0;
       }
       b = (uint32_t)x;  // code > 3 after the window: interval kept, position consumed
-      if (prune_at(x, s)) {
-        next_task();
-        return;
-      }
+      if (prune_at(x, s)) return 1;
       --x;
     }
   };
@@ -420,46 +411,50 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     uint4 d2 = d1;
     if (a2) d2 = *a2;
 
+    // the new interval: a table entry (op 1) or one backward step (op 2);
+    // both then share one copy of the task logic, so a wave whose lanes are
+    // in different phases runs it once
+    uint32_t nl, nu, nb;
     if (op == 1) {
-      const uint32_t il = (tidx & 1u) ? d1.z : d1.x, iu = (tidx & 1u) ? d1.w : d1.y;
-      if (il > iu) {
-        if (tlen > gs::kPrecalc)
-          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
-        else
-          next_task();
-      } else {
-        l = il;
-        u = iu;
-        s = u - l + 1u;
-        b = (uint32_t)e - tlen;
-        if (prune_at((int32_t)b, s)) {
-          next_task();
-        } else {
-          x = (int32_t)b - 1;
-          phase = 1;
-          prepare_x();
-        }
-      }
-    } else if (op == 2) {
+      nl = (tidx & 1u) ? d1.z : d1.x;
+      nu = (tidx & 1u) ? d1.w : d1.y;
+      nb = (uint32_t)e - tlen;
+    } else {
       const uint32_t o2 = gs::rank_in_entry(d1, i2 - b2 * gs::kBlockBases);
       const uint32_t o1 = have1 ? gs::rank_in_entry(d2, i1 - b1 * gs::kBlockBases) : 0u;
       const uint32_t base = pick4(cx, L2_0, L2_1, L2_2, L2_3);
-      const uint32_t l2 = base + o1 + 1u, u2 = base + o2;
-      if (l2 > u2) {
-        finish_task();
-        next_task();
+      nl = base + o1 + 1u;
+      nu = base + o2;
+      nb = (uint32_t)x;
+    }
+    if (op != 0) {
+      // how the task goes on: 0 continues, 1 pruned or a table miss (no
+      // candidate, 2145), 2 its interval emptied, 3 it reached the read
+      // start; the task logic below is inlined once for every lane
+      int res = 0;
+      if (nl > nu) {
+        if (op == 1 && tlen > gs::kPrecalc)
+          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
+        else
+          res = op == 2 ? 2 : 1;
       } else {
-        l = l2;
-        u = u2;
+        l = nl;
+        u = nu;
         s = u - l + 1u;
-        b = (uint32_t)x;
-        if (prune_at(x, s)) {
-          next_task();
+        b = nb;
+        if (prune_at((int32_t)b, s)) {
+          res = 1;
         } else {
-          --x;
-          prepare_x();
+          x = (int32_t)b - 1;
+          phase = 1;
+          res = prepare_x();
         }
       }
+      if (res >= 2) finish_task();
+      if (res == 3 && P.smem)
+        end_stream();  // later tasks of the item end at the read start too and are filtered out
+      else if (res != 0)
+        next_task();
     }
     if (P.dyn_cap) fold_running();
   }
