@@ -144,6 +144,19 @@ def lib():
         L.gs_result_free.argtypes = [C.POINTER(Result)]
         L.gs_result_free.restype = None
         L.gs_format_text.argtypes = [C.POINTER(Result), C.c_int, C.c_uint64, C.c_int]
+        L.gs_index_convert.argtypes = [C.c_char_p, C.c_char_p]
+        L.gs_bwa_ref_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.gs_bwa_ref_free.argtypes = [C.c_void_p]
+        L.gs_bwa_ref_free.restype = None
+        L.gs_bwa_ref_length.argtypes = [C.c_void_p]
+        L.gs_bwa_ref_length.restype = C.c_uint64
+        L.gs_bwa_ref_codes.argtypes = [C.c_void_p]
+        L.gs_bwa_ref_codes.restype = u8p
+        L.gs_bwa_ref_n_contigs.argtypes = [C.c_void_p]
+        L.gs_bwa_ref_n_holes.argtypes = [C.c_void_p]
+        L.gs_bwa_ref_contig.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), u64p, u64p]
+        L.gs_bwa_ref_hole.argtypes = [C.c_void_p, C.c_int, u64p, u64p, C.c_char_p]
+        L.gs_bwa_ref_locate.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), u64p]
         _lib = L
     return _lib
 
@@ -194,6 +207,61 @@ def build_index_host(codes, sa_intv, prefix):
     codes = np.ascontiguousarray(codes, dtype=np.uint8)
     _check("gs_build_index_host", lib().gs_build_index_host(_ptr(codes, C.c_uint8), len(codes),
                                                             sa_intv, prefix.encode()))
+
+
+def index_convert(src_prefix, dst_prefix):
+    """Writes dst_prefix.bwt/.sa in the fork's layout from either the fork's
+    or upstream bwa 0.7's files at src_prefix (gs_index_convert)."""
+    _check("gs_index_convert", lib().gs_index_convert(src_prefix.encode(), dst_prefix.encode()))
+
+
+class BwaRef:
+    """bwa's <prefix>.pac/.ann/.amb (gs_bwa_ref_*): codes of the forward
+    reference, its sequences and its runs of ambiguous bases."""
+
+    def __init__(self, prefix):
+        self.h = C.c_void_p()
+        _check("gs_bwa_ref_load", lib().gs_bwa_ref_load(prefix.encode(), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gs_bwa_ref_free(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return int(lib().gs_bwa_ref_length(self.h))
+
+    def codes(self):
+        n = len(self)
+        return np.ctypeslib.as_array(lib().gs_bwa_ref_codes(self.h), shape=(n,)).copy()
+
+    def contigs(self):
+        out = []
+        for i in range(lib().gs_bwa_ref_n_contigs(self.h)):
+            name, off, ln = C.c_char_p(), C.c_uint64(), C.c_uint64()
+            _check("gs_bwa_ref_contig", lib().gs_bwa_ref_contig(self.h, i, C.byref(name), C.byref(off),
+                                                                C.byref(ln)))
+            out.append((name.value.decode(), off.value, ln.value))
+        return out
+
+    def holes(self):
+        out = []
+        for i in range(lib().gs_bwa_ref_n_holes(self.h)):
+            off, ln, ch = C.c_uint64(), C.c_uint64(), C.create_string_buffer(1)
+            _check("gs_bwa_ref_hole", lib().gs_bwa_ref_hole(self.h, i, C.byref(off), C.byref(ln), ch))
+            out.append((off.value, ln.value, ch.raw.decode()))
+        return out
+
+    def locate(self, pos):
+        c, off = C.c_int(), C.c_uint64()
+        _check("gs_bwa_ref_locate", lib().gs_bwa_ref_locate(self.h, pos, C.byref(c), C.byref(off)))
+        return c.value, off.value
 
 
 def roofline_random(buffer_bytes, line_bytes=16, mode=0, inflight=1, grid=8192, iters=256, reps=3):

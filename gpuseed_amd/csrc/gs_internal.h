@@ -66,7 +66,9 @@ struct PersistParams {
   uint4* wmeta;              // per whole-read stream (2 * read + strand) that was split
   uint32_t* wlink;
   uint32_t dyn_cap, split_work, split_tasks, epoch;
-  uint32_t split_below, pad2[3];  // split only while fewer items than this run (set at launch)
+  uint32_t split_below;           // split only while fewer items than this run (set at launch)
+  uint32_t refill_group;          // idle lanes of a wave that trigger a refill (set at launch)
+  uint32_t pad2[2];
 };
 
 // Joins pieces (segments of long reads, split-off pieces) into the per-read
@@ -92,6 +94,9 @@ struct JoinParams {
 
 constexpr uint32_t kSplitStream = 0xFFFFFFFFu;  // nF/nR of a whole-read stream left for join_whole
 constexpr uint32_t kSplitItemsPerLane = 4;      // work splitting only in batches this small
+constexpr uint32_t kRefillGroup = 4;            // idle lanes of a wave that trigger a refill
+                                                // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
+                                                // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms)
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
 constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start

@@ -340,7 +340,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     // lanes without an item take the next one of the main queue
     const bool need = !have && !finished && !main_done;
     const uint64_t needmask = __ballot(need);
-    if (needmask) {
+    // refills go in groups: the whole wave runs the refill path (code copy,
+    // record clear, first pattern) whenever any lane takes it, so idle lanes
+    // wait until refill_group of them are idle or the wave has no work
+    if (needmask && ((uint32_t)__popcll(needmask) >= P.refill_group || __ballot(have) == 0)) {
       const int leader = __ffsll((unsigned long long)needmask) - 1;
       uint32_t base = 0;
       if (lane == leader) base = atomicAdd(P.queue, (uint32_t)__popcll(needmask));
@@ -609,6 +612,7 @@ int grid_t(unsigned* grid) {
 template <typename AT, int LMAX, int BS>
 int launch_t(gs::PersistParams P, unsigned grid, hipStream_t st) {
   P.split_below = grid * (uint32_t)BS / 2;  // half the lanes of the launch
+  P.refill_group = gs::kRefillGroup;
   hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, P);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }

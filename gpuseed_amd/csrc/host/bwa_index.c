@@ -111,9 +111,12 @@ void gs_host_index_free(gs_host_index* idx) {
 }
 
 /* Mirrors bwt_restore_bwt / bwt_restore_sa (seed_gen.cu:2685-2736), with the
- * exit() calls turned into return codes and the missing size checks added. */
+ * exit() calls turned into return codes and the missing size checks added.
+ * Files in upstream bwa 0.7's 64-bit layout are recognised by their header
+ * and size and converted (bwa07.c). */
 int gs_host_index_load(const char* prefix, gs_host_index* idx) {
   if (!prefix || !idx) return GS_EINVAL;
+  if (gs_host_is_bwa07(prefix)) return gs_host_index_load_bwa07(prefix, idx);
   memset(idx, 0, sizeof(*idx));
   size_t plen = strlen(prefix);
   char* fn = (char*)malloc(plen + 8);

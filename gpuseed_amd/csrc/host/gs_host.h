@@ -47,6 +47,11 @@ typedef struct {
 int gs_host_index_load(const char* prefix, gs_host_index* idx);
 void gs_host_index_free(gs_host_index* idx);
 
+/* Upstream bwa 0.7 files (64-bit integers, 128-base blocks): detection by
+ * header and size, and loading into the fork layout above. */
+int gs_host_is_bwa07(const char* prefix);
+int gs_host_index_load_bwa07(const char* prefix, gs_host_index* idx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,12 +2,18 @@
  * gs_build_index - builds <prefix>.bwt and <prefix>.sa from a FASTA file.
  *
  *   gs_build_index [-r RATIO] [-p PREFIX] ref.fa
+ *   gs_build_index [-r RATIO] -p PREFIX -b BWA_PREFIX
+ *   gs_build_index -p PREFIX -c BWA07_PREFIX
  *
  * Same command line as build_index.sh (build_index.sh:24-50: -r is the
  * suffix-array sampling interval, default 7; -p defaults to the FASTA name),
  * same output files (build_index.sh:63-75), without the bwa fork: all records
  * are concatenated, A/C/G/T (either case) map to 0..3 and any other letter is
  * replaced by a pseudo-random base drawn from a fixed-seed stream.
+ * -b takes the reference from bwa's BWA_PREFIX.pac/.ann/.amb instead (the
+ * concatenated sequences with bwa's own random bases in runs of N) and lists
+ * its sequences; -c converts the .bwt/.sa of upstream bwa 0.7 at BWA07_PREFIX
+ * to the layout seed_gen reads.
  */
 #define _GNU_SOURCE
 #include <stdio.h>
@@ -20,12 +26,49 @@
 int main(int argc, char** argv) {
   int ratio = 7, c;
   const char* prefix = NULL;
-  while ((c = getopt(argc, argv, "r:p:")) >= 0) {
+  const char* bwa_prefix = NULL;
+  const char* convert = NULL;
+  while ((c = getopt(argc, argv, "r:p:b:c:")) >= 0) {
     switch (c) {
       case 'r': ratio = atoi(optarg); break;
       case 'p': prefix = optarg; break;
+      case 'b': bwa_prefix = optarg; break;
+      case 'c': convert = optarg; break;
       default: break;
     }
+  }
+  if (convert || bwa_prefix) {
+    if (!prefix || ratio <= 0) {
+      fprintf(stderr, "-b and -c need -p PREFIX (and a positive -r)\n");
+      return 1;
+    }
+    int rc;
+    if (convert) {
+      rc = gs_index_convert(convert, prefix);
+      if (rc == GS_OK) fprintf(stderr, "Wrote %s.bwt and %s.sa from %s\n", prefix, prefix, convert);
+    } else {
+      gs_bwa_ref* R = NULL;
+      rc = gs_bwa_ref_load(bwa_prefix, &R);
+      if (rc == GS_OK) {
+        for (int i = 0; i < gs_bwa_ref_n_contigs(R); ++i) {
+          const char* name = NULL;
+          uint64_t off = 0, len = 0;
+          gs_bwa_ref_contig(R, i, &name, &off, &len);
+          fprintf(stderr, "%s\toffset %llu\tlength %llu\n", name, (unsigned long long)off, (unsigned long long)len);
+        }
+        fprintf(stderr, "Building the suffix position array with compression ratio = %d\n", ratio);
+        rc = gs_build_index_host(gs_bwa_ref_codes(R), gs_bwa_ref_length(R), (uint32_t)ratio, prefix);
+        if (rc == GS_OK)
+          fprintf(stderr, "Wrote %s.bwt and %s.sa (%llu bases, %d sequences, %d ambiguous runs)\n", prefix, prefix,
+                  (unsigned long long)gs_bwa_ref_length(R), gs_bwa_ref_n_contigs(R), gs_bwa_ref_n_holes(R));
+        gs_bwa_ref_free(R);
+      }
+    }
+    if (rc != GS_OK) {
+      fprintf(stderr, "failed (%d)\n", rc);
+      return 1;
+    }
+    return 0;
   }
   if (optind >= argc) {
     fprintf(stderr, "Must specify the FASTA file\nExiting\n");

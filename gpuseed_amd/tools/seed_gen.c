@@ -290,17 +290,23 @@ static void* writer_main(void* a) {
 }
 
 /* ---- index files: the reference's checks and messages (2685-2736) --------- */
+/* The reference's checks of the index headers (seed_gen.cu:2692-2723), with
+ * its messages and exit statuses.  Headers in upstream bwa 0.7's 64-bit
+ * layout (whose 64-bit fields agree) are left to gs_index_open, which checks
+ * them completely. */
 static void check_index_files(const char* prefix) {
   size_t n = strlen(prefix) + 8;
   char* fn = (char*)malloc(n);
-  uint32_t bwt_head[5], sa_head[7];
+  uint32_t bwt_head[10], sa_head[14];
+  memset(bwt_head, 0, sizeof bwt_head);
+  memset(sa_head, 0, sizeof sa_head);
   snprintf(fn, n, "%s.bwt", prefix);
   FILE* fp = fopen(fn, "rb");
   if (!fp) {
     fprintf(stderr, "Unable to open .bwt file.");
     exit(1);
   }
-  if (fread(bwt_head, 4, 5, fp) != 5) {
+  if (fread(bwt_head, 4, 10, fp) < 5) {
     fprintf(stderr, "seed_gen: %s is truncated\n", fn);
     exit(1);
   }
@@ -311,12 +317,18 @@ static void check_index_files(const char* prefix) {
     fprintf(stderr, "Unable to open .sa file.");
     exit(1);
   }
-  if (fread(sa_head, 4, 7, fp) != 7) {
+  if (fread(sa_head, 4, 14, fp) < 7) {
     fprintf(stderr, "seed_gen: %s is truncated\n", fn);
     exit(1);
   }
   fclose(fp);
   free(fn);
+  uint64_t b64[5], s64[7];
+  memcpy(b64, bwt_head, sizeof b64);
+  memcpy(s64, sa_head, sizeof s64);
+  if (b64[4] && s64[0] == b64[0] && s64[6] == b64[4] && s64[5] && b64[0] <= b64[4] && b64[1] <= b64[2] &&
+      b64[2] <= b64[3] && b64[3] <= b64[4])
+    return; /* bwa 0.7 layout */
   if (sa_head[0] != bwt_head[0]) {
     fprintf(stderr, "SA-BWT inconsistency: primary is not the same.\n");
     exit(EXIT_FAILURE);

@@ -113,6 +113,39 @@ GS_API int gs_synth_preset(int config, gs_synth_ref_params* ref, gs_synth_read_p
  * CPU SA-IS; n < 2^32 - 2. */
 GS_API int gs_build_index_host(const uint8_t* codes, uint64_t n, uint32_t sa_intv, const char* prefix);
 
+/* ---- upstream bwa 0.7 files --------------------------------------------
+ * gs_index_open (and gs_index_convert) also read the .bwt/.sa that upstream
+ * bwa 0.7's `bwa index` writes: 64-bit integers and 128-base occurrence
+ * blocks instead of the fork's 32-bit integers and 64-base blocks
+ * (seed_gen.cu:2685-2736), recognised by header and file size.  Its text is
+ * usually the reference followed by its reverse complement; any text of fewer
+ * than 2^32 - 2 bases loads, and seeds are then reported against that text.
+ * No counterpart in the reference, which reads only the fork's files. */
+
+/* Writes <dst_prefix>.bwt/.sa in the fork's layout from either layout at
+ * <src_prefix> (CPU only).  GS_EIO, GS_EFORMAT as gs_index_open. */
+GS_API int gs_index_convert(const char* src_prefix, const char* dst_prefix);
+
+/* bwa's reference files <prefix>.pac/.ann/.amb: the forward reference as codes
+ * 0..3 (runs of N hold the random bases bwa put there; .amb lists them), the
+ * sequences' names and offsets in the concatenated text.  The codes can be
+ * indexed with gs_index_build / gs_build_index_host, which gives the index the
+ * fork's build_index.sh makes for the same FASTA; seed positions are offsets
+ * into that concatenation, and gs_bwa_ref_locate maps them to a sequence.
+ * CPU only.  No counterpart in the reference (which prints raw positions). */
+typedef struct gs_bwa_ref gs_bwa_ref;
+GS_API int gs_bwa_ref_load(const char* prefix, gs_bwa_ref** out);
+GS_API void gs_bwa_ref_free(gs_bwa_ref* ref);
+GS_API uint64_t gs_bwa_ref_length(const gs_bwa_ref* ref);
+GS_API const uint8_t* gs_bwa_ref_codes(const gs_bwa_ref* ref);
+GS_API int gs_bwa_ref_n_contigs(const gs_bwa_ref* ref);
+GS_API int gs_bwa_ref_contig(const gs_bwa_ref* ref, int i, const char** name, uint64_t* offset, uint64_t* len);
+GS_API int gs_bwa_ref_n_holes(const gs_bwa_ref* ref);
+GS_API int gs_bwa_ref_hole(const gs_bwa_ref* ref, int i, uint64_t* offset, uint64_t* len, char* c);
+/* pos in [0, length): the sequence holding it and the offset inside it;
+ * GS_ERANGE beyond the end. */
+GS_API int gs_bwa_ref_locate(const gs_bwa_ref* ref, uint64_t pos, int* contig, uint64_t* offset);
+
 /* ---- the device-resident index -----------------------------------------
  * gs_index_open replaces bwt_restore_bwt + bwt_restore_sa + the upload and the
  * 13-mer table of seed_gen (seed_gen.cu:2685-2736, 2840-2895).  It reads

@@ -66,8 +66,8 @@ def test_seg_tasks_out_of_range_is_refused(g, long_case):
 
 @pytest.fixture(scope="module")
 def mix(g, tmp_path_factory):
-    """Configuration-4 read mix (70 % 100-250 bp, 25 % 1-5 kbp, 5 % 5-20 kbp)
-    on a 4 Mbp reference with configuration 3's repeat structure."""
+    """The uneven read mix (70 % 100-250 bp, 25 % 1-5 kbp, 5 % 5-20 kbp)
+    on a 4 Mbp reference with the repeat structure of the 3.1 Gbp preset."""
     d = tmp_path_factory.mktemp("mix")
     rp, qp = g.preset(4)
     rp.length = 4000000

@@ -27,8 +27,8 @@ def g(built):
 
 @pytest.fixture(scope="module")
 def world(g, tmp_path_factory):
-    """A 3 Mbp reference with configuration 3's repeat structure and a pool of
-    configuration-4 reads (100 bp to 20 kbp)."""
+    """A 3 Mbp reference with the repeat structure of the 3.1 Gbp preset and a pool of
+    uneven reads (100 bp to 20 kbp)."""
     d = tmp_path_factory.mktemp("serve")
     rp, qp = g.preset(4)
     rp.length = 3000000

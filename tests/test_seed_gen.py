@@ -188,7 +188,7 @@ def test_cli_argument_errors(built, c1):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("flags", [
-    ["-k", "20"],                                   # configuration 1: MEM, k=20
+    ["-k", "20"],                                   # MEM, k=20 (the 1 Mbp preset)
     ["-s", "-k", "19"],
     ["-s", "-k", "19", "--kernel", "0"],
     ["-k", "20", "--device-bases", "300000"],       # one reference batch per device batch
