@@ -121,7 +121,7 @@ def make_requests(pool, n, max_reads, seed):
     return out
 
 
-def run_load(g, X, requests, rate, opts, exclusive, seed, label, growth=0):
+def run_load(g, X, requests, rate, opts, exclusive, seed, label, growth=0, workers=0):
     """Poisson arrivals at `rate` requests/s (0: all at once) into a server;
     returns latencies (server clock: submit to results complete) and stats."""
     import queue
@@ -133,7 +133,7 @@ def run_load(g, X, requests, rate, opts, exclusive, seed, label, growth=0):
     arrivals = np.cumsum(gaps) - gaps[0]
     lat, infos, errors = [], [], []
     tickets = queue.Queue()
-    with g.Server(X, opts=opts, exclusive=exclusive, round_growth=growth) as S:
+    with g.Server(X, opts=opts, exclusive=exclusive, round_growth=growth, workers=workers) as S:
         def waiter():
             while True:
                 t = tickets.get()
@@ -190,14 +190,15 @@ def serve_bench(g, X, ref, k, max_occ, args):
     plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
     # peak: a burst the server drains at full speed
     burst = make_requests(pool, args.serve_burst, 200000, seed=50)
-    peak = run_load(g, X, burst, 0, opts, False, 1, "peak (burst of %d)" % len(burst), args.serve_growth)
+    peak = run_load(g, X, burst, 0, opts, False, 1, "peak (burst of %d)" % len(burst), args.serve_growth,
+                    args.serve_workers)
     levels = []
     for i, frac in enumerate(args.serve_levels):
         rate = frac * peak["requests_per_s"]
         n = max(20, int(rate * args.serve_seconds))
         reqs = make_requests(pool, n, 200000, seed=100 + i)
         b = run_load(g, X, reqs, rate, opts, False, 200 + i, "batched at %.0f%% (%.1f req/s)" % (frac * 100, rate),
-                     args.serve_growth)
+                     args.serve_growth, args.serve_workers)
         nb = min(n, args.serve_plain_requests)
         p = None if not nb else run_load(g, X, reqs[:nb], rate, plain, True, 200 + i,
                                          "plain per-request at %.0f%%" % (frac * 100))
@@ -279,6 +280,7 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=60, help="requests of the peak burst")
     ap.add_argument("--serve-plain-requests", type=int, default=25,
                     help="requests per load level for the plain per-request server (0: none)")
+    ap.add_argument("--serve-workers", type=int, default=0, help="device workers of the server (0: its default)")
     ap.add_argument("--serve-growth", type=int, default=0,
                     help="rounds grow up to this many times 16 Mbases under backlog (0: fixed rounds)")
     args = ap.parse_args()
@@ -304,7 +306,13 @@ def main():
     if not args.cpu_sample:
         args.cpu_sample = 20000 if args.config == 4 else 1000000
     n_batches = 1 if reuse else args.warmup + args.steps
-    qp.n_reads = n_batches * args.batch
+    # at most about 120 GB of resident batches (candidate slots, 32 bytes per
+    # task, dominate); further steps cycle through them, each re-running the
+    # whole pipeline on its batch
+    w = [qp.classes[i].weight for i in range(qp.n_classes)]
+    mean_len = sum(wi * (qp.classes[i].len_min + qp.classes[i].len_max) / 2 for i, wi in enumerate(w)) / sum(w)
+    resident = max(1, min(n_batches, int(120e9 // ((36.0 * mean_len + 200) * args.batch))))
+    qp.n_reads = resident * args.batch
 
     log("generating %.2f Gbp reference (config %d)" % (rp.length / 1e9, args.config))
     ref = g.synth_reference(rp)
@@ -337,15 +345,14 @@ def main():
     serve_ref = ref if (args.config == 3 and not args.no_serve and not args.profile_steps) else None
     del ref
     batches = []
-    for b in range(n_batches):
+    for b in range(resident):
         lo, hi = b * args.batch, (b + 1) * args.batch
         base = int(offs[lo])
         end = int(offs[hi - 1] + lens[hi - 1])
         arrays = (buf[base:end], offs[lo:hi] - np.uint64(base), lens[lo:hi])
         batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks)))
-    log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (n_batches, args.batch, len(buf) / 1e9))
-    if reuse:
-        batches = batches * (args.warmup + args.steps)
+    log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (resident, args.batch, len(buf) / 1e9))
+    batches = [batches[i % resident] for i in range(args.warmup + args.steps)]
 
     elapsed, stats = timed_steps(batches, args.warmup, args.kernel)
     reads = args.steps * args.batch

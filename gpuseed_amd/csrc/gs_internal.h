@@ -142,6 +142,17 @@ gs_batch* batch_new(gs_index* X);
 int batch_load(gs_batch* B, const BatchPlan& P);
 void batch_free(gs_batch* B);
 
+// Results of the batch's last run, downloaded into pinned staging that the
+// batch keeps (valid until the next download or the batch's end).
+struct ResultView {
+  uint64_t n_reads, n_seeds, n_pos;
+  const uint32_t *nseeds, *npos;
+  const uint64_t *soff, *poff;
+  const gs_seed_rec* seeds;
+  const uint32_t* pos;
+};
+int batch_download(gs_batch* B, ResultView* v);
+
 }  // namespace gs
 
 // Tuning fields of an existing batch (the autotuner's knobs that need no new

@@ -479,17 +479,24 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
 // earlier pieces; K candidates so far, LB = left end of the last finished
 // task so far.  A piece filters its own tasks in the lane, but its first
 // finished task had no predecessor there: it survives iff its left end is
-// smaller than LB (finish_task, 991-1018).  Candidates only move toward the
-// region's end they are packed against, in an order that reads every source
-// before it can be overwritten.
+// smaller than LB (finish_task, 991-1018).  Candidate i moves from t0 + i to
+// K + i - drop (counted from the end the strand's region is packed against),
+// never toward a higher source index, so a candidate's destination is a
+// source that was read before it: moving `width` consecutive candidates at
+// once (all reads of a group before its writes) keeps that order.  width is
+// 1 for one thread, 64 for a wave whose lanes hold the same K and LB.
 __device__ __forceinline__ void join_piece(uint4* out, uint32_t ntask, uint32_t strand, uint32_t t0, uint4 m,
-                                           uint32_t smem, uint32_t& K, uint32_t& LB) {
+                                           uint32_t smem, uint32_t& K, uint32_t& LB, uint32_t lane = 0,
+                                           uint32_t width = 1) {
   const uint32_t drop = (smem && (m.w & 1u) && !(m.y < LB)) ? 1u : 0u;
-  for (uint32_t i = drop; i < m.x; ++i) {
-    if (strand == 0)
-      out[ntask - 1 - K - (i - drop)] = out[ntask - 1 - t0 - i];
-    else
-      out[ntask + K + (i - drop)] = out[ntask + t0 + i];
+  for (uint32_t i0 = drop; i0 < m.x; i0 += width) {
+    const uint32_t i = i0 + lane;
+    if (i < m.x) {
+      if (strand == 0)
+        out[ntask - 1 - K - (i - drop)] = out[ntask - 1 - t0 - i];
+      else
+        out[ntask + K + (i - drop)] = out[ntask + t0 + i];
+    }
   }
   K += m.x - drop;
   if (m.w & 2u) LB = m.z;
@@ -497,17 +504,21 @@ __device__ __forceinline__ void join_piece(uint4* out, uint32_t ntask, uint32_t 
 
 // the pieces linked after an item, in task order (at most dyn_cap of them)
 __device__ __forceinline__ void join_chain(const gs::JoinParams& J, uint4* out, uint32_t ntask, uint32_t strand,
-                                           uint32_t link, uint32_t& K, uint32_t& LB) {
+                                           uint32_t link, uint32_t& K, uint32_t& LB, uint32_t lane = 0,
+                                           uint32_t width = 1) {
   for (uint32_t guard = 0; link != kNone && guard < J.dyn_cap; ++guard) {
-    join_piece(out, ntask, strand, J.dyn_items[link].z, J.dyn_meta[link], J.smem, K, LB);
+    join_piece(out, ntask, strand, J.dyn_items[link].z, J.dyn_meta[link], J.smem, K, LB, lane, width);
     link = J.dyn_link[link];
   }
 }
 
-// Streams of long reads (one thread per stream): segments in order, each
-// followed by the chain of pieces split off it.
+// Streams of long reads, one wave per stream (a 20 kbp read has about 20,000
+// candidates per strand to move): segments in order, each followed by the
+// chain of pieces split off it; every lane follows the same metadata and the
+// lanes move 64 candidates at a time.
 __global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
-  const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  const uint64_t t = blockIdx.x * 4ull + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
   if (t >= 2ull * J.n_long) return;
   const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
   const uint32_t L = J.lens[r];
@@ -519,10 +530,10 @@ __global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
   uint32_t K = 0, LB = 0xFFFFFFFFu;
   for (uint32_t g = 0; g < nseg; ++g) {
     const uint64_t it = i0 + 2 * (uint64_t)g;
-    join_piece(out, ntask, strand, g * S, J.seg_meta[it], J.smem, K, LB);
-    if (J.dyn_cap) join_chain(J, out, ntask, strand, J.seg_link[it], K, LB);
+    join_piece(out, ntask, strand, g * S, J.seg_meta[it], J.smem, K, LB, lane, 64);
+    if (J.dyn_cap) join_chain(J, out, ntask, strand, J.seg_link[it], K, LB, lane, 64);
   }
-  (strand ? J.nR : J.nF)[r] = K;
+  if (lane == 0) (strand ? J.nR : J.nF)[r] = K;
 }
 
 // Whole-read streams that were split (one thread per stream of the batch;
@@ -635,8 +646,8 @@ int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t
 
 int launch_join(const JoinParams& J, hipStream_t st) {
   if (J.n_long) {
-    const uint64_t threads = 2ull * J.n_long;
-    hipLaunchKernelGGL(join_segments, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, J);
+    const uint64_t waves = 2ull * J.n_long;  // four streams per workgroup of 256
+    hipLaunchKernelGGL(join_segments, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, J);
   }
   if (J.dyn_cap && J.n_reads) {
     const uint64_t threads = 2ull * J.n_reads;

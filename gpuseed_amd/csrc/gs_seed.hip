@@ -282,6 +282,9 @@ struct PostParams {
   const uint64_t* pos_off;
   gs_seed_rec* seeds;
   uint32_t* rows;
+  uint32_t long_min;          // reads longer than this go to finalize_long
+  uint32_t n_long;
+  const uint32_t* long_reads;
 };
 
 struct ReadLists {
@@ -400,48 +403,121 @@ __device__ uint2 mem_skip(const PostParams& P, uint32_t r, int rev, uint32_t i) 
 // read.  Write pass: seed records and suffix-array rows at the scanned offsets,
 // merged by qbeg with forward entries first on ties (the stable segmented sort
 // at 1233 and 3570).
+// One entry of the merged lists: its seed record and rows (write pass), or
+// only its counts; the seeds and positions it adds.
+template <bool WRITE>
+__device__ __forceinline__ void finalize_entry(const PostParams& P, uint32_t r, const ReadLists& Ls, int rev,
+                                               uint32_t i, uint64_t so, uint64_t po, uint32_t& ns,
+                                               uint32_t& np) {
+  const uint4 en = rev ? Ls.R[i] : Ls.F[i];
+  const uint32_t sz = isize(en);
+  uint32_t cnt = P.smem ? sz : mem_count(Ls, rev, i);
+  if (cnt == 0) return;
+  if (P.max_occ && sz > P.max_occ) cnt = 0;
+  if (WRITE) {
+    gs_seed_rec rec;
+    rec.qbeg = en.x;
+    rec.qend = en.y;
+    rec.l = en.z;
+    rec.u = en.w;
+    rec.n_pos = cnt;
+    rec.pad = 0;
+    P.seeds[so + ns] = rec;
+    if (cnt) {
+      uint2 skip = make_uint2(1, 0);
+      if (!P.smem) skip = mem_skip(P, r, rev, i);
+      uint32_t row = en.z, got = 0;
+      while (got < cnt) {
+        if (row < skip.x || row > skip.y) P.rows[po + np + got++] = row;
+        ++row;
+      }
+    }
+  }
+  ++ns;
+  np += cnt;
+}
+
+// the next entry of the merge: forward entries first on ties of qbeg
+__device__ __forceinline__ bool take_forward(const ReadLists& Ls, uint32_t a, uint32_t z) {
+  return z >= Ls.nR || (a < Ls.nF && Ls.F[a].x <= (Ls.R[z].x & 0x7FFFFFFFu));
+}
+
 template <bool WRITE>
 __global__ void __launch_bounds__(256) finalize_reads(PostParams P) {
   const uint32_t r = blockIdx.x * 256 + threadIdx.x;
   if (r >= P.n_reads) return;
+  if (P.lens[r] > P.long_min && P.lens[r] >= P.k) return;  // finalize_long's (the reads of long_reads)
   const ReadLists Ls = lists_of(P, r);
   uint32_t ns = 0, np = 0;
   uint64_t so = WRITE ? P.seed_off[r] : 0, po = WRITE ? P.pos_off[r] : 0;
   uint32_t a = 0, z = 0;
   while (a < Ls.nF || z < Ls.nR) {
-    const bool takeF = z >= Ls.nR || (a < Ls.nF && Ls.F[a].x <= (Ls.R[z].x & 0x7FFFFFFFu));
-    const int rev = takeF ? 0 : 1;
-    const uint32_t i = takeF ? a++ : z++;
-    const uint4 en = takeF ? Ls.F[i] : Ls.R[i];
-    const uint32_t sz = isize(en);
-    uint32_t cnt = P.smem ? sz : mem_count(Ls, rev, i);
-    if (cnt == 0) continue;
-    if (P.max_occ && sz > P.max_occ) cnt = 0;
-    if (WRITE) {
-      gs_seed_rec rec;
-      rec.qbeg = en.x;
-      rec.qend = en.y;
-      rec.l = en.z;
-      rec.u = en.w;
-      rec.n_pos = cnt;
-      rec.pad = 0;
-      P.seeds[so + ns] = rec;
-      if (cnt) {
-        uint2 skip = make_uint2(1, 0);
-        if (!P.smem) skip = mem_skip(P, r, rev, i);
-        uint32_t row = en.z, got = 0;
-        while (got < cnt) {
-          if (row < skip.x || row > skip.y) P.rows[po + np + got++] = row;
-          ++row;
-        }
-      }
-    }
-    ++ns;
-    np += cnt;
+    const bool takeF = take_forward(Ls, a, z);
+    finalize_entry<WRITE>(P, r, Ls, takeF ? 0 : 1, takeF ? a++ : z++, so, po, ns, np);
   }
   if (!WRITE) {
     P.nseeds[r] = ns;
     P.npos[r] = np;
+  }
+}
+
+// Reads longer than long_min bases (thousands of entries each): one wave per
+// read.  Lane l takes positions [l*T/64, (l+1)*T/64) of the merged order of
+// the T = nF + nR entries; its starting split between the two lists is found
+// by binary search on the merge path (the number a of forward entries among
+// the first d merged is the smallest a at which forward entry a does not come
+// before reverse entry d-a-1).  The count pass sums the lanes' seeds and
+// positions; the write pass counts first, scans the lanes' counts for their
+// output offsets, then merges its part again and writes.
+template <bool WRITE>
+__global__ void __launch_bounds__(256) finalize_long(PostParams P) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (w >= P.n_long) return;
+  const uint32_t r = P.long_reads[w];
+  const ReadLists Ls = lists_of(P, r);
+  const uint32_t T = Ls.nF + Ls.nR;
+  const uint32_t d0 = (uint32_t)((uint64_t)T * lane / 64), d1 = (uint32_t)((uint64_t)T * (lane + 1) / 64);
+  uint32_t lo = d0 > Ls.nR ? d0 - Ls.nR : 0, hi = d0 < Ls.nF ? d0 : Ls.nF;
+  while (lo < hi) {
+    const uint32_t a = (lo + hi) >> 1, b = d0 - a;
+    if (b > 0 && Ls.F[a].x <= (Ls.R[b - 1].x & 0x7FFFFFFFu))
+      lo = a + 1;  // forward entry a is merged before reverse entry b-1
+    else
+      hi = a;
+  }
+  const uint32_t a0 = lo, z0 = d0 - lo;
+  uint32_t ns = 0, np = 0;
+  {
+    uint32_t a = a0, z = z0;
+    for (uint32_t d = d0; d < d1; ++d) {
+      const bool takeF = take_forward(Ls, a, z);
+      finalize_entry<false>(P, r, Ls, takeF ? 0 : 1, takeF ? a++ : z++, 0, 0, ns, np);
+    }
+  }
+  // inclusive scans over the lanes (64-bit positions: a read's rows can
+  // exceed 2^32 only through pathological counts, so sum in 64 bits)
+  uint64_t sn = ns, sp = np;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t tn = __shfl_up(sn, o, 64), tp = __shfl_up(sp, o, 64);
+    if ((int)lane >= o) {
+      sn += tn;
+      sp += tp;
+    }
+  }
+  if (!WRITE) {
+    if (lane == 63) {
+      P.nseeds[r] = (uint32_t)sn;
+      P.npos[r] = (uint32_t)sp;
+    }
+    return;
+  }
+  const uint64_t so = P.seed_off[r] + (sn - ns), po = P.pos_off[r] + (sp - np);
+  uint32_t ws = 0, wp = 0;
+  uint32_t a = a0, z = z0;
+  for (uint32_t d = d0; d < d1; ++d) {
+    const bool takeF = take_forward(Ls, a, z);
+    finalize_entry<true>(P, r, Ls, takeF ? 0 : 1, takeF ? a++ : z++, so, po, ws, wp);
   }
 }
 
@@ -646,6 +722,31 @@ struct gs_batch {
   uint32_t *hr_ns = nullptr, *hr_np = nullptr, *hr_rows = nullptr;
   uint64_t *hr_soff = nullptr, *hr_poff = nullptr;
   gs_seed_rec* hr_seeds = nullptr;
+  // pinned result staging of batch_download (grow-only; the server reuses it
+  // round after round)
+  struct Pinned {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+      if (bytes <= cap) return GS_OK;
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      const size_t want = bytes + bytes / 4 + 4096;
+      if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        return GS_ENOMEM;
+      }
+      cap = want;
+      return GS_OK;
+    }
+    void release() {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+  };
+  Pinned dl_ns, dl_np, dl_soff, dl_poff, dl_seeds, dl_pos;
 };
 
 static void free_graph(gs_batch* B);
@@ -655,6 +756,7 @@ static void destroy_batch(gs_batch* B) {
   if (!B) return;
   free_graph(B);
   free_host_results(B);
+  for (auto* q : {&B->dl_ns, &B->dl_np, &B->dl_soff, &B->dl_poff, &B->dl_seeds, &B->dl_pos}) q->release();
   if (B->h_tot) (void)hipHostFree(B->h_tot);
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
@@ -915,6 +1017,10 @@ static void fill_params(gs_batch* B, SeedParams& S, PostParams& Q) {
   Q.pos_off = B->poff.as<uint64_t>();
   Q.seeds = B->seeds.as<gs_seed_rec>();
   Q.rows = B->rows.as<uint32_t>();
+  // reads longer than the persistent kernel's window are finalized a wave each
+  Q.long_min = B->n_long ? gs::kPersistLmax : 0xFFFFFFFFu;
+  Q.n_long = B->n_long;
+  Q.long_reads = B->long_reads.as<uint32_t>();
 }
 
 static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsigned pgrid, hipStream_t st) {
@@ -990,6 +1096,7 @@ static int enqueue_counts(gs_batch* B, const PostParams& Q, hipStream_t st) {
   const uint64_t n = B->n_reads;
   if (n) {
     hipLaunchKernelGGL(finalize_reads<false>, dim3(grid_for(n, 256)), dim3(256), 0, st, Q);
+    if (Q.n_long) hipLaunchKernelGGL(finalize_long<false>, dim3((Q.n_long + 3) / 4), dim3(256), 0, st, Q);
     scan_counts(Q.nseeds, n, B->soff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
     scan_counts(Q.npos, n, B->poff.as<uint64_t>(), B->tiles.as<uint64_t>(), B->tot.as<uint64_t>() + 1, st);
     GS_HIP(hipMemcpyAsync(B->h_tot, B->tot.p, 16, hipMemcpyDeviceToHost, st));
@@ -1002,6 +1109,7 @@ static int enqueue_counts(gs_batch* B, const PostParams& Q, hipStream_t st) {
 
 static int enqueue_write(gs_batch* B, const PostParams& Q, hipStream_t st) {
   if (B->n_reads) hipLaunchKernelGGL(finalize_reads<true>, dim3(grid_for(B->n_reads, 256)), dim3(256), 0, st, Q);
+  if (Q.n_long) hipLaunchKernelGGL(finalize_long<true>, dim3((Q.n_long + 3) / 4), dim3(256), 0, st, Q);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
@@ -1267,6 +1375,36 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
   out->t_d2h = ms * 1e-3;
+  return GS_OK;
+}
+
+int gs::batch_download(gs_batch* B, ResultView* v) {
+  if (!B || !v || !B->ran) return GS_EINVAL;
+  if (hipSetDevice(B->X->device) != hipSuccess) return GS_EHIP;
+  const uint64_t n = B->n_reads, nr1 = n + 1;
+  if (B->dl_ns.ensure(4 * nr1) || B->dl_np.ensure(4 * nr1) || B->dl_soff.ensure(8 * nr1) ||
+      B->dl_poff.ensure(8 * nr1) || B->dl_seeds.ensure(sizeof(gs_seed_rec) * (B->n_seeds + 1)) ||
+      B->dl_pos.ensure(4 * (B->n_pos + 1)))
+    return GS_ENOMEM;
+  hipStream_t st = B->stream;
+  if ((n && (hipMemcpyAsync(B->dl_ns.p, B->ns.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipMemcpyAsync(B->dl_np.p, B->np.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+      hipMemcpyAsync(B->dl_soff.p, B->soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(B->dl_poff.p, B->poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      (B->n_seeds && hipMemcpyAsync(B->dl_seeds.p, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds,
+                                    hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      (B->n_pos && hipMemcpyAsync(B->dl_pos.p, B->rows.p, 4 * B->n_pos, hipMemcpyDeviceToHost, st) != hipSuccess) ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return GS_EHIP;
+  v->n_reads = n;
+  v->n_seeds = B->n_seeds;
+  v->n_pos = B->n_pos;
+  v->nseeds = static_cast<const uint32_t*>(B->dl_ns.p);
+  v->npos = static_cast<const uint32_t*>(B->dl_np.p);
+  v->soff = static_cast<const uint64_t*>(B->dl_soff.p);
+  v->poff = static_cast<const uint64_t*>(B->dl_poff.p);
+  v->seeds = static_cast<const gs_seed_rec*>(B->dl_seeds.p);
+  v->pos = static_cast<const uint32_t*>(B->dl_pos.p);
   return GS_OK;
 }
 

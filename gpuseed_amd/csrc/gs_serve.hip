@@ -9,14 +9,17 @@
 //  * a planner builds the next round on the CPU while the device works:
 //    every request with unscheduled reads first gets a fair share of the
 //    round (round_bases / active requests, at least one reference batch),
-//    then the round is filled in arrival order up to round_bases and to the
-//    device budget divided among the rounds that can hold memory at once;
-//  * `workers` device threads (default 2), each with its own batch and HIP
+//    then the round is filled in arrival order up to the round size and to
+//    the device budget divided among the rounds that can hold memory at
+//    once.  The round size is round_bases, growing with the backlog up to
+//    round_growth x round_bases (a launch's tail is a fixed cost);
+//  * `workers` device threads (default 3), each with its own batch and HIP
 //    stream, take rounds in turn: upload, the whole pipeline (one
 //    persistent-kernel launch over the reads of every request in the round),
-//    download, then hand each chunk's share to its request.  Two workers keep
-//    two rounds on the GPU at once, so the waves of the next round fill the
-//    compute units that the tail of the previous one leaves idle.
+//    download into the batch's pinned staging, then hand each chunk's share
+//    to its request.  Several rounds on the GPU at once let the waves of the
+//    next round fill the compute units that the tail of another leaves idle,
+//    and hide each worker's host work (upload, download, hand-off).
 // Requests arriving while rounds run join the next one; a large request is
 // spread over several rounds, interleaved with the requests behind it, and
 // its chunks are put back in read order when it is collected.
@@ -128,10 +131,10 @@ void advance(Request* q, uint64_t end) {
 std::unique_ptr<Round> plan_round(gs_server* S) {
   auto R = std::make_unique<Round>();
   gs::plan_reset(R->plan, S->o.opts);
-  // rounds may grow with the backlog, up to round_growth x round_bases: a
-  // launch has a fixed cost (its tail), so larger rounds raise throughput
-  // under load, at the price of longer waits for the requests that arrive
-  // behind them (fixed-size rounds are the default)
+  // rounds grow with the backlog, up to round_growth x round_bases: a launch
+  // has a fixed cost (its tail), so larger rounds raise throughput under
+  // load, at the price of longer waits for the requests that arrive behind
+  // them; a short queue keeps rounds small
   uint64_t backlog = 0;
   for (Request* q : S->active) backlog += q->n_bases - q->sched_bases;
   const uint64_t grown = backlog / (uint64_t)S->o.workers;
@@ -244,9 +247,9 @@ void device_main(gs_server* S, gs_batch* B) {
     memset(&bs, 0, sizeof bs);
     if (!rc) rc = gs_batch_run(B, -1, &bs);
     const double t2 = S->now();
-    gs_result res;
+    gs::ResultView res;  // pinned staging of this worker's batch
     memset(&res, 0, sizeof res);
-    if (!rc) rc = gs_batch_fetch(B, &res);
+    if (!rc) rc = gs::batch_download(B, &res);
     const double t3 = S->now();
     if (!rc) {
       for (size_t i = 0; i < R->chunks.size(); ++i) {  // each chunk's results into its piece
@@ -254,13 +257,12 @@ void device_main(gs_server* S, gs_batch* B) {
         if (skip[i]) continue;
         Request::Piece& pc = *c.piece;
         const uint64_t a = c.at, z = c.at + (c.r1 - c.r0);
-        pc.nseeds.assign(res.read_nseeds + a, res.read_nseeds + z);
-        pc.npos.assign(res.read_npos + a, res.read_npos + z);
-        pc.seeds.assign(res.seeds + res.read_seed_off[a], res.seeds + res.read_seed_off[z]);
-        pc.pos.assign(res.pos + res.read_pos_off[a], res.pos + res.read_pos_off[z]);
+        pc.nseeds.assign(res.nseeds + a, res.nseeds + z);
+        pc.npos.assign(res.npos + a, res.npos + z);
+        pc.seeds.assign(res.seeds + res.soff[a], res.seeds + res.soff[z]);
+        pc.pos.assign(res.pos + res.poff[a], res.pos + res.poff[z]);
       }
     }
-    gs_result_free(&res);
     const double t_end = S->now();
     lk.lock();
     --S->running;
@@ -306,9 +308,9 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (!S->o.budget_bytes) S->o.budget_bytes = 64ull << 30;
   if (!S->o.round_bases) S->o.round_bases = 16000000ull;
   if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 1;
-  if (S->o.workers <= 0) S->o.workers = S->o.exclusive ? 1 : 2;
+  if (S->o.workers <= 0) S->o.workers = S->o.exclusive ? 1 : 3;
   if (S->o.workers > 8) S->o.workers = 8;
-  if (S->o.round_growth <= 0) S->o.round_growth = 1;
+  if (S->o.round_growth <= 0) S->o.round_growth = S->o.exclusive ? 1 : 4;
   if (S->o.round_growth > 64) S->o.round_growth = 64;
   S->t0 = Clock::now();
   for (int w = 0; w < S->o.workers; ++w) S->batches.push_back(gs::batch_new(X));

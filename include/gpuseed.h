@@ -314,7 +314,7 @@ GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64_t* offsets
  * round is filled in arrival order), bounded by round_bases and by a share of
  * budget_bytes of device memory for read and seed buffers (the budget over
  * workers + 1 rounds); device threads run rounds as one pipeline each (one
- * persistent-kernel launch over all its requests), two rounds on the GPU at
+ * persistent-kernel launch over all its requests), three rounds on the GPU at
  * once by default, while the planner prepares the next.  Requests arriving
  * during a round join the next one.  A request's results are exactly those of gs_seed
  * on the request alone.  The caller's read buffers must stay valid until
@@ -329,11 +329,11 @@ typedef struct {
   int exclusive;         /* 1: a round holds one whole request, in arrival order
                             (the per-request baseline)                        */
   int workers;           /* device threads, each with its own stream and round
-                            buffers (0 = 2, or 1 when exclusive; at most 8)   */
+                            buffers (0 = 3, or 1 when exclusive; at most 8)   */
   int round_growth;      /* rounds grow to backlog / workers bases, at most
-                            round_growth x round_bases, at most 64 (0 or 1:
-                            fixed size): more throughput under load, longer
-                            waits for small requests                          */
+                            round_growth x round_bases, at most 64 (0 = 4, or
+                            1 when exclusive; 1 = fixed size): more throughput
+                            under load, longer waits for small requests       */
   int reserved;
 } gs_server_opts;
 

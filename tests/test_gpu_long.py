@@ -58,6 +58,25 @@ def test_segments_match_oracle(g, long_case, smem, k, seg_tasks):
     assert not bad, "read %d (len %d)" % (bad[0], len(long_case["reads"][bad[0]]))
 
 
+@pytest.mark.parametrize("smem", [True, False])
+def test_k_above_the_window(g, long_case, smem):
+    """k = 170: reads of 161-169 bases are longer than the LDS window but have
+    no task; longer ones are segmented; every read's seeds equal the oracle's."""
+    k = 170
+    say("long reads k=%d smem=%d" % (k, smem))
+    reads = [r for r in long_case["reads"] if len(r) > 150][:300]
+    reads += [r[:165] for r in reads[:20]] + [r[:170] for r in reads[20:30]]
+    ox = ob.Index(long_case["prefix"])
+    want, _ = ob.seed(ox, reads, k, smem=smem, prune=1, threads=8)
+    ox.close()
+    want = [[(x, y, l, u, list(p)) for (x, y, l, u, p) in r] for r in want]
+    with g.Index(long_case["prefix"]) as X:
+        for kernel in (0, 1):
+            got = g.seed(X, reads, k=k, smem=smem, kernel=kernel).as_lists()
+            bad = [i for i in range(len(want)) if got[i] != want[i]]
+            assert not bad, "kernel %d read %d (len %d)" % (kernel, bad[0], len(reads[bad[0]]))
+
+
 def test_seg_tasks_out_of_range_is_refused(g, long_case):
     with g.Index(long_case["prefix"]) as X:
         with pytest.raises(g.GsError):
