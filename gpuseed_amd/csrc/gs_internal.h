@@ -96,7 +96,8 @@ constexpr uint32_t kSplitStream = 0xFFFFFFFFu;  // nF/nR of a whole-read stream 
 constexpr uint32_t kSplitItemsPerLane = 4;      // work splitting only in batches this small
 constexpr uint32_t kRefillGroup = 4;            // idle lanes of a wave that trigger a refill
                                                 // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
-                                                // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms)
+                                                // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms;
+                                                // 2, 3, 4 and 6 within 1 %)
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
 constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start
