@@ -141,6 +141,9 @@ void plan_finish(BatchPlan& P);
 uint64_t plan_device_bytes(const BatchPlan& P);  // device buffers the plan needs, outputs excluded
 gs_batch* batch_new(gs_index* X);
 int batch_load(gs_batch* B, const BatchPlan& P);
+// Grows B's device buffers for any plan of at most `bases` bases and `reads`
+// reads, so that loading such plans allocates (and synchronises) nothing.
+int batch_reserve(gs_batch* B, uint64_t bases, uint64_t reads, uint32_t seg_tasks);
 void batch_free(gs_batch* B);
 
 // Results of the batch's last run, downloaded into pinned staging that the

@@ -873,6 +873,42 @@ static int batch_events(gs_batch* B) {
   return GS_OK;
 }
 
+// Device buffers of a batch of `span` bases, n reads, `tasks` task slots per
+// strand, `code_chunks` 16-byte code chunks, n_items work items (n_long of
+// them long reads), reads of at most maxlen bases and an overflow queue of
+// dyn_cap pieces; buffers only grow.
+static int ensure_buffers(gs_batch* B, uint64_t span, uint64_t n, uint64_t tasks, uint64_t code_chunks,
+                          uint64_t n_items, uint64_t n_long, uint32_t maxlen, uint32_t dyn_cap) {
+  const uint64_t nr1 = n + 1;
+  if (B->codes.ensure(16 * (code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
+      B->items.ensure(8 * (n_items + 1)) || B->long_reads.ensure(4 * (n_long + 1)) ||
+      B->item0.ensure(8 * (n_long + 1)) || B->seg_meta.ensure(n_long ? 16 * n_items : 16) ||
+      B->bases.ensure(span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
+      B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * tasks + 1)) ||
+      B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
+      B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
+      B->tot.ensure(16) || (maxlen > 256 && B->scratch.ensure(4 * (span + 16))) ||
+      B->queue.ensure(16) || B->counters.ensure(32) ||
+      B->seg_link.ensure(n_long ? 4 * n_items : 16) ||
+      (dyn_cap &&
+       (B->dyn_items.ensure(16ull * dyn_cap) || B->dyn_meta.ensure(16ull * dyn_cap) ||
+        B->dyn_link.ensure(4ull * dyn_cap) || B->dyn_ready.ensure(4ull * dyn_cap) ||
+        B->dyn_ctl.ensure(1024) || B->wmeta.ensure(32 * nr1) || B->wlink.ensure(8 * nr1))))
+    return GS_ENOMEM;
+  return GS_OK;
+}
+
+int gs::batch_reserve(gs_batch* B, uint64_t bases, uint64_t reads, uint32_t seg_tasks) {
+  if (!B) return GS_EINVAL;
+  if (hipSetDevice(B->X->device) != hipSuccess) return GS_EHIP;
+  if (!seg_tasks) seg_tasks = 64;
+  // every read and long-read segment is an item of each strand; a read has
+  // at most as many tasks as bases and code chunks as 32-base spans (+1)
+  const uint64_t items = 2 * (reads + bases / seg_tasks + 1);
+  const uint32_t dyn = (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, items));
+  return ensure_buffers(B, bases, reads, bases, 2 * (bases / 32 + reads), items, reads, 0xFFFFFFFFu, dyn);
+}
+
 // Uploads a finished plan into B, reusing B's device buffers where they are
 // large enough.  B's results and any captured graph are dropped.
 int gs::batch_load(gs_batch* B, const BatchPlan& P) {
@@ -907,20 +943,7 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
       split_on = false;
   }
   B->dyn_cap = split_on ? (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, B->n_items)) : 0u;
-  if (B->codes.ensure(16 * (B->code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
-      B->items.ensure(8 * (B->n_items + 1)) || B->long_reads.ensure(4 * (B->n_long + 1)) ||
-      B->item0.ensure(8 * (B->n_long + 1)) || B->seg_meta.ensure(B->n_long ? 16 * B->n_items : 16) ||
-      B->bases.ensure(B->span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
-      B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * B->tasks + 1)) ||
-      B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
-      B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
-      B->tot.ensure(16) || (B->maxlen > 256 && B->scratch.ensure(4 * (B->span + 16))) ||
-      B->queue.ensure(16) || B->counters.ensure(32) ||
-      B->seg_link.ensure(B->n_long ? 4 * B->n_items : 16) ||
-      (B->dyn_cap &&
-       (B->dyn_items.ensure(16ull * B->dyn_cap) || B->dyn_meta.ensure(16ull * B->dyn_cap) ||
-        B->dyn_link.ensure(4ull * B->dyn_cap) || B->dyn_ready.ensure(4ull * B->dyn_cap) ||
-        B->dyn_ctl.ensure(1024) || B->wmeta.ensure(32 * nr1) || B->wlink.ensure(8 * nr1))))
+  if (ensure_buffers(B, B->span, n, B->tasks, B->code_chunks, B->n_items, B->n_long, B->maxlen, B->dyn_cap))
     return GS_ENOMEM;
   hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;

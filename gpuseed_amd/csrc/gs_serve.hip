@@ -26,6 +26,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -252,16 +253,26 @@ void device_main(gs_server* S, gs_batch* B) {
     if (!rc) rc = gs::batch_download(B, &res);
     const double t3 = S->now();
     if (!rc) {
-      for (size_t i = 0; i < R->chunks.size(); ++i) {  // each chunk's results into its piece
-        const Chunk& c = R->chunks[i];
-        if (skip[i]) continue;
-        Request::Piece& pc = *c.piece;
-        const uint64_t a = c.at, z = c.at + (c.r1 - c.r0);
-        pc.nseeds.assign(res.nseeds + a, res.nseeds + z);
-        pc.npos.assign(res.npos + a, res.npos + z);
-        pc.seeds.assign(res.seeds + res.soff[a], res.seeds + res.soff[z]);
-        pc.pos.assign(res.pos + res.poff[a], res.pos + res.poff[z]);
-      }
+      // each chunk's results into its piece; large rounds copy on several
+      // threads (the pieces are disjoint)
+      auto hand_off = [&](size_t first, size_t step) {
+        for (size_t i = first; i < R->chunks.size(); i += step) {
+          const Chunk& c = R->chunks[i];
+          if (skip[i]) continue;
+          Request::Piece& pc = *c.piece;
+          const uint64_t a = c.at, z = c.at + (c.r1 - c.r0);
+          pc.nseeds.assign(res.nseeds + a, res.nseeds + z);
+          pc.npos.assign(res.npos + a, res.npos + z);
+          pc.seeds.assign(res.seeds + res.soff[a], res.seeds + res.soff[z]);
+          pc.pos.assign(res.pos + res.poff[a], res.pos + res.poff[z]);
+        }
+      };
+      const uint64_t bytes = res.n_seeds * sizeof(gs_seed_rec) + 4 * res.n_pos;
+      const size_t nt = std::min<size_t>(R->chunks.size(), bytes > (32u << 20) ? 4 : 1);
+      std::vector<std::thread> helpers;
+      for (size_t t = 1; t < nt; ++t) helpers.emplace_back(hand_off, t, nt);
+      hand_off(0, nt);
+      for (auto& h : helpers) h.join();
     }
     const double t_end = S->now();
     lk.lock();
@@ -314,6 +325,19 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (S->o.round_growth > 64) S->o.round_growth = 64;
   S->t0 = Clock::now();
   for (int w = 0; w < S->o.workers; ++w) S->batches.push_back(gs::batch_new(X));
+  // device buffers for the largest round up front (100 bases per read on
+  // average at most), so that rounds allocate, and synchronise the device,
+  // only when they exceed it
+  const uint64_t max_round = S->o.round_bases * (uint64_t)S->o.round_growth;
+  for (gs_batch* B : S->batches) {
+    rc = gs::batch_reserve(B, max_round, max_round / 100, S->o.opts.seg_tasks);
+    if (rc) break;
+  }
+  if (rc) {
+    for (gs_batch* B : S->batches) gs::batch_free(B);
+    delete S;
+    return rc;
+  }
   S->planner = std::thread(planner_main, S);
   for (int w = 0; w < S->o.workers; ++w) S->devices.emplace_back(device_main, S, S->batches[w]);
   *out = S;
