@@ -186,7 +186,7 @@ def serve_bench(g, X, ref, k, max_occ, args):
     qp.n_reads = args.serve_pool
     log("serving: generating a pool of %d configuration-4 reads" % qp.n_reads)
     pool = g.synth_reads(ref, qp)[:3]
-    opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel)
+    opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel, seg_tasks=args.seg_tasks)
     plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
     # peak: a burst the server drains at full speed
     burst = make_requests(pool, args.serve_burst, 200000, seed=50)
@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement")
     ap.add_argument("--serve-levels", type=float, nargs="*", default=[0.1, 0.5, 0.8, 0.9],
                     help="offered loads as fractions of the measured peak")
-    ap.add_argument("--serve-seconds", type=float, default=6.0, help="arrival window per load level")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="arrival window per load level")
     ap.add_argument("--serve-pool", type=int, default=600000, help="reads in the request pool")
     ap.add_argument("--serve-burst", type=int, default=60, help="requests of the peak burst")
     ap.add_argument("--serve-plain-requests", type=int, default=25,

@@ -319,6 +319,9 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (!S->o.budget_bytes) S->o.budget_bytes = 64ull << 30;
   if (!S->o.round_bases) S->o.round_bases = 16000000ull;
   if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 1;
+  // rounds hold few work items per lane, so shorter segments of long reads
+  // balance better (peak with 64: 1.31 M reads/s, 48: 1.42 M, 32: 1.48 M)
+  if (!S->o.opts.seg_tasks) S->o.opts.seg_tasks = 32;
   if (S->o.workers <= 0) S->o.workers = S->o.exclusive ? 1 : 3;
   if (S->o.workers > 8) S->o.workers = 8;
   if (S->o.round_growth <= 0) S->o.round_growth = S->o.exclusive ? 1 : 4;

@@ -323,7 +323,7 @@ typedef struct gs_server gs_server;
 
 typedef struct {
   gs_opts opts;          /* seeding options of every request; kernel 0 means 1
-                            unless exclusive                                  */
+                            unless exclusive, seg_tasks 0 means 32            */
   uint64_t budget_bytes; /* device memory of the rounds in flight (0 = 64 GiB) */
   uint64_t round_bases;  /* bases per round at most (0 = 16,000,000)         */
   int exclusive;         /* 1: a round holds one whole request, in arrival order
