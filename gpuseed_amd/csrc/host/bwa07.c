@@ -80,9 +80,10 @@ int gs_host_index_load_bwa07(const char* prefix, gs_host_index* idx) {
   const long fsz = file_size(fp);
   if (fsz < 40 || fread(h, 8, 5, fp) != 5) rc = GS_EIO;
   const uint64_t n = rc ? 0 : h[4];
-  if (!rc && (n == 0 || n >= 0xFFFFFFFEull || h[0] > n || h[1] > h[2] || h[2] > h[3] || h[3] > h[4] ||
+  if (!rc && (n == 0 || h[0] > n || h[1] > h[2] || h[2] > h[3] || h[3] > h[4] ||
               (uint64_t)fsz != 40 + 4 * bwa07_words(n)))
     rc = GS_EFORMAT;
+  if (!rc && n >= 0xFFFFFFFEull) rc = GS_ERANGE; /* e.g. both strands of a human genome: 6.2 G */
   uint32_t* src = NULL;
   const uint64_t src_words = rc ? 0 : bwa07_words(n);
   if (!rc && !(src = (uint32_t*)malloc(4 * src_words))) rc = GS_ENOMEM;

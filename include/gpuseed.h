@@ -119,7 +119,8 @@ GS_API int gs_build_index_host(const uint8_t* codes, uint64_t n, uint32_t sa_int
  * blocks instead of the fork's 32-bit integers and 64-base blocks
  * (seed_gen.cu:2685-2736), recognised by header and file size.  Its text is
  * usually the reference followed by its reverse complement; any text of fewer
- * than 2^32 - 2 bases loads, and seeds are then reported against that text.
+ * than 2^32 - 2 bases loads (longer ones give GS_ERANGE), and seeds are then
+ * reported against that text.
  * No counterpart in the reference, which reads only the fork's files. */
 
 /* Writes <dst_prefix>.bwt/.sa in the fork's layout from either layout at

@@ -269,3 +269,18 @@ def test_build_index_tool_reads_bwa_files(built, tmp_path):
     p = subprocess.run([tool, "-p", str(tmp_path / "x"), "-c", str(tmp_path / "missing")], stderr=subprocess.PIPE,
                        timeout=60)
     assert p.returncode == 1
+
+
+def test_bwa07_text_beyond_32_bits_is_out_of_range(built, tmp_path):
+    """Both strands of a human genome (6.2 G bases) exceed the 32-bit rows the
+    device index uses: GS_ERANGE, decided from the header (the body of this
+    file is sparse and never read)."""
+    import gpuseed_amd as g
+    n = 1 << 32
+    p = str(tmp_path / "big")
+    with open(p + ".bwt", "wb") as f:
+        f.write(struct.pack("<5Q", 12345, n // 4, n // 2, 3 * n // 4, n))
+    os.truncate(p + ".bwt", 40 + 4 * ((n + 15) // 16 + 8 * ((n + 127) // 128 + 1)))
+    with pytest.raises(g.GsError) as e:
+        g.index_convert(p, str(tmp_path / "cv"))
+    assert e.value.rc == -7
