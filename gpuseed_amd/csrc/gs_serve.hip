@@ -81,6 +81,7 @@ struct Round {
   gs::BatchPlan plan;
   std::vector<Chunk> chunks;
   uint64_t device_bytes = 0;
+  int rc = GS_OK;  // the plan could not be built: the round's requests fail with it
 };
 
 }  // namespace
@@ -101,6 +102,7 @@ struct gs_server {
   std::vector<std::thread> devices;
   std::vector<gs_batch*> batches;  // one per device thread
   int running = 0;                 // rounds on the device
+  bool building = false;           // the planner builds a round outside the lock
   gs_server_stats st{};
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -128,10 +130,11 @@ void advance(Request* q, uint64_t end) {
   q->next_read = end;
 }
 
-// Builds the next round from the active requests (the server lock is held).
+// Chooses the next round's chunks from the active requests (the server lock
+// is held); build_round then makes its plan without the lock.
 std::unique_ptr<Round> plan_round(gs_server* S) {
   auto R = std::make_unique<Round>();
-  gs::plan_reset(R->plan, S->o.opts);
+  uint64_t n_reads = 0;
   // rounds grow with the backlog, up to round_growth x round_bases: a launch
   // has a fixed cost (its tail), so larger rounds raise throughput under
   // load, at the price of longer waits for the requests that arrive behind
@@ -152,9 +155,9 @@ std::unique_ptr<Round> plan_round(gs_server* S) {
     for (uint64_t i = q->next_read; i < end; ++i) b += q->lens[i];
     const uint64_t e = chunk_bytes(b, end - q->next_read);
     if (!R->chunks.empty() && est + e > mem_cap) return false;  // over the budget: a later round
-    if (R->plan.n_reads + (end - q->next_read) > 0x7FFFFFFFull) return false;
-    const uint64_t at = R->plan.n_reads;
-    if (gs::plan_add(R->plan, q->bases, q->offs, q->lens, q->next_read, end) != GS_OK) return false;
+    if (n_reads + (end - q->next_read) > 0x7FFFFFFFull) return false;
+    const uint64_t at = n_reads;
+    n_reads += end - q->next_read;
     bases += b;
     est += e;
     q->sched_bases += b;
@@ -194,9 +197,23 @@ std::unique_ptr<Round> plan_round(gs_server* S) {
   }
   for (auto it = S->active.begin(); it != S->active.end();)
     it = (*it)->next_read >= (*it)->n ? S->active.erase(it) : it + 1;
-  gs::plan_finish(R->plan);
-  R->device_bytes = gs::plan_device_bytes(R->plan);
   return R;
+}
+
+// The round's plan, from its chunks in order (no lock needed: the requests'
+// read arrays do not change, and each request stays alive while it has
+// chunks in flight).
+void build_round(gs_server* S, Round& R) {
+  gs::plan_reset(R.plan, S->o.opts);
+  for (const Chunk& c : R.chunks) {
+    const int rc = gs::plan_add(R.plan, c.q->bases, c.q->offs, c.q->lens, c.r0, c.r1);
+    if (rc) {
+      R.rc = rc;
+      return;
+    }
+  }
+  gs::plan_finish(R.plan);
+  R.device_bytes = gs::plan_device_bytes(R.plan);
 }
 
 void finish_request(gs_server* S, Request* q, int rc) {
@@ -213,12 +230,17 @@ void planner_main(gs_server* S) {
     if (S->active.empty() && S->stop) break;
     if (S->ready || S->active.empty()) continue;
     Request* front = S->active.front();
-    S->ready = plan_round(S);
-    if (S->ready->chunks.empty()) {  // nothing could be planned for the front request
-      S->ready.reset();
+    std::unique_ptr<Round> R = plan_round(S);
+    if (R->chunks.empty()) {  // nothing could be planned for the front request
       if (!front->done) finish_request(S, front, GS_ERANGE);
       continue;
     }
+    S->building = true;  // workers wait for this round even if no request is active
+    lk.unlock();
+    build_round(S, *R);
+    lk.lock();
+    S->building = false;
+    S->ready = std::move(R);
     S->cv_dev.notify_all();
   }
   S->cv_dev.notify_all();
@@ -227,7 +249,7 @@ void planner_main(gs_server* S) {
 void device_main(gs_server* S, gs_batch* B) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
-    S->cv_dev.wait(lk, [&] { return S->ready || (S->stop && S->active.empty()); });
+    S->cv_dev.wait(lk, [&] { return S->ready || (S->stop && S->active.empty() && !S->building); });
     if (!S->ready) break;
     std::unique_ptr<Round> R = std::move(S->ready);
     ++S->running;
@@ -242,7 +264,7 @@ void device_main(gs_server* S, gs_batch* B) {
     }
     lk.unlock();
     // device work without the lock: the planner and the other workers go on
-    int rc = gs::batch_load(B, R->plan);
+    int rc = R->rc ? R->rc : gs::batch_load(B, R->plan);
     const double t1 = S->now();
     gs_batch_stats bs;
     memset(&bs, 0, sizeof bs);
