@@ -218,6 +218,7 @@ extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
 
 static void free_index(gs_index* X) {
   if (!X) return;
+  if (X->work) gs::batch_free(X->work);
   (void)hipFree(X->occ);
   (void)hipFree(X->sa);
   (void)hipFree(X->kmer);

@@ -30,6 +30,7 @@ struct gs_index {
   uint64_t device_bytes = 0;
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0;
   int sa_rounds = 0;
+  gs_batch* work = nullptr;  // gs_seed's batch, kept for the next call (its buffers only grow)
   gs::DevIndex dev() const;
 };
 

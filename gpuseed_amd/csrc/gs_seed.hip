@@ -1454,9 +1454,19 @@ extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* of
                               gs_result* out) {
   if (!out) return GS_EINVAL;
   memset(out, 0, sizeof(*out));
-  gs_batch* B = nullptr;
-  int rc = gs_batch_create(X, bases, offsets, lens, n_reads, opts, &B);
+  if (!X || !opts || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
+  int rc = gs::plan_check(opts);
   if (rc) return rc;
+  if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
+  gs::BatchPlan P;
+  gs::plan_reset(P, *opts);
+  if ((rc = gs::plan_add(P, bases, offsets, lens, 0, n_reads))) return rc;
+  gs::plan_finish(P);
+  // the index keeps this batch for the next call: its device buffers are
+  // reused (no allocation, which would synchronise the device, once they fit)
+  if (!X->work) X->work = gs::batch_new(X);
+  gs_batch* B = X->work;
+  if ((rc = gs::batch_load(B, P))) return rc;
   gs_batch_stats s;
   rc = gs_batch_run(B, -1, &s);
   if (!rc) rc = gs_batch_fetch(B, out);
@@ -1466,7 +1476,6 @@ extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* of
     out->t_post = s.t_count + s.t_write;
     out->t_locate = s.t_locate;
   }
-  gs_batch_destroy(B);
   return rc;
 }
 

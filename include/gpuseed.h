@@ -246,6 +246,8 @@ typedef struct {
   double t_h2d, t_seed, t_post, t_locate, t_d2h;
 } gs_result;
 
+/* The device buffers of a call stay with the index for the next one (they
+ * only grow; gs_index_close frees them). */
 GS_API int gs_seed(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
                    uint64_t n_reads, const gs_opts* opts, gs_result* out);
 GS_API void gs_result_free(gs_result* res);
