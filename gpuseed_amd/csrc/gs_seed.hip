@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "gs_internal.h"
@@ -747,6 +748,8 @@ struct gs_batch {
     }
   };
   Pinned dl_ns, dl_np, dl_soff, dl_poff, dl_seeds, dl_pos;
+  Pinned st_a, st_b;            // staged_d2h's two chunks
+  hipEvent_t cev[2] = {};       // their copies' completion
 };
 
 static void free_graph(gs_batch* B);
@@ -756,7 +759,10 @@ static void destroy_batch(gs_batch* B) {
   if (!B) return;
   free_graph(B);
   free_host_results(B);
-  for (auto* q : {&B->dl_ns, &B->dl_np, &B->dl_soff, &B->dl_poff, &B->dl_seeds, &B->dl_pos}) q->release();
+  for (auto* q : {&B->dl_ns, &B->dl_np, &B->dl_soff, &B->dl_poff, &B->dl_seeds, &B->dl_pos, &B->st_a, &B->st_b})
+    q->release();
+  for (auto& e : B->cev)
+    if (e) (void)hipEventDestroy(e);
   if (B->h_tot) (void)hipHostFree(B->h_tot);
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1346,6 +1352,51 @@ extern "C" GS_API int gs_batch_replay(gs_batch* B, gs_batch_stats* stats) {
   return GS_OK;
 }
 
+// Device-to-host copy of `bytes` into pageable memory through two pinned
+// chunks: the DMA of chunk c+1 runs while chunk c is copied out, on several
+// threads when it is large.  Pageable destinations otherwise go through the
+// runtime's own staging at a fraction of the link's rate.
+static int staged_d2h(gs_batch* B, void* dst, const void* src, size_t bytes) {
+  constexpr size_t kChunk = 64u << 20;
+  hipStream_t st = B->stream;
+  if (bytes <= (4u << 20))  // small: one plain copy
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                   hipStreamSynchronize(st) == hipSuccess
+               ? GS_OK
+               : GS_EHIP;
+  const size_t ch = std::min(bytes, kChunk);
+  if (B->st_a.ensure(ch) || B->st_b.ensure(ch)) return GS_ENOMEM;
+  for (auto& e : B->cev)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return GS_EHIP;
+  char* stage[2] = {static_cast<char*>(B->st_a.p), static_cast<char*>(B->st_b.p)};
+  const size_t nch = (bytes + ch - 1) / ch;
+  auto issue = [&](size_t c) -> bool {
+    const size_t off = c * ch, len = std::min(ch, bytes - off);
+    return hipMemcpyAsync(stage[c & 1], static_cast<const char*>(src) + off, len, hipMemcpyDeviceToHost, st) ==
+               hipSuccess &&
+           hipEventRecord(B->cev[c & 1], st) == hipSuccess;
+  };
+  if (!issue(0)) return GS_EHIP;
+  for (size_t c = 0; c < nch; ++c) {
+    // the buffer of chunk c+1 held chunk c-1, already copied out
+    if (c + 1 < nch && !issue(c + 1)) return GS_EHIP;
+    if (hipEventSynchronize(B->cev[c & 1]) != hipSuccess) return GS_EHIP;
+    const size_t off = c * ch, len = std::min(ch, bytes - off);
+    char* d = static_cast<char*>(dst) + off;
+    const char* sp = stage[c & 1];
+    constexpr size_t kThreads = 4;
+    const size_t part = (len + kThreads - 1) / kThreads;
+    std::thread th[kThreads - 1];
+    for (size_t t = 1; t < kThreads; ++t) {
+      const size_t a = std::min(len, t * part), z = std::min(len, a + part);
+      th[t - 1] = std::thread([=] { memcpy(d + a, sp + a, z - a); });
+    }
+    memcpy(d, sp, std::min(len, part));
+    for (auto& t : th) t.join();
+  }
+  return GS_OK;
+}
+
 extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   if (!B || !out || !B->ran) return GS_EINVAL;
   memset(out, 0, sizeof(*out));
@@ -1386,11 +1437,11 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
              hipMemcpyAsync(out->read_npos, B->np.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
       hipMemcpyAsync(out->read_seed_off, B->soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipMemcpyAsync(out->read_pos_off, B->poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      (B->n_seeds && hipMemcpyAsync(out->seeds, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds,
-                                    hipMemcpyDeviceToHost, st) != hipSuccess) ||
-      (B->n_pos && hipMemcpyAsync(out->pos, B->rows.p, 4 * B->n_pos, hipMemcpyDeviceToHost, st) != hipSuccess) ||
-      hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      hipStreamSynchronize(st) != hipSuccess)
     rc = GS_EHIP;
+  if (!rc && B->n_seeds) rc = staged_d2h(B, out->seeds, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds);
+  if (!rc && B->n_pos) rc = staged_d2h(B, out->pos, B->rows.p, 4 * B->n_pos);
+  if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) rc = GS_EHIP;
   if (rc) {
     gs_result_free(out);
     return rc;

@@ -132,3 +132,38 @@ def test_autotune_picks_valid_options(g, case):
         tuned = g.seed(X, arrays=case["arrays"], opts=opts)
         plain = g.seed(X, arrays=case["arrays"], k=19, smem=True, max_occ=100, kernel=0)
         assert same(tuned, plain)
+
+
+def test_large_outputs_download_in_chunks(g, tmp_path):
+    """Positions of more than 64 MB go through the staged device-to-host copy
+    in several pinned chunks (gs_seed, gs_batch_fetch); they equal what a
+    graph replay downloads into its own pinned buffers, and the per-read
+    counts equal the oracle's."""
+    rng = np.random.default_rng(5)
+    unit = rng.integers(0, 4, 400, dtype=np.uint8)
+    ref = np.concatenate([rng.integers(0, 4, 20000, dtype=np.uint8)] + [unit] * 3000 +
+                         [rng.integers(0, 4, 20000, dtype=np.uint8)])
+    prefix = str(tmp_path / "rep")
+    g.build_index_host(ref, 7, prefix)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    starts = rng.integers(20000, 20000 + 400 * 2990, 20000)
+    reads = [acgt[ref[s:s + 150]].tobytes() for s in starts]
+    arrays = g.pack_reads(reads)
+    say("large outputs: %d reads from a 3000-copy repeat" % len(reads))
+    with g.Index(prefix) as X:
+        staged = g.seed(X, arrays=arrays, k=19, smem=True, kernel=1)
+        assert 4 * len(staged.pos) > 2 * (64 << 20)  # more than two 64 MB chunks of positions
+        b = g.Batch(X, arrays, k=19, smem=True, kernel=1)
+        b.run()
+        b.capture()
+        b.replay()
+        replayed = b.fetch()
+        b.close()
+    assert np.array_equal(staged.pos, replayed.pos)
+    assert np.array_equal(staged.seeds, replayed.seeds)
+    assert np.array_equal(staged.read_pos_off, replayed.read_pos_off)
+    import oracle_bind as ob
+    ox = ob.Index(prefix)
+    _, _, so, po, _ = ob.seed_arrays(ox, *arrays, 19, True, threads=8)
+    ox.close()
+    assert np.array_equal(po, staged.read_pos_off) and np.array_equal(so, staged.read_seed_off)
