@@ -1386,12 +1386,18 @@ static int staged_d2h(gs_batch* B, void* dst, const void* src, size_t bytes) {
     const char* sp = stage[c & 1];
     constexpr size_t kThreads = 4;
     const size_t part = (len + kThreads - 1) / kThreads;
-    std::thread th[kThreads - 1];
-    for (size_t t = 1; t < kThreads; ++t) {
-      const size_t a = std::min(len, t * part), z = std::min(len, a + part);
-      th[t - 1] = std::thread([=] { memcpy(d + a, sp + a, z - a); });
+    std::vector<std::thread> th;
+    size_t done = part;  // bytes the helper threads do not cover
+    try {
+      for (size_t t = 1; t < kThreads; ++t) {
+        const size_t a = std::min(len, t * part), z = std::min(len, a + part);
+        th.emplace_back([=] { memcpy(d + a, sp + a, z - a); });
+        done = std::min(len, (t + 1) * part);
+      }
+    } catch (...) {  // no thread to spare: this one copies the rest
     }
     memcpy(d, sp, std::min(len, part));
+    if (done < len) memcpy(d + done, sp + done, len - done);
     for (auto& t : th) t.join();
   }
   return GS_OK;

@@ -292,8 +292,19 @@ void device_main(gs_server* S, gs_batch* B) {
       const uint64_t bytes = res.n_seeds * sizeof(gs_seed_rec) + 4 * res.n_pos;
       const size_t nt = std::min<size_t>(R->chunks.size(), bytes > (32u << 20) ? 4 : 1);
       std::vector<std::thread> helpers;
-      for (size_t t = 1; t < nt; ++t) helpers.emplace_back(hand_off, t, nt);
-      hand_off(0, nt);
+      size_t step = nt;
+      try {
+        for (size_t t = 1; t < nt; ++t) helpers.emplace_back(hand_off, t, nt);
+      } catch (...) {  // fewer helpers than planned: the chunks they would take are done here
+        step = 1;
+      }
+      if (step == 1 && !helpers.empty()) {
+        for (auto& h : helpers) h.join();
+        helpers.clear();
+        // the helpers that started took chunks t, t+nt, ...; redo every chunk
+        // (assign replaces a piece's contents, so copying one twice is harmless)
+      }
+      hand_off(0, step);
       for (auto& h : helpers) h.join();
     }
     const double t_end = S->now();
