@@ -283,14 +283,15 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=60, help="requests of the peak burst")
     ap.add_argument("--serve-plain-requests", type=int, default=25,
                     help="requests per load level for the plain per-request server (0: none)")
-    ap.add_argument("--serve-workers", type=int, default=0, help="device workers of the server (0: its default)")
+    ap.add_argument("--serve-workers", type=int, default=0, help="device workers of the server (0: its default, 3)")
     ap.add_argument("--serve-growth", type=int, default=0,
-                    help="rounds grow up to this many times 16 Mbases under backlog (0: fixed rounds)")
+                    help="rounds grow up to this many times 16 Mbases under backlog (0: the server's "
+                         "default, 4; 1: fixed rounds)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this project has exactly one MI355X, "
-              "and reads shard across GPUs without exchange (see DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU. Reads shard across GPUs without exchange (each GPU holds a "
+              "replica of the index and seeds its own share of the reads), so N GPUs give N times the "
+              "one-GPU rate; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
