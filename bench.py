@@ -53,7 +53,8 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True):
     """Memory operations per read of the device algorithm, counted by the
     oracle on the first `sample` reads: 13-mer table lookups, extended-table
     lookups (8 bytes each) and occ entries (16 bytes each), with distance-1
-    pruning and the SMEM read-start stop; and the same count for the
+    pruning, the SMEM read-start stop and tasks whose clean extended-table
+    window is absent ending at once; and the same count for the
     reference's procedure (13-mer table, distance-1 pruning only)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
@@ -63,7 +64,7 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True):
     sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
     c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
-                       ext_len=ext_len if ext_len > 13 else 0)[4]
+                       ext_len=ext_len if ext_len > 13 else 0, miss_skip=1)[4]
     ox.close()
     table = (c["w_tasks"] + c["w_ext"]) / n
     occ = c["w_blocks96"] / n

@@ -436,8 +436,15 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       // start; the task logic below is inlined once for every lane
       int res = 0;
       if (nl > nu) {
-        if (op == 1 && tlen > gs::kPrecalc)
-          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
+        // An absent clean tlen-base window: the task's candidate could only be
+        // shorter than tlen.  With k >= tlen it can emit nothing, and what
+        // else it would leave cannot change the output (left ends of finished
+        // tasks do not grow as the end falls, so a later candidate of length
+        // >= k lies left of this task's left end and of the previous one's;
+        // size records only prune), so it ends here.  Otherwise the 13-mer
+        // lookup and steps decide.
+        if (op == 1 && tlen > gs::kPrecalc && k < tlen)
+          ext_miss = true;
         else
           res = op == 2 ? 2 : 1;
       } else {

@@ -225,8 +225,10 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
     if (op == 1) {
       const uint32_t il = (tidx & 1) ? d1.z : d1.x, iu = (tidx & 1) ? d1.w : d1.y;
       if (il > iu) {
-        if (tlen > gs::kPrecalc)
-          ext_miss = true;  // absent longer pattern: the 13-mer lookup and steps decide
+        // absent clean tlen-base window: with k >= tlen the task can emit
+        // nothing and ends (as in seed_persistent); else the 13-mer path decides
+        if (tlen > gs::kPrecalc && P.k < tlen)
+          ext_miss = true;
         else if (!next_task())
           done = true;
       } else {

@@ -204,6 +204,29 @@ static uint32_t locate_count(const or_index* X, uint32_t row, uint64_t* lf_steps
 
 uint32_t or_locate(const or_index* X, uint32_t row) { return locate_count(X, row, NULL); }
 
+/* Textbook LF (row of the suffix one position to the left) and the sampled
+ * suffix-array walk built on it.  The reference's walk differs only at row
+ * primary+1 (or_lf above); this one is the true SA, which the device's direct
+ * extension reads (not an output of the reference). */
+static uint32_t lf_true(const or_index* X, uint32_t row) {
+  if (row == X->primary) return 0; /* the suffix at text position 0 */
+  uint32_t k = row - (row > X->primary ? 1u : 0u);
+  const uint32_t* blk = X->bwt + (uint64_t)(k >> 6) * 8;
+  uint32_t c = (blk[4 + ((k & 63) >> 4)] >> (((~k) & 15) << 1)) & 3;
+  return X->L2[c] + occ_idx(X, k, c);
+}
+
+uint32_t or_sa_true(const or_index* X, uint32_t row) {
+  if (row > X->seq_len) return 0xFFFFFFFFu;
+  if (row == 0) return X->seq_len;
+  uint32_t itr = 0;
+  while (row % X->sa_intv) {
+    ++itr;
+    row = lf_true(X, row);
+  }
+  return row == 0 ? X->seq_len + itr : X->sa[row / X->sa_intv] + itr;
+}
+
 /* ---- 13-mer table (2648-2675) ------------------------------------------- */
 
 static uint64_t kmer13(const or_index* X, uint32_t p) {
@@ -304,7 +327,7 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct;
 } work_t;
 
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
@@ -327,11 +350,18 @@ static void count_blocks(const or_index* X, uint32_t l, uint32_t u, work_t* w) {
  * output is the same whatever prune, smem_stop and ext_len say; they only
  * change which steps run and are counted (see or_opts). */
 static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t L, uint32_t k,
-                                  int prune, int smem_stop, uint32_t ext_len, uint32_t sat, cand_t* out,
-                                  uint32_t* A, work_t* w) {
+                                  int prune, int smem_stop, uint32_t ext_len, uint32_t sat,
+                                  uint32_t direct_min, int miss_skip, int smem, cand_t* out, uint32_t* A,
+                                  work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
+  uint32_t last_b = 0xFFFFFFFFu; /* left end of the last unpruned task (the device's in-lane SMEM filter) */
+  /* the last direct run's region: read positions [reg_lo, reg_hi) aligned to
+   * the text from reg_t (text position of read[reg_lo]); the 128-position
+   * chunk of the uniqueness bitmap read last */
+  uint32_t reg_lo = 0, reg_hi = 0, reg_t = 0;
+  uint64_t uchunk = ~0ull;
   for (uint32_t e = L; e >= k; --e) {
     uint32_t p = 0;
     int bad = 0;
@@ -349,6 +379,21 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     if (ext_len > PRECALC && e >= ext_len) {
       int clean = 1;
       for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
+      const uint32_t wb = e - ext_len;
+      if (clean && direct_min && prune && wb >= reg_lo && wb < reg_hi && A[wb] == 1) {
+        /* the window is the text at reg_t + (wb - reg_lo): its uniqueness bit
+         * decides the task without a lookup when the pattern occurs once (then
+         * its size equals the record and the task is pruned) */
+        const uint64_t t = (uint64_t)reg_t + (wb - reg_lo);
+        if ((t >> 7) != uchunk) {
+          uchunk = t >> 7;
+          ++w->direct;
+        }
+        uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
+        uint32_t l3 = (uint32_t)iv, u3 = (uint32_t)(iv >> 32);
+        for (uint32_t j = PRECALC; j < ext_len && l3 <= u3; ++j) step(X, &l3, &u3, v[e - 1 - j]);
+        if (l3 <= u3 && l3 == u3) continue; /* unique: pruned (A[wb] == 1) */
+      }
       if (clean) { /* one lookup answers the first ext_len bases when they occur */
         ++w->ext;
         uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
@@ -358,6 +403,12 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         if (l <= u) {
           have = 1;
           b = e - ext_len;
+        } else if (miss_skip && k >= ext_len) {
+          /* the clean window does not occur, so no extension of the task
+           * reaches ext_len bases, let alone k: it can emit nothing, and what
+           * else it leaves (its left end for the in-lane SMEM filter, its
+           * size records) cannot change any output (see or_opts.miss_skip) */
+          continue;
         }
       }
     }
@@ -376,18 +427,52 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       if (A[b] == s && s != sat) pruned = 1;
       A[b] = s;
     }
+    /* direct-extension model: in a run, the text position of read[b] and
+     * the last 64-base text chunk read */
+    int run = 0, sawn = 0;
+    uint32_t tq = 0, ones = 0, b0 = 0;
+    uint64_t chunk = ~0ull;
     for (int64_t x = (int64_t)b - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
       uint32_t c = v[x], l2 = l, u2 = u;
-      if (c <= 3) count_blocks(X, l, u, w);
+      if (direct_min && prune && !run && c <= 3 && l == u && ones >= direct_min && A[x] != 1) {
+        run = 1; /* suffix-array read: the text position of the single row */
+        ++w->direct;
+        tq = or_sa_true(X, l);
+        chunk = ~0ull;
+        b0 = b;
+        /* without an N in the pattern, all of [b, e) is aligned to the text */
+        reg_lo = b;
+        reg_hi = sawn ? b : e;
+        reg_t = tq;
+      }
+      if (c > 3) sawn = 1;
+      if (run && c > 3) { /* an N ends the run: the row of read[b] is read back */
+        run = 0;
+        ++w->direct;
+      }
+      if (c <= 3) {
+        if (!run) {
+          count_blocks(X, l, u, w);
+        } else if (tq > 0 && (uint64_t)((tq - 1) >> 6) != chunk) {
+          chunk = (tq - 1) >> 6; /* text chunk holding text[tq - 1] */
+          ++w->direct;
+        }
+      }
       step(X, &l2, &u2, c);
       ++w->steps;
       if (l2 > u2) { /* 2246 */
         reached = 0;
         break;
       }
+      if (c <= 3) ones = l2 == u2 ? ones + 1 : 0;
       l = l2;
       u = u2;
       b = (uint32_t)x; /* 2252-2255 */
+      if (run) {
+        --tq;
+        reg_lo = b;
+        reg_t = tq;
+      }
       if (prune) {
         uint32_t s = u - l + 1;
         if (sat && s >= sat) s = sat;
@@ -396,6 +481,9 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       }
     }
     if (pruned) continue;
+    const int emit = e - b >= k && (!smem || b < last_b);
+    if (run && emit) ++w->direct; /* the row where the run ended */
+    last_b = b;
     if (e - b >= k) out[n++] = (cand_t){b, e, l, u}; /* 2259-2264 */
     if (smem_stop && reached) break; /* later tasks end at 0 too and fail the SMEM filter */
   }
@@ -455,12 +543,15 @@ static void phase_candidates(void* vctx, uint64_t i, int tid) {
   const int stop = C->o->smem && C->o->smem_stop;
   const uint32_t ext = (uint32_t)C->o->ext_len;
   const uint32_t sat = C->o->prune_sat;
-  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, tmp + ntask, A, &C->work[tid]);
+  const uint32_t dmin = C->o->direct_min;
+  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip, C->o->smem, tmp + ntask, A,
+                                  &C->work[tid]);
   /* forward: decreasing end -> slot order (increasing end) */
   for (uint32_t q = 0; q < nf; ++q) Ls->F[q] = tmp[ntask + nf - 1 - q];
   Ls->nF = nf;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_rev(s[L - 1 - q]);
-  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, Ls->R, A, &C->work[tid]);
+  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip, C->o->smem, Ls->R, A,
+                                  &C->work[tid]);
   for (uint32_t q = 0; q < nr; ++q) { /* virtual [b,e) -> reverse (L-e, L-b) with strand bits */
     cand_t c = Ls->R[q];
     Ls->R[q] = (cand_t){(L - c.e) | 0x80000000u, (L - c.b) | 0x80000000u, c.l, c.u};
@@ -749,6 +840,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_blocks96 += work[t].blocks96;
     res->w_lf += work[t].lf;
     res->w_sa += work[t].sa;
+    res->w_direct += work[t].direct;
     free(scratch[t]);
   }
   if (!o->prune) res->w_steps_unpruned = res->w_steps;

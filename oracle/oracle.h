@@ -47,6 +47,11 @@ uint32_t or_occ(const or_index* idx, uint32_t row, uint32_t c);
 uint64_t or_step(const or_index* idx, uint32_t l, uint32_t u, uint32_t c);
 uint32_t or_lf(const or_index* idx, uint32_t row);
 uint32_t or_locate(const or_index* idx, uint32_t row);
+/* The true suffix-array value of a row (seq_len for row 0, the empty
+ * suffix): the sampled-suffix-array walk with the textbook LF, i.e. without
+ * the primary+1 behaviour of or_lf.  Not part of the reference's output;
+ * it checks the device's full suffix array, inverse and text. */
+uint32_t or_sa_true(const or_index* idx, uint32_t row);
 
 /* pack_4bit_fow / pack_4bit_rev code of one input byte (seed_gen.cu:2536-2578). */
 uint32_t or_code_fwd(uint8_t ascii);
@@ -74,6 +79,24 @@ typedef struct {
    * never prunes, like the device's 8-bit (255) and 16-bit (65535) records;
    * 0: exact records. */
   uint32_t prune_sat;
+  /* direct_min: work model of the device's direct extension.  A task whose
+   * interval has a single row, with at least direct_min positions left to
+   * its left and no size record of 1 at the next one, reads the reference
+   * text instead of stepping: one suffix-array read (the text position), one
+   * 16-byte read per 64 text bases compared, and one inverse-suffix-array read
+   * for the row where the run ends, skipped when the task emits nothing
+   * there or is pruned inside the run.  Counted in w_direct; the steps it
+   * replaces are not counted in w_blocks64/96.  0: off. */
+  uint32_t direct_min;
+  /* miss_skip: with ext_len <= k, a task whose clean ext_len-base window does
+   * not occur ends at once instead of falling back to the 13-mer table.  Its
+   * candidate could only be shorter than ext_len, hence than k, so it emits
+   * nothing.  Its left end cannot change an in-lane SMEM filter decision:
+   * left ends of finished tasks do not increase as the end decreases, so a
+   * later candidate of length >= k lies left of both this task's left end
+   * and the previous one's.  Its size records only prune (an early exit
+   * that never changes the output).  Output unchanged; counters only. */
+  int miss_skip;
 } or_opts;
 
 typedef struct {
@@ -103,6 +126,7 @@ typedef struct {
   uint64_t w_steps_unpruned; /* steps with no pruning at all                 */
   uint64_t n_short;       /* reads shorter than k (no tasks)                  */
   uint64_t w_ext;         /* extended-table lookups (opts.ext_len)            */
+  uint64_t w_direct;      /* direct-extension reads (opts.direct_min)         */
 } or_result;
 
 /* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line

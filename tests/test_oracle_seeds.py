@@ -166,7 +166,9 @@ def test_mem_lookahead_crosses_reads_within_a_batch(built, tmp_path):
 def test_work_model_options_do_not_change_output(built, tmp_path):
     """The device kernels' shortcuts (a strand stops once a task reaches the
     read start in SMEM mode; an extended interval table answers the first
-    14..16 bases) change only the counted work, never the seeds."""
+    14..16 bases; with k at least the table's length, a task whose clean
+    window is absent ends at once) change only the counted work, never the
+    seeds."""
     import gpuseed_amd
     rp, qp = gpuseed_amd.preset(1)
     ref = gpuseed_amd.synth_reference(rp)
@@ -175,17 +177,25 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
     prefix = os.path.join(str(tmp_path), "c1")
     gpuseed_amd.build_index_host(ref, 7, prefix)
     ox = ob.Index(prefix)
-    for smem, k in ((True, 19), (False, 20)):
+    for smem, k in ((True, 19), (False, 20), (True, 16), (False, 16), (True, 15)):
         base = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4)
-        for stop, ext in ((1, 0), (0, 16), (1, 14), (1, 16)):
-            got = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=stop, ext_len=ext)
+        noskip = {}
+        for stop, ext, skip in ((1, 0, 0), (0, 16, 0), (1, 14, 0), (1, 16, 0), (0, 16, 1), (1, 14, 1), (1, 16, 1)):
+            got = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=stop, ext_len=ext,
+                                 miss_skip=skip)
             for a, b in zip(base[:4], got[:4]):
-                assert (a == b).all(), (smem, stop, ext)
+                assert (a == b).all(), (smem, k, stop, ext, skip)
             cb, cg = base[4], got[4]
             if stop and smem:
                 assert cg["w_blocks96"] < cb["w_blocks96"]
             if ext:
                 assert cg["w_ext"] > 0 and cg["w_steps"] < cb["w_steps"]
+            if not skip:
+                noskip[(stop, ext)] = cg
+            elif k >= ext:  # absent windows end their tasks: fewer 13-mer lookups
+                assert cg["w_tasks"] < noskip[(stop, ext)]["w_tasks"], (smem, k, stop, ext)
+            else:  # k below the window length: the option must not apply
+                assert cg == noskip[(stop, ext)], (smem, k, stop, ext)
     # saturating size records only lose pruning: same seeds, at least as
     # much work, and more work the lower the saturation (repeats of the
     # 1 Mbp reference reach interval sizes of 10, so 4 and 8 lose prunes)
