@@ -317,7 +317,8 @@ class Index:
         _check("gs_index_set_kmer_len", lib().gs_index_set_kmer_len(self.h, length))
 
     def read_kext(self, first, count):
-        out = np.empty((count, 2), dtype=np.uint32)
+        """Extended-table entries: rows of (l, u, presence mask low, high)."""
+        out = np.empty((count, 4), dtype=np.uint32)
         _check("gs_index_read_kext", lib().gs_index_read_kext(self.h, first, count, _ptr(out, C.c_uint32)))
         return out
 

@@ -36,7 +36,10 @@ __device__ __forceinline__ uint32_t rank_in_entry(uint4 e, uint32_t bit) {
 struct DevIndex {
   const uint4* occ;      // [n_blocks * 4]
   const uint2* kmer;     // 4^13 (l, u) intervals
-  const uint2* kext;     // optional 4^ext_len intervals (ext_len 14..16), else null
+  // optional 4^ext_len entries (ext_len 14..16), else null: {l, u} of the
+  // pattern and a 64-bit presence mask (bit c1 + 4*c2 + 16*c3 set iff the
+  // pattern extended to the left by c1, then c2, then c3 occurs)
+  const uint4* kext;
   const uint32_t* sa;    // sampled suffix array, sa[0] = 0xFFFFFFFF
   uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)

@@ -127,6 +127,35 @@ __global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uin
   next[t] = make_uint2(l, u);
 }
 
+// Presence bits of one extended-table entry: which of the 64 three-base
+// left extensions c3.c2.c1.P of pattern P occur (bit c1 + 4*c2 + 16*c3),
+// found by a depth-first search that stops at empty intervals.
+__global__ void kmer_presence(const DevIndex X, const uint2* __restrict__ level, uint4* __restrict__ out,
+                              uint64_t first, uint64_t count) {
+  const uint64_t t = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const uint2 iv = level[t];
+  uint64_t mask = 0;
+  if (iv.x <= iv.y) {
+    for (uint32_t c1 = 0; c1 < 4; ++c1) {
+      uint32_t l1 = iv.x, u1 = iv.y;
+      gs::extend(X, c1, l1, u1);
+      if (l1 > u1) continue;
+      for (uint32_t c2 = 0; c2 < 4; ++c2) {
+        uint32_t l2 = l1, u2 = u1;
+        gs::extend(X, c2, l2, u2);
+        if (l2 > u2) continue;
+        for (uint32_t c3 = 0; c3 < 4; ++c3) {
+          uint32_t l3 = l2, u3 = u2;
+          gs::extend(X, c3, l3, u3);
+          if (l3 <= u3) mask |= 1ull << (c1 + 4 * c2 + 16 * c3);
+        }
+      }
+    }
+  }
+  out[t] = make_uint4(iv.x, iv.y, (uint32_t)mask, (uint32_t)(mask >> 32));
+}
+
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // One table level in launches of at most 2^30 work-items (a dispatch's
@@ -168,23 +197,26 @@ int gs_build_kmer_table(gs_index* X) {
 
 // The extended table: the intervals of all 4^len patterns (len 14..16),
 // entry p = sum of code_j * 4^j in backward-search order, empty patterns
-// stored as l > u.  Built from the 13-mer table one level at a time with the
-// same kernel; seeding then answers the first len bases of a task with one
-// lookup and goes back to the 13-mer table only when that pattern is absent
-// or its window holds a code above 3.
+// stored as l > u, each with the presence bits of its 64 three-base left
+// extensions.  The intervals are built from the 13-mer table one level at a
+// time with the same kernel, then kmer_presence fills the 16-byte entries.
+// Seeding answers the first len bases of a task with one lookup, ends the
+// task at once when that pattern, or (with k >= len + 3) its clean
+// (len + 3)-base extension, is absent, and goes back to the 13-mer table only
+// when the pattern is absent with k < len or its window holds a code above 3.
 extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
   if (!X || len < gs::kPrecalc || len > 16) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   if (X->kext) {
     GS_HIP(hipStreamSynchronize(X->stream));
     (void)hipFree(X->kext);
-    X->device_bytes -= (1ull << (2 * X->ext_len)) * sizeof(uint2);
+    X->device_bytes -= (1ull << (2 * X->ext_len)) * sizeof(uint4);
     X->kext = nullptr;
     X->ext_len = 0;
   }
   if (len == gs::kPrecalc) return GS_OK;
   const double t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  uint2* levels[4] = {nullptr, nullptr, nullptr, nullptr};  // levels 14..len
+  uint2* levels[3] = {nullptr, nullptr, nullptr};  // levels 14..len
   int rc = GS_OK;
   const DevIndex D = X->dev();
   const uint2* prev = X->kmer;
@@ -200,16 +232,29 @@ extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
     prev = next;
   }
   if (!rc && hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
-  for (int lev = 14; lev <= 16; ++lev) {
-    uint2* a = levels[lev - 14];
-    if (!a) continue;
-    if (lev == len && !rc) {
-      X->kext = a;
-      X->ext_len = (uint32_t)len;
-      X->device_bytes += (1ull << (2 * len)) * sizeof(uint2);
-    } else {
-      (void)hipFree(a);
+  for (int lev = 14; lev < len; ++lev) {  // only the last level is read from here on
+    (void)hipFree(levels[lev - 14]);
+    levels[lev - 14] = nullptr;
+  }
+  uint4* table = nullptr;
+  const uint64_t cnt = 1ull << (2 * len);
+  if (!rc && hipMalloc(&table, cnt * sizeof(uint4)) != hipSuccess) rc = GS_ENOMEM;
+  if (!rc) {
+    const uint64_t step = 1ull << 30;  // a dispatch's work-item count is a 32-bit quantity
+    for (uint64_t first = 0; first < cnt; first += step) {
+      const uint64_t n = cnt - first < step ? cnt - first : step;
+      hipLaunchKernelGGL(kmer_presence, dim3(grid_for(n, 256)), dim3(256), 0, X->stream, D,
+                         levels[len - 14], table, first, cnt);
     }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
+  }
+  (void)hipFree(levels[len - 14]);
+  if (!rc) {
+    X->kext = table;
+    X->ext_len = (uint32_t)len;
+    X->device_bytes += cnt * sizeof(uint4);
+  } else {
+    (void)hipFree(table);
   }
   const double t1 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
   X->t_ext = t1 - t0;
@@ -360,10 +405,10 @@ extern "C" GS_API int gs_index_read_occ(const gs_index* X, uint64_t first, uint6
   return GS_OK;
 }
 
-extern "C" GS_API int gs_index_read_kext(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out2) {
-  if (!X || !out2 || !X->kext || first + count > (1ull << (2 * X->ext_len))) return GS_EINVAL;
+extern "C" GS_API int gs_index_read_kext(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out4) {
+  if (!X || !out4 || !X->kext || first + count > (1ull << (2 * X->ext_len))) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
-  GS_HIP(hipMemcpy(out2, X->kext + first, count * sizeof(uint2), hipMemcpyDeviceToHost));
+  GS_HIP(hipMemcpy(out4, X->kext + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
   return GS_OK;
 }
 

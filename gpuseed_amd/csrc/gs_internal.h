@@ -25,7 +25,7 @@ struct gs_index {
   uint4* occ = nullptr;
   uint32_t* sa = nullptr;
   uint2* kmer = nullptr;
-  uint2* kext = nullptr;  // extended interval table (gs_index_set_kmer_len)
+  uint4* kext = nullptr;  // extended interval table with presence bits (gs_index_set_kmer_len)
   uint32_t ext_len = 0;
   uint64_t device_bytes = 0;
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0;

@@ -379,8 +379,9 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
     uint32_t tlen = gs::kPrecalc, tidx = p;  // table lookup: pattern length and entry
+    uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
     if (op == 1) {
-      const uint2* tbl = P.X.kmer;
+      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);  // two 8-byte entries per load
       if (P.X.ext_len && !ext_miss && e >= (int32_t)P.X.ext_len) {
         uint32_t q = p, ok = 1;
         for (uint32_t j = gs::kPrecalc; j < P.X.ext_len; ++j) {
@@ -391,10 +392,14 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         if (ok) {
           tlen = P.X.ext_len;
           tidx = q;
-          tbl = P.X.kext;
+          a1 = P.X.kext + q;
+          if (k >= tlen + 3 && e >= (int32_t)tlen + 3) {
+            const uint32_t c1 = code(e - (int32_t)tlen - 1), c2 = code(e - (int32_t)tlen - 2),
+                           c3 = code(e - (int32_t)tlen - 3);
+            if ((c1 | c2 | c3) <= 3) pbit = c1 + 4 * c2 + 16 * c3;
+          }
         }
       }
-      a1 = reinterpret_cast<const uint4*>(tbl) + (tidx >> 1);
       ++n_table;
       ++work;
     } else if (op == 2) {
@@ -418,9 +423,16 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     // both then share one copy of the task logic, so a wave whose lanes are
     // in different phases runs it once
     uint32_t nl, nu, nb;
+    bool absent = false;  // the clean (tlen + 3)-base window does not occur
     if (op == 1) {
-      nl = (tidx & 1u) ? d1.z : d1.x;
-      nu = (tidx & 1u) ? d1.w : d1.y;
+      if (tlen > gs::kPrecalc) {
+        nl = d1.x;
+        nu = d1.y;
+        absent = pbit < 64 && !(((pbit < 32 ? d1.z : d1.w) >> (pbit & 31)) & 1u);
+      } else {
+        nl = (tidx & 1u) ? d1.z : d1.x;
+        nu = (tidx & 1u) ? d1.w : d1.y;
+      }
       nb = (uint32_t)e - tlen;
     } else {
       const uint32_t o2 = gs::rank_in_entry(d1, i2 - b2 * gs::kBlockBases);
@@ -447,6 +459,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
           ext_miss = true;
         else
           res = op == 2 ? 2 : 1;
+      } else if (absent) {
+        res = 1;  // the same holds for an absent clean (tlen + 3)-base window with k >= tlen + 3
       } else {
         l = nl;
         u = nu;

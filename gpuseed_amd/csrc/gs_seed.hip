@@ -191,8 +191,9 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
     uint32_t tlen = gs::kPrecalc, tidx = p;  // table lookup: pattern length and entry
+    uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
     if (op == 1) {
-      const uint2* tbl = P.X.kmer;
+      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);  // two 8-byte entries per load
       if (P.X.ext_len && !ext_miss && e >= (int32_t)P.X.ext_len) {
         uint32_t q = p, ok = 1;
         for (uint32_t j = gs::kPrecalc; j < P.X.ext_len; ++j) {
@@ -203,10 +204,14 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
         if (ok) {
           tlen = P.X.ext_len;
           tidx = q;
-          tbl = P.X.kext;
+          a1 = P.X.kext + q;
+          if (k >= tlen + 3 && e >= (int32_t)tlen + 3) {
+            const uint32_t c1 = V(e - (int32_t)tlen - 1), c2 = V(e - (int32_t)tlen - 2),
+                           c3 = V(e - (int32_t)tlen - 3);
+            if ((c1 | c2 | c3) <= 3) pbit = c1 + 4 * c2 + 16 * c3;
+          }
         }
       }
-      a1 = reinterpret_cast<const uint4*>(tbl) + (tidx >> 1);
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
       have1 = r1 != 0xFFFFFFFFu;
@@ -223,14 +228,18 @@ __global__ void __launch_bounds__(kSeedBlock) seed_streams(SeedParams P) {
     if (a2) d2 = *a2;
 
     if (op == 1) {
-      const uint32_t il = (tidx & 1) ? d1.z : d1.x, iu = (tidx & 1) ? d1.w : d1.y;
+      const bool ext = tlen > gs::kPrecalc;
+      const uint32_t il = ext ? d1.x : ((tidx & 1) ? d1.z : d1.x), iu = ext ? d1.y : ((tidx & 1) ? d1.w : d1.y);
+      const bool absent = pbit < 64 && !(((pbit < 32 ? d1.z : d1.w) >> (pbit & 31)) & 1u);
       if (il > iu) {
         // absent clean tlen-base window: with k >= tlen the task can emit
         // nothing and ends (as in seed_persistent); else the 13-mer path decides
-        if (tlen > gs::kPrecalc && P.k < tlen)
+        if (ext && P.k < tlen)
           ext_miss = true;
         else if (!next_task())
           done = true;
+      } else if (absent) {  // absent clean (tlen + 3)-base window, k >= tlen + 3: ends likewise
+        if (!next_task()) done = true;
       } else {
         l = il;
         u = iu;

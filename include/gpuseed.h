@@ -177,15 +177,21 @@ GS_API int gs_index_read_occ(const gs_index* idx, uint64_t first, uint64_t count
 GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
 
 /* Adds (len 14..16) or drops (len 13) an extended interval table: the SA
- * interval of every pattern of len bases, 8 * 4^len bytes of HBM (2.1 GB,
- * 8.6 GB, 34.4 GB).  Seeding then answers a task's first len bases with one
- * lookup where the reference takes the 13-mer table plus len - 13 backward
- * steps (seed_gen.cu:2132-2174); it falls back to those steps when the
- * pattern does not occur or its window holds a code above 3, so the output
- * is unchanged.  No counterpart in the reference. */
+ * interval of every pattern of len bases and which of its 64 three-base left
+ * extensions occur, 16 * 4^len bytes of HBM (4.3 GB, 17.2 GB, 68.7 GB).
+ * Seeding then answers a task's first len bases with one lookup where the
+ * reference takes the 13-mer table plus len - 13 backward steps
+ * (seed_gen.cu:2132-2174).  A task whose clean window is absent (with
+ * k >= len), or whose clean (len + 3)-base window is absent (with
+ * k >= len + 3), cannot reach k bases and ends at once; otherwise it falls
+ * back to those steps when the pattern does not occur or its window holds a
+ * code above 3.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
-/* Entries [first, first+count) of the extended table (tests). */
-GS_API int gs_index_read_kext(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out2);
+/* Entries [first, first+count) of the extended table (tests): four words
+ * per entry, l, u and the low and high words of the presence mask (bit
+ * c1 + 4*c2 + 16*c3 set iff the pattern extended to the left by c1, then c2,
+ * then c3 occurs). */
+GS_API int gs_index_read_kext(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
 
 /* Builds the index of codes[0..n) (values 0..3) on the device: suffix array
  * by prefix doubling, then the same device structures gs_index_open makes.

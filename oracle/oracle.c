@@ -400,6 +400,17 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         l = (uint32_t)iv;
         u = (uint32_t)(iv >> 32);
         for (uint32_t j = PRECALC; j < ext_len && l <= u; ++j) step(X, &l, &u, v[e - 1 - j]);
+        if (l <= u && miss_skip && k >= ext_len + 3 && e >= ext_len + 3) {
+          /* the table's presence bits: the clean (ext_len + 3)-base window */
+          const uint32_t c1 = v[e - ext_len - 1], c2 = v[e - ext_len - 2], c3 = v[e - ext_len - 3];
+          if (c1 <= 3 && c2 <= 3 && c3 <= 3) {
+            uint32_t l3 = l, u3 = u;
+            step(X, &l3, &u3, c1);
+            if (l3 <= u3) step(X, &l3, &u3, c2);
+            if (l3 <= u3) step(X, &l3, &u3, c3);
+            if (l3 > u3) continue; /* absent: as below, with k >= ext_len + 3 */
+          }
+        }
         if (l <= u) {
           have = 1;
           b = e - ext_len;

@@ -95,7 +95,9 @@ typedef struct {
    * left ends of finished tasks do not increase as the end decreases, so a
    * later candidate of length >= k lies left of both this task's left end
    * and the previous one's.  Its size records only prune (an early exit
-   * that never changes the output).  Output unchanged; counters only. */
+   * that never changes the output).  Likewise, with k >= ext_len + 3, a task
+   * whose clean (ext_len + 3)-base window is absent ends after the lookup
+   * (the device table's presence bits).  Output unchanged; counters only. */
   int miss_skip;
 } or_opts;
 

@@ -1,6 +1,7 @@
 """The extended interval table (gs_index_set_kmer_len): its entries are the
-SA intervals of all patterns of 14..16 bases, and seeding with it gives the
-same output as without (the 13-mer table plus backward steps)."""
+SA intervals of all patterns of 14..16 bases with the presence bits of their
+three-base left extensions, and seeding with it gives the same output as
+without (the 13-mer table plus backward steps)."""
 import random
 import sys
 
@@ -49,15 +50,21 @@ def test_entries_are_sa_intervals(g, tmp_path, length):
         ps = [rng.randrange(4 ** length) for _ in range(2000)]
         for i in range(0, len(text) - length, 5):   # patterns that occur
             ps.append(sum("ACGT".index(text[i + length - 1 - j]) << (2 * j) for j in range(length)))
-        for p in ps:
-            l, u = (int(v) for v in X.read_kext(p, 1)[0])
+        for i, p in enumerate(ps):
+            l, u, mlo, mhi = (int(v) for v in X.read_kext(p, 1)[0])
+            mask = mlo | (mhi << 32)
             want = steps(ox, p, length)
             if want is None:
-                assert l > u, p
+                assert l > u and mask == 0, p
             else:
                 assert (l, u) == want, p
                 pat = "".join("ACGT"[(p >> (2 * j)) & 3] for j in reversed(range(length)))
                 assert bi.interval(pat) == want
+                if i % 4 == 0:  # presence bits: c3.c2.c1.pat occurs, by substring search
+                    for bit in range(64):
+                        c1, c2, c3 = bit & 3, (bit >> 2) & 3, bit >> 4
+                        occurs = bi.interval("ACGT"[c3] + "ACGT"[c2] + "ACGT"[c1] + pat) is not None
+                        assert bool(mask >> bit & 1) == occurs, (p, bit)
         X.set_kmer_len(13)
         assert X.info().kmer_len == 13
 
