@@ -57,14 +57,15 @@ class IndexInfo(C.Structure):
                 ("L2", C.c_uint32 * 5), ("n_blocks", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("t_read", C.c_double), ("t_upload", C.c_double), ("t_repack", C.c_double),
                 ("t_table", C.c_double), ("t_sa", C.c_double), ("t_write", C.c_double),
-                ("sa_rounds", C.c_int), ("kmer_len", C.c_int), ("t_ext", C.c_double)]
+                ("sa_rounds", C.c_int), ("kmer_len", C.c_int), ("t_ext", C.c_double),
+                ("direct", C.c_int), ("t_direct", C.c_double)]
 
 
 class Opts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
                 ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("seg_tasks", C.c_uint32),
                 ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("split_tasks", C.c_uint16),
-                ("split_work", C.c_uint16)]
+                ("split_work", C.c_uint16), ("direct_min", C.c_uint16), ("pad", C.c_uint16)]
 
 
 class ServerOpts(C.Structure):
@@ -139,6 +140,9 @@ def lib():
         L.gs_index_read_kmer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_read_kext.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_set_kmer_len.argtypes = [C.c_void_p, C.c_int]
+        L.gs_index_set_direct.argtypes = [C.c_void_p, C.c_int]
+        L.gs_index_read_direct.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p,
+                                           C.POINTER(C.c_uint8)]
         L.gs_seed.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint64, C.POINTER(Opts),
                               C.POINTER(Result)]
         L.gs_result_free.argtypes = [C.POINTER(Result)]
@@ -322,6 +326,20 @@ class Index:
         _check("gs_index_read_kext", lib().gs_index_read_kext(self.h, first, count, _ptr(out, C.c_uint32)))
         return out
 
+    def set_direct(self, on=True):
+        """Adds or drops the direct-extension structures (full suffix array,
+        its inverse, 2-bit text)."""
+        _check("gs_index_set_direct", lib().gs_index_set_direct(self.h, 1 if on else 0))
+
+    def read_direct(self, first, count):
+        """(suffix array rows, inverse positions, text codes) over [first, first+count)."""
+        sa = np.empty(count, dtype=np.uint32)
+        isa = np.empty(count, dtype=np.uint32)
+        text = np.empty(count, dtype=np.uint8)
+        _check("gs_index_read_direct", lib().gs_index_read_direct(self.h, first, count, _ptr(sa, C.c_uint32),
+                                                                  _ptr(isa, C.c_uint32), _ptr(text, C.c_uint8)))
+        return sa, isa, text
+
     def read_kmer(self, first, count):
         out = np.empty((count, 2), dtype=np.uint32)
         _check("gs_index_read_kmer", lib().gs_index_read_kmer(self.h, first, count, _ptr(out, C.c_uint32)))
@@ -374,10 +392,13 @@ def pack_reads(reads):
 NO_SPLIT = 0xFFFF
 
 
+NO_DIRECT = 0xFFFF
+
+
 def make_opts(k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0, wg=0, walk_rows=0,
-              split_tasks=0, split_work=0):
+              split_tasks=0, split_work=0, direct_min=0):
     return Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks, wg, walk_rows, split_tasks,
-                split_work)
+                split_work, direct_min)
 
 
 def _arrays(arrays):
@@ -387,11 +408,11 @@ def _arrays(arrays):
 
 
 def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
-         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None):
+         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0):
     """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
     buf, offs, lens = _arrays(arrays if arrays is not None else pack_reads(reads))
     o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                walk_rows, split_tasks, split_work)
+                                                walk_rows, split_tasks, split_work, direct_min)
     smem = bool(o.smem)
     res = Result()
     _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
@@ -409,7 +430,7 @@ class BatchStats(C.Structure):
     _fields_ = [("t_h2d", C.c_double), ("t_seed", C.c_double), ("t_count", C.c_double),
                 ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
                 ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
-                ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64)]
+                ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64), ("n_direct_reads", C.c_uint64)]
 
 
 def _bind_batch(L):
@@ -444,12 +465,12 @@ class Batch:
     """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
 
     def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0,
-                 wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None):
+                 wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0):
         L = lib()
         _bind_batch(L)
         buf, offs, lens = _arrays(arrays)
         o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                    walk_rows, split_tasks, split_work)
+                                                    walk_rows, split_tasks, split_work, direct_min)
         self.smem = bool(o.smem)
         self.h = C.c_void_p()
         _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),

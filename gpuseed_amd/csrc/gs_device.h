@@ -41,6 +41,13 @@ struct DevIndex {
   // pattern extended to the left by c1, then c2, then c3 occurs)
   const uint4* kext;
   const uint32_t* sa;    // sampled suffix array, sa[0] = 0xFFFFFFFF
+  // optional direct-extension structures, else null: the full suffix array
+  // (sa_full[row] = text position, sa_full[0] = seq_len) and its inverse
+  // (isa[pos] = row), seq_len + 1 words each, and the text at 2 bits per base
+  // (base i at bits 2*(i & 15) of word i >> 4), read 64 bases per 16 bytes
+  const uint32_t* sa_full;
+  const uint32_t* isa;
+  const uint4* text;
   uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)
   uint32_t L2[5];

@@ -33,6 +33,9 @@ DevIndex gs_index::dev() const {
   d.kmer = kmer;
   d.kext = kext;
   d.sa = sa;
+  d.sa_full = sa_full;
+  d.isa = isa;
+  d.text = reinterpret_cast<const uint4*>(text);
   d.primary = primary;
   d.seq_len = seq_len;
   d.sa_intv = sa_intv;
@@ -156,6 +159,60 @@ __global__ void kmer_presence(const DevIndex X, const uint2* __restrict__ level,
   out[t] = make_uint4(iv.x, iv.y, (uint32_t)mask, (uint32_t)(mask >> 32));
 }
 
+// The full suffix array, its inverse and the text, from the occurrence
+// layout and the sampled suffix array: thread i starts at sampled row
+// i * sa_intv, whose text position the sampled array holds, and walks LF
+// (the textbook one: the row of the suffix one position to the left) until
+// it reaches position 0 or another sampled row, recording each row's
+// position and the base it precedes.  Every row is visited by exactly one
+// thread; positions fall at every step, so each walk ends.  The bounds
+// checks only guard against an inconsistent index.
+__global__ void direct_chains(const DevIndex X, uint64_t n_sa, uint32_t* __restrict__ sa_full,
+                              uint32_t* __restrict__ isa, uint8_t* __restrict__ text8) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= n_sa) return;
+  const uint32_t n = X.seq_len;
+  uint32_t row = (uint32_t)i * X.sa_intv;
+  uint32_t pos = i ? X.sa[i] : n;
+  while (row <= n && pos <= n) {
+    sa_full[row] = pos;
+    isa[pos] = row;
+    if (pos == 0) break;  // row primary: the suffix at position 0, preceded by $
+    const uint32_t kk = row - (row > X.primary ? 1u : 0u);
+    const uint32_t b = gs::div96(kk), bit = kk - b * gs::kBlockBases;
+    const uint4* p = X.occ + (size_t)b * 4;
+    const uint32_t w = bit >> 5, sh = bit & 31;
+    uint32_t c = 3;
+    uint4 e = p[3];
+    for (uint32_t s = 0; s < 3; ++s) {
+      const uint4 es = p[s];
+      const uint32_t word = w == 0 ? es.y : (w == 1 ? es.z : es.w);
+      if ((word >> sh) & 1u) {
+        c = s;
+        e = es;
+        break;
+      }
+    }
+    text8[pos - 1] = (uint8_t)c;
+    row = X.L2[c] + gs::rank_in_entry(e, bit);
+    --pos;
+    if (row - gs::div_sa(X, row) * X.sa_intv == 0) break;  // another thread's start
+  }
+}
+
+// 16 bases per word, base i at bits 2*(i & 15); words past the text are 0
+__global__ void pack_text(const uint8_t* __restrict__ text8, uint64_t n, uint32_t* __restrict__ text,
+                          uint64_t n_words) {
+  const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  uint32_t v = 0;
+  for (uint32_t q = 0; q < 16; ++q) {
+    const uint64_t pos = 16 * w + q;
+    if (pos < n) v |= (uint32_t)(text8[pos] & 3u) << (2 * q);
+  }
+  text[w] = v;
+}
+
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // One table level in launches of at most 2^30 work-items (a dispatch's
@@ -261,6 +318,80 @@ extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
   return rc;
 }
 
+static void free_direct(gs_index* X) {
+  const uint64_t rows = (uint64_t)X->seq_len + 1;
+  if (X->sa_full) X->device_bytes -= 2 * rows * sizeof(uint32_t) + X->text_words * sizeof(uint32_t);
+  (void)hipFree(X->sa_full);
+  (void)hipFree(X->isa);
+  (void)hipFree(X->text);
+  X->sa_full = X->isa = X->text = nullptr;
+  X->text_words = 0;
+}
+
+// Direct extension: once a task's interval has a single row, the seeding
+// kernels can read that row's text position from the full suffix array,
+// compare the read with the text 64 bases per 16-byte load instead of one
+// backward step per base, and read the row where the match ends from the
+// inverse suffix array.  The structures are derived here from the index
+// itself (direct_chains), so they describe exactly its text.
+extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
+  if (!X) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  if (X->sa_full) {
+    GS_HIP(hipStreamSynchronize(X->stream));
+    free_direct(X);
+  }
+  if (!on) return GS_OK;
+  const double t0 = wall();
+  const uint64_t n = X->seq_len, rows = n + 1;
+  const uint64_t words = 4 * ((n + 63) / 64 + 1);  // whole 64-base chunks plus one
+  uint8_t* text8 = nullptr;
+  int rc = GS_OK;
+  if (hipMalloc(&X->sa_full, rows * 4) != hipSuccess || hipMalloc(&X->isa, rows * 4) != hipSuccess ||
+      hipMalloc(&X->text, words * 4) != hipSuccess || hipMalloc(&text8, n + 1) != hipSuccess)
+    rc = GS_ENOMEM;
+  if (!rc) {
+    X->text_words = words;
+    X->device_bytes += 2 * rows * 4 + words * 4;
+    const DevIndex D = X->dev();
+    hipLaunchKernelGGL(direct_chains, dim3(grid_for(X->n_sa, 256)), dim3(256), 0, X->stream, D, X->n_sa,
+                       X->sa_full, X->isa, text8);
+    hipLaunchKernelGGL(pack_text, dim3(grid_for(words, 256)), dim3(256), 0, X->stream, text8, n, X->text, words);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
+  }
+  (void)hipFree(text8);
+  if (rc) {
+    if (!X->text_words) {  // not counted yet
+      (void)hipFree(X->sa_full);
+      (void)hipFree(X->isa);
+      (void)hipFree(X->text);
+      X->sa_full = X->isa = X->text = nullptr;
+    } else {
+      free_direct(X);
+    }
+  }
+  X->t_direct = wall() - t0;
+  return rc;
+}
+
+extern "C" GS_API int gs_index_read_direct(const gs_index* X, uint32_t first, uint32_t count, uint32_t* sa_out,
+                                           uint32_t* isa_out, uint8_t* text_out) {
+  if (!X || !X->sa_full || (uint64_t)first + count > (uint64_t)X->seq_len + 1) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  if (sa_out) GS_HIP(hipMemcpy(sa_out, X->sa_full + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+  if (isa_out) GS_HIP(hipMemcpy(isa_out, X->isa + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+  if (text_out && count) {
+    const uint64_t w0 = first >> 4, w1 = ((uint64_t)first + count + 15) >> 4;
+    std::vector<uint32_t> w(w1 - w0);
+    GS_HIP(hipMemcpy(w.data(), X->text + w0, w.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t q = 0; q < count; ++q) {
+      const uint64_t pos = (uint64_t)first + q;
+      text_out[q] = (uint8_t)((w[(pos >> 4) - w0] >> (2 * (pos & 15))) & 3u);
+    }
+  }
+  return GS_OK;
+}
+
 static void free_index(gs_index* X) {
   if (!X) return;
   if (X->work) gs::batch_free(X->work);
@@ -268,6 +399,7 @@ static void free_index(gs_index* X) {
   (void)hipFree(X->sa);
   (void)hipFree(X->kmer);
   (void)hipFree(X->kext);
+  free_direct(X);
   if (X->stream) (void)hipStreamDestroy(X->stream);
   delete X;
 }
@@ -393,6 +525,8 @@ extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) 
   info->sa_rounds = X->sa_rounds;
   info->kmer_len = X->kext ? (int)X->ext_len : gs::kPrecalc;
   info->t_ext = X->t_ext;
+  info->direct = X->sa_full ? 1 : 0;
+  info->t_direct = X->t_direct;
   return GS_OK;
 }
 

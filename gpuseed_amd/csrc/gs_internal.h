@@ -27,8 +27,14 @@ struct gs_index {
   uint2* kmer = nullptr;
   uint4* kext = nullptr;  // extended interval table with presence bits (gs_index_set_kmer_len)
   uint32_t ext_len = 0;
+  // direct-extension structures (gs_index_set_direct): full suffix array and
+  // its inverse (seq_len + 1 words each), the text at 2 bits per base
+  uint32_t* sa_full = nullptr;
+  uint32_t* isa = nullptr;
+  uint32_t* text = nullptr;
+  uint64_t text_words = 0;
   uint64_t device_bytes = 0;
-  double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0;
+  double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0, t_direct = 0;
   int sa_rounds = 0;
   gs_batch* work = nullptr;  // gs_seed's batch, kept for the next call (its buffers only grow)
   gs::DevIndex dev() const;
@@ -57,7 +63,8 @@ struct PersistParams {
   uint4* seg_meta;           // per item of a segmented read: {emitted, first b, last b, flags}
   uint32_t* seg_link;        // per item of a segmented read: first piece split off it, or ~0
   uint32_t* queue;           // work-queue counter
-  uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads
+  uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads, [2] direct-extension reads
+  uint32_t direct_min;       // size-1 steps before a task reads the text (0: no direct extension)
   // work splitting (dyn_cap 0: off)
   uint4* dyn_items;          // pieces {read, strand, first task, tasks}
   uint4* dyn_meta;           // per piece, as seg_meta
@@ -69,7 +76,7 @@ struct PersistParams {
   uint32_t dyn_cap, split_work, split_tasks, epoch;
   uint32_t split_below;           // split only while fewer items than this run (set at launch)
   uint32_t refill_group;          // idle lanes of a wave that trigger a refill (set at launch)
-  uint32_t pad2[2];
+  uint32_t pad2;
 };
 
 // Joins pieces (segments of long reads, split-off pieces) into the per-read

@@ -84,9 +84,14 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   uint64_t item = 0;
   int phase = 0;
   bool ext_miss = false;  // the task's extended-table pattern is absent: use the 13-mer path
+  // direct extension: single-row steps of the task so far, the text position
+  // of read[b] in a run, and why a run reads the inverse suffix array (1: the
+  // next step empties, 2: the read start, 3: an N at x)
+  uint32_t ones = 0, tq = 0;
+  int rout = 0;
   uint4* out = nullptr;
   const uint4* gcodes = nullptr;  // the stream's codes in device memory
-  uint64_t n_table = 0, n_occ = 0;
+  uint64_t n_table = 0, n_occ = 0, n_direct = 0;
 
   // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
   // an extension may run left of it and then reads device memory
@@ -148,6 +153,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   auto next_task = [&]() __attribute__((always_inline)) {
     phase = 0;
     ext_miss = false;
+    ones = 0;
     // split only in the tail: the main queue is exhausted and fewer items run
     // than half the lanes, so pieces find idle lanes instead of adding work
     if (P.dyn_cap && work > P.split_work && e - e_min >= 2 * (int32_t)P.split_tasks &&
@@ -201,6 +207,32 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       --x;
     }
   };
+  // Direct extension.  Once the interval has kept a single row for
+  // direct_min backward steps and no size record of 1 waits at x (which would
+  // prune the task at once), the task reads the row's text position from the
+  // full suffix array (phase 3) and compares the read with the text 64 bases
+  // per 16-byte load (phase 4): while the bases agree, the pattern occurs
+  // exactly once, one position further left, so each step's interval is one
+  // row and its size record is 1.  A mismatch or text position 0 means the
+  // next step empties.  The row where the match ends comes from the inverse
+  // suffix array (phase 5) when it is needed: for a candidate that is
+  // emitted, or to go on with backward steps past an N.
+  auto try_direct = [&]() __attribute__((always_inline)) {
+    if (P.direct_min && s == 1u && ones >= P.direct_min) {
+      const int32_t j = x - w0;
+      if (j < 0 || (uint32_t)A[j] != 1u) phase = 3;
+    }
+  };
+  // how a run ends: oc 1 the next step empties, 2 the read start, 3 an N at x;
+  // returns the task result (0 while the inverse read is pending)
+  auto run_end = [&](int oc) __attribute__((always_inline)) -> int {
+    if (oc == 3 || ((uint32_t)e - b >= k && (!P.smem || b < last_b))) {
+      rout = oc;
+      phase = 5;
+      return 0;
+    }
+    return oc == 1 ? 2 : 3;  // nothing is emitted, so the row is not needed
+  };
   // Loads task ends e_hi .. e_hi - tcount + 1 of stream (r, strand) into the
   // lane: the LDS code window [w0, top) covers every task's 13-base start.
   auto start_item = [&](uint32_t tstart, uint32_t tcount, uint32_t link0) __attribute__((always_inline)) {
@@ -242,6 +274,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     ++started;
     phase = 0;
     ext_miss = false;
+    ones = 0;
     if (nbad != 0) next_task();  // no work done yet, so this never splits
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
@@ -362,7 +395,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       poll_pieces();
       take_pending();
     }
-    const int op = have ? (phase == 0 ? 1 : 2) : 0;
+    const int op = have ? (phase == 0 ? 1 : (phase == 1 ? 2 : phase)) : 0;
     if (__ballot(op != 0) == 0) {
       if (P.dyn_cap) fold_running();
       if (__ballot(!finished) == 0) break;
@@ -414,15 +447,29 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       n_occ += a2 ? 2 : 1;
       work += a2 ? 2 : 1;
     }
+    const uint32_t* w4 = nullptr;  // a 4-byte read: suffix array (op 3) or its inverse (op 5)
+    if (op == 3) {
+      w4 = P.X.sa_full + l;
+    } else if (op == 4) {
+      a1 = P.X.text + ((tq - 1u) >> 6);  // tq >= 1 whenever phase 4 is entered
+    } else if (op == 5) {
+      w4 = P.X.isa + tq;
+    }
+    if (op >= 3) {
+      ++n_direct;
+      ++work;
+    }
     uint4 d1 = make_uint4(0, 0, 0, 0);
     if (a1) d1 = *a1;
     uint4 d2 = d1;
     if (a2) d2 = *a2;
+    uint32_t v4 = 0;
+    if (w4) v4 = *w4;
 
     // the new interval: a table entry (op 1) or one backward step (op 2);
     // both then share one copy of the task logic, so a wave whose lanes are
     // in different phases runs it once
-    uint32_t nl, nu, nb;
+    uint32_t nl = 0, nu = 0, nb = 0;
     bool absent = false;  // the clean (tlen + 3)-base window does not occur
     if (op == 1) {
       if (tlen > gs::kPrecalc) {
@@ -434,7 +481,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         nu = (tidx & 1u) ? d1.w : d1.y;
       }
       nb = (uint32_t)e - tlen;
-    } else {
+    } else if (op == 2) {
       const uint32_t o2 = gs::rank_in_entry(d1, i2 - b2 * gs::kBlockBases);
       const uint32_t o1 = have1 ? gs::rank_in_entry(d2, i1 - b1 * gs::kBlockBases) : 0u;
       const uint32_t base = pick4(cx, L2_0, L2_1, L2_2, L2_3);
@@ -442,7 +489,67 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       nu = base + o2;
       nb = (uint32_t)x;
     }
-    if (op != 0) {
+    if (op >= 3) {
+      int res = 0;
+      if (op == 3) {  // the text position of read[b]
+        tq = v4;
+        if (tq == 0)
+          res = run_end(1);  // the pattern starts the text: no step can extend it
+        else
+          phase = 4;
+      } else if (op == 4) {  // compare down to the start of the loaded chunk
+        const uint32_t ci = (tq - 1u) >> 6;
+        int oc = 0;  // 0: chunk used up, load the next one
+        while (true) {
+          if (tq == 0) {
+            oc = 1;
+            break;
+          }
+          const uint32_t tp = tq - 1u;
+          if ((tp >> 6) != ci) break;
+          const uint32_t wv = pick4((tp >> 4) & 3u, d1.x, d1.y, d1.z, d1.w);
+          if (((wv >> ((tp & 15u) << 1)) & 3u) != cx) {
+            oc = 1;
+            break;
+          }
+          b = (uint32_t)x;  // one row, one position further left
+          --tq;
+          if (prune_at(x, 1u)) {
+            oc = 4;
+            break;
+          }
+          x = (int32_t)b - 1;
+          if (x < 0) {
+            oc = 2;
+            break;
+          }
+          const uint32_t c = code(x);
+          if (c > 3) {
+            oc = 3;
+            break;
+          }
+          cx = c;
+        }
+        if (oc == 4)
+          res = 1;
+        else if (oc != 0)
+          res = run_end(oc);
+      } else {  // op 5: the row of read[b .. e)
+        l = u = v4;
+        if (rout == 3) {
+          phase = 1;
+          res = prepare_x();  // consumes the N as a backward step would
+          if (res == 0) try_direct();
+        } else {
+          res = rout == 1 ? 2 : 3;
+        }
+      }
+      if (res >= 2) finish_task();
+      if (res == 3 && P.smem)
+        end_stream();
+      else if (res != 0)
+        next_task();
+    } else if (op != 0) {
       // how the task goes on: 0 continues, 1 pruned or a table miss (no
       // candidate, 2145), 2 its interval emptied, 3 it reached the read
       // start; the task logic below is inlined once for every lane
@@ -462,6 +569,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       } else if (absent) {
         res = 1;  // the same holds for an absent clean (tlen + 3)-base window with k >= tlen + 3
       } else {
+        if (op == 2) ones = nl == nu ? ones + 1u : 0u;
         l = nl;
         u = nu;
         s = u - l + 1u;
@@ -472,6 +580,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
           x = (int32_t)b - 1;
           phase = 1;
           res = prepare_x();
+          if (res == 0) try_direct();
         }
       }
       if (res >= 2) finish_task();
@@ -487,10 +596,12 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     for (int d = 32; d > 0; d >>= 1) {
       n_table += __shfl_down(n_table, d, 64);
       n_occ += __shfl_down(n_occ, d, 64);
+      n_direct += __shfl_down(n_direct, d, 64);
     }
     if (lane == 0) {
       atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)n_table);
       atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)n_occ);
+      atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)n_direct);
     }
   }
 }
@@ -698,7 +809,7 @@ int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid
     hipLaunchKernelGGL(zero_words, dim3(3), dim3(64), 0, st, P.dyn_ctl, 192u);
     hipLaunchKernelGGL(zero_words, dim3((P.dyn_cap + 63) / 64), dim3(64), 0, st, P.dyn_ready, P.dyn_cap);
   }
-  if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 4u);
+  if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 6u);
   constexpr int L = kPersistLmax;
   switch (variant * 1000 + (wg ? wg : 128)) {
     case 64: return launch_t<uint8_t, L, 64>(P, grid, st);

@@ -796,6 +796,7 @@ int gs::plan_check(const gs_opts* o) {
   if (o->kernel < 0 || o->kernel > 2) return GS_EINVAL;
   if (o->split_tasks > 64 && o->split_tasks != 0xFFFF) return GS_EINVAL;
   if (o->split_work == 0xFFFF) return GS_EINVAL;
+  if (o->direct_min > 64 && o->direct_min != 0xFFFF) return GS_EINVAL;
   return GS_OK;
 }
 
@@ -1091,6 +1092,7 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   PP.seg_link = B->seg_link.as<uint32_t>();
   PP.queue = B->queue.as<uint32_t>();
   PP.counters = B->counters.as<uint64_t>();
+  PP.direct_min = !S.X.isa || B->opts.direct_min == 0xFFFF ? 0u : (B->opts.direct_min ? B->opts.direct_min : 3u);
   PP.dyn_items = B->dyn_items.as<uint4>();
   PP.dyn_meta = B->dyn_meta.as<uint4>();
   PP.dyn_link = B->dyn_link.as<uint32_t>();
@@ -1229,10 +1231,11 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
     stats->n_seeds = B->n_seeds;
     stats->n_pos = B->n_pos;
     if (kernel >= 1) {
-      uint64_t c[2] = {0, 0};
-      if (hipMemcpy(c, B->counters.p, 16, hipMemcpyDeviceToHost) == hipSuccess) {
+      uint64_t c[3] = {0, 0, 0};
+      if (hipMemcpy(c, B->counters.p, 24, hipMemcpyDeviceToHost) == hipSuccess) {
         stats->n_table_reads = c[0];
         stats->n_occ_reads = c[1];
+        stats->n_direct_reads = c[2];
       }
     }
   }

@@ -166,6 +166,8 @@ typedef struct {
   int sa_rounds;        /* gs_index_build: prefix-doubling rounds            */
   int kmer_len;         /* longest interval table on the device (13..16)     */
   double t_ext;         /* gs_index_set_kmer_len: table build, seconds       */
+  int direct;           /* 1: direct-extension structures on the device      */
+  double t_direct;      /* gs_index_set_direct: build, seconds               */
 } gs_index_info;
 
 GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
@@ -187,6 +189,20 @@ GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t coun
  * back to those steps when the pattern does not occur or its window holds a
  * code above 3.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
+/* Adds (on != 0) or drops the direct-extension structures: the full suffix
+ * array, its inverse and the text at 2 bits per base, derived on the device
+ * from the index itself (8 * (seq_len + 1) + seq_len / 4 bytes of HBM: 25.6 GB
+ * at 3.1 Gbp).  The batched kernels then finish a task whose interval has kept
+ * a single row for gs_opts.direct_min backward steps by reading that row's
+ * text position, comparing the read with the text 64 bases per 16-byte load
+ * and reading the row where the match ends, in place of one backward step
+ * per base.  The output is unchanged.  No counterpart in the reference. */
+GS_API int gs_index_set_direct(gs_index* idx, int on);
+/* Rows [first, first+count) of the full suffix array and positions
+ * [first, first+count) of its inverse and of the text (one code 0..3 per
+ * byte); any output may be NULL (tests). */
+GS_API int gs_index_read_direct(const gs_index* idx, uint32_t first, uint32_t count, uint32_t* sa_out,
+                                uint32_t* isa_out, uint8_t* text_out);
 /* Entries [first, first+count) of the extended table (tests): four words
  * per entry, l, u and the low and high words of the presence mask (bit
  * c1 + 4*c2 + 16*c3 set iff the pattern extended to the left by c1, then c2,
@@ -228,9 +244,14 @@ typedef struct {
                        2 * split_tasks remain, 1..64; 0 = 8, and only in
                        batches of at most 4 work items per lane of the
                        persistent grid; 0xFFFF: no splitting                 */
-  uint16_t split_work;  /* in units of 64 memory operations, 1..0xFFFE (0 = 32).
-                       The output depends on none of the tuning fields
-                       (kernel, seg_tasks, wg, walk_rows, split_*)           */
+  uint16_t split_work;  /* in units of 64 memory operations, 1..0xFFFE (0 = 32) */
+  uint16_t direct_min;  /* batched kernels on an index with gs_index_set_direct:
+                       a task reads the text once its interval has kept one
+                       row for this many backward steps, 1..64 (0 = 3);
+                       0xFFFF: never.  The output depends on none of the
+                       tuning fields (kernel, seg_tasks, wg, walk_rows,
+                       split_*, direct_min)                                  */
+  uint16_t pad;
 } gs_opts;
 
 typedef struct {
@@ -276,6 +297,7 @@ typedef struct {
   uint64_t n_reads, n_seeds, n_pos;
   uint64_t n_table_reads, n_occ_reads; /* memory operations of the batched
                       extension kernel (0 for the plain kernel)                */
+  uint64_t n_direct_reads; /* its suffix-array, text and inverse reads       */
 } gs_batch_stats;
 
 GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,

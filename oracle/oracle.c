@@ -209,7 +209,6 @@ uint32_t or_locate(const or_index* X, uint32_t row) { return locate_count(X, row
  * primary+1 (or_lf above); this one is the true SA, which the device's direct
  * extension reads (not an output of the reference). */
 static uint32_t lf_true(const or_index* X, uint32_t row) {
-  if (row == X->primary) return 0; /* the suffix at text position 0 */
   uint32_t k = row - (row > X->primary ? 1u : 0u);
   const uint32_t* blk = X->bwt + (uint64_t)(k >> 6) * 8;
   uint32_t c = (blk[4 + ((k & 63) >> 4)] >> (((~k) & 15) << 1)) & 3;
@@ -221,10 +220,11 @@ uint32_t or_sa_true(const or_index* X, uint32_t row) {
   if (row == 0) return X->seq_len;
   uint32_t itr = 0;
   while (row % X->sa_intv) {
+    if (row == X->primary) return itr; /* text position 0: the walk would wrap to $ */
     ++itr;
     row = lf_true(X, row);
   }
-  return row == 0 ? X->seq_len + itr : X->sa[row / X->sa_intv] + itr;
+  return row == 0 ? X->seq_len : X->sa[row / X->sa_intv] + itr;
 }
 
 /* ---- 13-mer table (2648-2675) ------------------------------------------- */
@@ -327,7 +327,7 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text;
 } work_t;
 
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
@@ -357,11 +357,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
   if (L < k) return 0;
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
   uint32_t last_b = 0xFFFFFFFFu; /* left end of the last unpruned task (the device's in-lane SMEM filter) */
-  /* the last direct run's region: read positions [reg_lo, reg_hi) aligned to
-   * the text from reg_t (text position of read[reg_lo]); the 128-position
-   * chunk of the uniqueness bitmap read last */
-  uint32_t reg_lo = 0, reg_hi = 0, reg_t = 0;
-  uint64_t uchunk = ~0ull;
   for (uint32_t e = L; e >= k; --e) {
     uint32_t p = 0;
     int bad = 0;
@@ -379,21 +374,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     if (ext_len > PRECALC && e >= ext_len) {
       int clean = 1;
       for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
-      const uint32_t wb = e - ext_len;
-      if (clean && direct_min && prune && wb >= reg_lo && wb < reg_hi && A[wb] == 1) {
-        /* the window is the text at reg_t + (wb - reg_lo): its uniqueness bit
-         * decides the task without a lookup when the pattern occurs once (then
-         * its size equals the record and the task is pruned) */
-        const uint64_t t = (uint64_t)reg_t + (wb - reg_lo);
-        if ((t >> 7) != uchunk) {
-          uchunk = t >> 7;
-          ++w->direct;
-        }
-        uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
-        uint32_t l3 = (uint32_t)iv, u3 = (uint32_t)(iv >> 32);
-        for (uint32_t j = PRECALC; j < ext_len && l3 <= u3; ++j) step(X, &l3, &u3, v[e - 1 - j]);
-        if (l3 <= u3 && l3 == u3) continue; /* unique: pruned (A[wb] == 1) */
-      }
       if (clean) { /* one lookup answers the first ext_len bases when they occur */
         ++w->ext;
         uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
@@ -440,8 +420,8 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     }
     /* direct-extension model: in a run, the text position of read[b] and
      * the last 64-base text chunk read */
-    int run = 0, sawn = 0;
-    uint32_t tq = 0, ones = 0, b0 = 0;
+    int run = 0;
+    uint32_t tq = 0, ones = 0;
     uint64_t chunk = ~0ull;
     for (int64_t x = (int64_t)b - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
       uint32_t c = v[x], l2 = l, u2 = u;
@@ -450,13 +430,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         ++w->direct;
         tq = or_sa_true(X, l);
         chunk = ~0ull;
-        b0 = b;
-        /* without an N in the pattern, all of [b, e) is aligned to the text */
-        reg_lo = b;
-        reg_hi = sawn ? b : e;
-        reg_t = tq;
       }
-      if (c > 3) sawn = 1;
       if (run && c > 3) { /* an N ends the run: the row of read[b] is read back */
         run = 0;
         ++w->direct;
@@ -467,6 +441,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         } else if (tq > 0 && (uint64_t)((tq - 1) >> 6) != chunk) {
           chunk = (tq - 1) >> 6; /* text chunk holding text[tq - 1] */
           ++w->direct;
+          ++w->text;
         }
       }
       step(X, &l2, &u2, c);
@@ -479,11 +454,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       l = l2;
       u = u2;
       b = (uint32_t)x; /* 2252-2255 */
-      if (run) {
-        --tq;
-        reg_lo = b;
-        reg_t = tq;
-      }
+      if (run) --tq;
       if (prune) {
         uint32_t s = u - l + 1;
         if (sat && s >= sat) s = sat;
@@ -852,6 +823,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_lf += work[t].lf;
     res->w_sa += work[t].sa;
     res->w_direct += work[t].direct;
+    res->w_text += work[t].text;
     free(scratch[t]);
   }
   if (!o->prune) res->w_steps_unpruned = res->w_steps;

@@ -128,7 +128,9 @@ typedef struct {
   uint64_t w_steps_unpruned; /* steps with no pruning at all                 */
   uint64_t n_short;       /* reads shorter than k (no tasks)                  */
   uint64_t w_ext;         /* extended-table lookups (opts.ext_len)            */
-  uint64_t w_direct;      /* direct-extension reads (opts.direct_min)         */
+  uint64_t w_direct;      /* direct-extension reads (opts.direct_min): 4-byte
+                             suffix-array and inverse reads and, of them,     */
+  uint64_t w_text;        /* 16-byte text reads                               */
 } or_result;
 
 /* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line

@@ -34,7 +34,7 @@ class OrResult(C.Structure):
                 ("w_tasks", C.c_uint64), ("w_steps", C.c_uint64), ("w_blocks64", C.c_uint64),
                 ("w_blocks96", C.c_uint64), ("w_lf", C.c_uint64), ("w_sa", C.c_uint64),
                 ("w_steps_unpruned", C.c_uint64), ("n_short", C.c_uint64), ("w_ext", C.c_uint64),
-                ("w_direct", C.c_uint64)]
+                ("w_direct", C.c_uint64), ("w_text", C.c_uint64)]
 
 
 _lib = None
@@ -60,6 +60,8 @@ def lib():
         L.or_lf.restype = C.c_uint32
         L.or_locate.argtypes = [C.c_void_p, C.c_uint32]
         L.or_locate.restype = C.c_uint32
+        L.or_sa_true.argtypes = [C.c_void_p, C.c_uint32]
+        L.or_sa_true.restype = C.c_uint32
         L.or_code_fwd.argtypes = [C.c_uint8]
         L.or_code_fwd.restype = C.c_uint32
         L.or_code_rev.argtypes = [C.c_uint8]
@@ -120,6 +122,9 @@ class Index:
     def locate(self, row):
         return lib().or_locate(self.h, row)
 
+    def sa_true(self, row):
+        return lib().or_sa_true(self.h, row)
+
 
 def pack_reads(reads):
     """reads: list of bytes -> (buffer, offsets u64, lens u32)"""
@@ -158,7 +163,7 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem
             out.append(lst)
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct")}
+                                                 "w_direct", "w_text")}
         return out, counters
     finally:
         lib().or_result_free(C.byref(res))
@@ -190,7 +195,7 @@ def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, 
         po = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct")}
+                                                 "w_direct", "w_text")}
         return seeds, pos, so, po, counters
     finally:
         lib().or_result_free(C.byref(res))

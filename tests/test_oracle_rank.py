@@ -86,6 +86,14 @@ def test_locate_is_sa_when_walk_avoids_row_after_primary(pair):
             assert ox.locate(row) == bi.sa[row]
 
 
+def test_true_sa_every_row(pair):
+    """or_sa_true (the textbook walk, which checks the device's full suffix
+    array) is the sorted-suffix order itself, row primary+1 included."""
+    bi, ox = pair
+    for row in range(bi.n + 1):
+        assert ox.sa_true(row) == bi.sa[row], row
+
+
 def test_known_answer_row_after_primary(built, tmp_path):
     """Hand-worked case of seed_gen.cu:455-457.  T = AC: suffixes $, AC$, C$ give
     SA = [2, 0, 1] and primary = 1.  With sa_intv = 4, row 2 (SA 1) walks once:
