@@ -49,13 +49,13 @@ def cpu_model():
     return "unknown"
 
 
-def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True):
+def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0):
     """Memory operations per read of the device algorithm, counted by the
-    oracle on the first `sample` reads: 13-mer table lookups, extended-table
-    lookups (8 bytes each) and occ entries (16 bytes each), with distance-1
-    pruning, the SMEM read-start stop and tasks whose clean extended-table
-    window is absent ending at once; and the same count for the
-    reference's procedure (13-mer table, distance-1 pruning only)."""
+    oracle on the first `sample` reads: 13-mer table lookups (8 bytes),
+    extended-table lookups (16 bytes: interval and presence bits), occ
+    entries (16 bytes) and direct-extension reads (4-byte suffix-array and
+    inverse words, 16-byte text chunks), with distance-1 pruning, the SMEM
+    read-start stop and tasks that cannot reach k bases ending at once."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     buf, offs, lens = arrays
@@ -64,12 +64,16 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True):
     sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
     c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
-                       ext_len=ext_len if ext_len > 13 else 0, miss_skip=1)[4]
+                       ext_len=ext_len if ext_len > 13 else 0, miss_skip=1, direct_min=direct_min)[4]
     ox.close()
     table = (c["w_tasks"] + c["w_ext"]) / n
     occ = c["w_blocks96"] / n
-    return dict(reads=n, table_per_read=table, occ_per_read=occ, requests_per_read=table + occ,
-                bytes_per_read=8.0 * table + 16.0 * occ)
+    direct, text = c["w_direct"] / n, c["w_text"] / n
+    ext_bytes = 16.0 if ext_len > 13 else 8.0
+    return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct,
+                requests_per_read=table + occ + direct,
+                bytes_per_read=8.0 * c["w_tasks"] / n + ext_bytes * c["w_ext"] / n + 16.0 * occ
+                + 4.0 * (direct - text) + 16.0 * text)
 
 
 def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ, smem=True):
@@ -212,7 +216,7 @@ def serve_bench(g, X, ref, k, max_occ, args):
                          "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
 
 
-def profiled_traffic(config, kernel, kmer_len, reads):
+def profiled_traffic(config, kernel, kmer_len, direct, reads):
     """HBM bytes per launch of the extension kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled to `reads`
     per launch, if it was measured on this workload; else None."""
@@ -224,7 +228,7 @@ def profiled_traffic(config, kernel, kmer_len, reads):
         parts = line.split()
         if len(parts) == 2 and not line.startswith("#"):
             kv[parts[0]] = float(parts[1])
-    if kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len:
+    if kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len or kv.get("direct", 0) != direct:
         return None, None
     per_launch = (kv["fetch_kib"] * kv["fetch_factor"] + kv["write_kib"] * kv["write_factor"]) * 1024.0
     src = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE at %d reads per launch (%s), scaled by the stream-copy "
@@ -269,6 +273,8 @@ def main():
     ap.add_argument("--kmer-len", type=int, default=0,
                     help="interval table length: 13 is the reference's table, 14..16 add a longer one "
                          "(default 16; configuration 1, a 1 Mbp reference: 13)")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="no direct-extension structures (full suffix array, inverse, text)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
@@ -339,6 +345,11 @@ def main():
         X.set_kmer_len(args.kmer_len)
         info = X.info()
         log("%d-mer interval table in %.2fs, %.2f GB on device" % (info.kmer_len, info.t_ext, info.device_bytes / 1e9))
+    direct = 0 if args.no_direct or args.config == 1 else 1
+    if direct:
+        X.set_direct(True)
+        info = X.info()
+        log("direct-extension structures in %.2fs, %.2f GB on device" % (info.t_direct, info.device_bytes / 1e9))
     if args.serve_only:
         serving = serve_bench(g, X, ref, k, max_occ, args)
         X.close()
@@ -421,9 +432,10 @@ def main():
             % (cpu["reads_per_s"], threads, cpu["mismatches"], cpu["reads"]))
     wm = None
     if not args.no_cpu:
-        wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem)
-        log("work model: %.1f table + %.1f occ requests per read (%.0f bytes)" %
-            (wm["table_per_read"], wm["occ_per_read"], wm["bytes_per_read"]))
+        wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem,
+                        direct_min=4 if direct else 0)
+        log("work model: %.1f table + %.1f occ + %.1f direct requests per read (%.0f bytes)" %
+            (wm["table_per_read"], wm["occ_per_read"], wm["direct_per_read"], wm["bytes_per_read"]))
     serving = None
     if serve_ref is not None:
         serving = serve_bench(g, X, serve_ref, k, max_occ, args)
@@ -437,13 +449,13 @@ def main():
         extra["parity"] = {"reads_checked": cpu["reads"], "mismatches": cpu["mismatches"],
                            "bit_exact": cpu["bit_exact"]}
     if wm:
-        # extension kernel, algorithmic: the device algorithm's table entries
-        # (8 bytes) and distinct 96-base occ entries per backward step (16
-        # bytes), counted by the oracle; W for the reference's procedure
+        # extension kernel, algorithmic: the device algorithm's table entries,
+        # distinct 96-base occ entries per backward step and direct-extension
+        # reads, counted by the oracle; W for the reference's procedure
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
-        act = sum(st.n_table_reads + st.n_occ_reads for st in stats) / (len(stats) * args.batch)
-        traffic, traffic_src = profiled_traffic(args.config, args.kernel, info.kmer_len, args.batch)
+        act = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads for st in stats) / (len(stats) * args.batch)
+        traffic, traffic_src = profiled_traffic(args.config, args.kernel, info.kmer_len, direct, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
                     "frac": achieved / 8e12, "traffic": traffic, "traffic_source": traffic_src,
@@ -481,6 +493,7 @@ def main():
                                   rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
                    "mode": "SMEM" if smem else "MEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
+                   "direct_extension": bool(direct),
                    "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {

@@ -219,8 +219,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   // emitted, or to go on with backward steps past an N.
   auto try_direct = [&]() __attribute__((always_inline)) {
     if (P.direct_min && s == 1u && ones >= P.direct_min) {
-      const int32_t j = x - w0;
-      if (j < 0 || (uint32_t)A[j] != 1u) phase = 3;
+      const int32_t j = x - w0;  // runs stay inside the LDS window
+      if (j >= 0 && (uint32_t)A[j] != 1u) phase = 3;
     }
   };
   // how a run ends: oc 1 the next step empties, 2 the read start, 3 an N at x;
@@ -466,6 +466,61 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     uint32_t v4 = 0;
     if (w4) v4 = *w4;
 
+    // Text comparison of the lanes in phase 4, one such lane at a time with
+    // the whole wave: lane i checks position x - i of that lane's read against
+    // text position tq - 1 - i (codes and size records from the lane's LDS
+    // slot, the text from the chunk it loaded), up to the end of the chunk or
+    // of the LDS window.  The first mismatch, N or size record of 1 ends the
+    // comparison; lanes before it write the records of 1.  A chunk costs a
+    // few wave instructions instead of a loop over 64 positions in one lane.
+    uint32_t run_g = 0;  // positions matched (and recorded) by this lane's comparison
+    int run_kind = 0;    // 0: all matched; 1: a mismatch, 3: an N at position x - run_g; 4: pruned there
+    {
+      uint64_t runners = __ballot(op == 4);
+      while (runners) {
+        const int R = __ffsll((unsigned long long)runners) - 1;
+        runners &= runners - 1;
+        const uint32_t r_tq = (uint32_t)__builtin_amdgcn_readlane((int)tq, R);
+        const int32_t r_x = __builtin_amdgcn_readlane(x, R);
+        const int32_t r_w0 = __builtin_amdgcn_readlane(w0, R);
+        const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)d1.x, R);
+        const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)d1.y, R);
+        const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)d1.z, R);
+        const uint32_t t3 = (uint32_t)__builtin_amdgcn_readlane((int)d1.w, R);
+        const uint32_t tp = r_tq - 1u;  // r_tq >= 1 in phase 4
+        const int32_t jx = r_x - r_w0;  // >= 0: runs start inside the window and stop at its edge
+        int32_t m = (int32_t)(tp & 63u) + 1;  // positions of the chunk at or below tp
+        m = m < jx + 1 ? m : jx + 1;
+        const uint32_t rs = ((threadIdx.x & ~63u) + (uint32_t)R) * SLOT;  // the runner's LDS slot
+        const bool valid = lane < m;
+        const int32_t j = jx - lane;
+        uint32_t cc = 0, rec = 0, tb = 0;
+        if (valid) {
+          cc = (lds[rs + (j >> 1)] >> ((j & 1) << 2)) & 15u;
+          rec = (uint32_t)reinterpret_cast<const AT*>(lds + rs + CODE_BYTES)[j];
+          const uint32_t t = tp - (uint32_t)lane;
+          tb = (pick4((t >> 4) & 3u, t0, t1, t2, t3) >> ((t & 15u) << 1)) & 3u;
+        }
+        const uint64_t bad = __ballot(valid && cc != tb);  // a code above 3 never equals a base
+        const uint64_t nmask = __ballot(valid && cc > 3);
+        const uint64_t prn = __ballot(valid && cc == tb && rec == 1u);
+        const uint32_t fb = bad ? (uint32_t)__ffsll((unsigned long long)bad) - 1u : 64u;
+        const uint32_t fp = prn ? (uint32_t)__ffsll((unsigned long long)prn) - 1u : 64u;
+        uint32_t g = fb < fp ? fb : fp;  // the first event, 64 if none
+        g = g < (uint32_t)m ? g : (uint32_t)m;
+        if ((uint32_t)lane < g) reinterpret_cast<AT*>(lds + rs + CODE_BYTES)[j] = (AT)1;
+        int kind = 0;
+        if (fp < fb)
+          kind = 4;
+        else if (fb < 64u)
+          kind = ((nmask >> fb) & 1u) ? 3 : 1;
+        if (lane == R) {
+          run_g = g;
+          run_kind = kind;
+        }
+      }
+    }
+
     // the new interval: a table entry (op 1) or one backward step (op 2);
     // both then share one copy of the task logic, so a wave whose lanes are
     // in different phases runs it once
@@ -497,38 +552,26 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
           res = run_end(1);  // the pattern starts the text: no step can extend it
         else
           phase = 4;
-      } else if (op == 4) {  // compare down to the start of the loaded chunk
-        const uint32_t ci = (tq - 1u) >> 6;
+      } else if (op == 4) {  // the wave compared this lane's chunk above
         int oc = 0;  // 0: chunk used up, load the next one
-        while (true) {
-          if (tq == 0) {
-            oc = 1;
-            break;
+        if (run_kind == 4) {
+          oc = 4;
+        } else {
+          if (run_g) {
+            b = (uint32_t)(x - (int32_t)run_g + 1);  // one row, run_g positions further left
+            tq -= run_g;
+            x = (int32_t)b - 1;
           }
-          const uint32_t tp = tq - 1u;
-          if ((tp >> 6) != ci) break;
-          const uint32_t wv = pick4((tp >> 4) & 3u, d1.x, d1.y, d1.z, d1.w);
-          if (((wv >> ((tp & 15u) << 1)) & 3u) != cx) {
-            oc = 1;
-            break;
-          }
-          b = (uint32_t)x;  // one row, one position further left
-          --tq;
-          if (prune_at(x, 1u)) {
-            oc = 4;
-            break;
-          }
-          x = (int32_t)b - 1;
-          if (x < 0) {
+          if (run_kind)
+            oc = run_kind;  // 1: the step at x empties; 3: x holds an N
+          else if (x < 0)
             oc = 2;
-            break;
-          }
-          const uint32_t c = code(x);
-          if (c > 3) {
-            oc = 3;
-            break;
-          }
-          cx = c;
+          else if (x < w0)
+            oc = 3;  // left of the window: backward steps go on from the row read back
+          else if (code(x) > 3)
+            oc = 3;  // checked before the next chunk's load, which an N does not need
+          else if (tq == 0)
+            oc = 1;  // the pattern now starts the text: the next step empties
         }
         if (oc == 4)
           res = 1;

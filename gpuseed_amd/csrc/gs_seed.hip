@@ -1092,7 +1092,7 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   PP.seg_link = B->seg_link.as<uint32_t>();
   PP.queue = B->queue.as<uint32_t>();
   PP.counters = B->counters.as<uint64_t>();
-  PP.direct_min = !S.X.isa || B->opts.direct_min == 0xFFFF ? 0u : (B->opts.direct_min ? B->opts.direct_min : 3u);
+  PP.direct_min = !S.X.isa || B->opts.direct_min == 0xFFFF ? 0u : (B->opts.direct_min ? B->opts.direct_min : 4u);
   PP.dyn_items = B->dyn_items.as<uint4>();
   PP.dyn_meta = B->dyn_meta.as<uint4>();
   PP.dyn_link = B->dyn_link.as<uint32_t>();

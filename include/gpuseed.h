@@ -247,7 +247,7 @@ typedef struct {
   uint16_t split_work;  /* in units of 64 memory operations, 1..0xFFFE (0 = 32) */
   uint16_t direct_min;  /* batched kernels on an index with gs_index_set_direct:
                        a task reads the text once its interval has kept one
-                       row for this many backward steps, 1..64 (0 = 3);
+                       row for this many backward steps, 1..64 (0 = 4);
                        0xFFFF: never.  The output depends on none of the
                        tuning fields (kernel, seg_tasks, wg, walk_rows,
                        split_*, direct_min)                                  */
