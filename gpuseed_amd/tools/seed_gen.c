@@ -25,9 +25,13 @@
  *                      (seed_gen.cu:3553-3558); off by default, the library
  *                      has no such limit
  *   --kmer-len N       13 (default): the reference's 13-mer table only;
- *                      14..16: also an N-mer interval table (8 * 4^N bytes
- *                      of HBM) that answers a task's first N bases with one
- *                      lookup; the output is the same
+ *                      14..16: also an N-mer interval table with presence
+ *                      bits (16 * 4^N bytes of HBM) that answers a task's
+ *                      first N bases with one lookup; the output is the same
+ *   --direct           also the full suffix array, its inverse and the text
+ *                      (8 bytes per base plus 2 bits): the batched kernels
+ *                      then compare single-row matches with the text 64
+ *                      bases per load; the output is the same
  *   --autotune         pick the batched kernel's launch parameters by timing
  *                      them on the first device batch (same output)
  *
@@ -358,7 +362,7 @@ int main(int argc, char** argv) {
   memset(&o, 0, sizeof o);
   o.k = 20;
   o.kernel = 1;
-  int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, c, li;
+  int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, direct = 0, c, li;
   uint64_t device_bases = 256000000ull;
   static const struct option longopts[] = {{"kernel", required_argument, 0, 1},
                                            {"max-occ", required_argument, 0, 2},
@@ -367,6 +371,7 @@ int main(int argc, char** argv) {
                                            {"reference-capacity", no_argument, 0, 5},
                                            {"kmer-len", required_argument, 0, 6},
                                            {"autotune", no_argument, 0, 7},
+                                           {"direct", no_argument, 0, 8},
                                            {0, 0, 0, 0}};
   while ((c = getopt_long(argc, argv, "k:so", longopts, &li)) >= 0) {
     switch (c) {
@@ -380,12 +385,13 @@ int main(int argc, char** argv) {
       case 5: reference_capacity = 1; break;
       case 6: kmer_len = atoi(optarg); break;
       case 7: autotune = 1; break;
+      case 8: direct = 1; break;
       default: break; /* like the reference: getopt's message, then go on */
     }
   }
   if (optind + 2 > argc) {
     fprintf(stderr, "Usage: seed_gen [-k INT] [-s] [-o] [--kernel N] [--max-occ N] [--device-bases N] "
-                    "[--device N] [--reference-capacity] [--kmer-len 13..16] [--autotune] "
+                    "[--device N] [--reference-capacity] [--kmer-len 13..16] [--direct] [--autotune] "
                     "<index-prefix> <reads.fa>\n");
     return 1;
   }
@@ -432,6 +438,16 @@ int main(int argc, char** argv) {
     }
     gs_index_get_info(idx, &info);
     fprintf(stderr, "%d-mer interval table in %.3f seconds, %.2f GB on the GPU\n", kmer_len, info.t_ext,
+            info.device_bytes / 1e9);
+  }
+  if (direct) {
+    rc = gs_index_set_direct(idx, 1);
+    if (rc) {
+      fprintf(stderr, "seed_gen: building the direct-extension structures failed: %s (%d)\n", errname(rc), rc);
+      return 1;
+    }
+    gs_index_get_info(idx, &info);
+    fprintf(stderr, "direct-extension structures in %.3f seconds, %.2f GB on the GPU\n", info.t_direct,
             info.device_bytes / 1e9);
   }
 

@@ -58,12 +58,20 @@ def same(a, b):
             np.array_equal(a.read_seed_off, b.read_seed_off) and np.array_equal(a.read_pos_off, b.read_pos_off))
 
 
-@pytest.mark.parametrize("smem,k,round_bases,growth", [(True, 19, 2000000, 0), (False, 20, 1500000, 0),
-                                                       (True, 19, 0, 0), (True, 19, 1000000, 8),
-                                                       (False, 20, 1000000, 4)])
-def test_each_result_is_the_same_alone_and_in_the_mix(g, world, smem, k, round_bases, growth):
+@pytest.mark.parametrize("smem,k,round_bases,growth,tables", [(True, 19, 2000000, 0, False),
+                                                              (False, 20, 1500000, 0, False),
+                                                              (True, 19, 0, 0, False), (True, 19, 1000000, 8, False),
+                                                              (False, 20, 1000000, 4, False),
+                                                              (True, 19, 2000000, 4, True), (False, 20, 0, 0, True)])
+def test_each_result_is_the_same_alone_and_in_the_mix(g, world, smem, k, round_bases, growth, tables):
+    """tables: the index also holds the 16-mer table with presence bits and
+    the direct-extension structures (the served rounds use both; the plain
+    kernel the results are compared with uses the table only)."""
     reqs = make_requests(world, 40, 2000, seed=7 + k + round_bases % 7 + growth)
     with g.Index(world["prefix"]) as X:
+        if tables:
+            X.set_kmer_len(16)
+            X.set_direct(True)
         alone = [g.seed(X, arrays=r, k=k, smem=smem, kernel=0, max_occ=200) for r in reqs]
         say("alone done: %d requests, %d reads" % (len(reqs), sum(len(r[2]) for r in reqs)))
         with g.Server(X, round_bases=round_bases, round_growth=growth, k=k, smem=smem, max_occ=200) as S:

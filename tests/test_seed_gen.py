@@ -195,6 +195,8 @@ def test_cli_argument_errors(built, c1):
     ["-s", "-k", "20", "--max-occ", "5", "--device-bases", "2500000"],
     ["-s", "-k", "19", "--kmer-len", "15", "--autotune"],   # extended table, tuned launch
     ["-k", "20", "--kmer-len", "14", "--device-bases", "400000"],
+    ["-s", "-k", "19", "--kmer-len", "16", "--direct"],     # presence bits and direct runs
+    ["-k", "20", "--direct", "--kernel", "2"],
 ])
 def test_cli_text_matches_oracle(built, c1, tmp_path, flags):
     say("seed_gen", " ".join(flags))
@@ -207,7 +209,7 @@ def test_cli_text_matches_oracle(built, c1, tmp_path, flags):
     while i < len(flags):           # the oracle takes -k, -s and -m; device options are dropped
         if flags[i] in ("--device-bases", "--kernel", "--kmer-len"):
             i += 2
-        elif flags[i] == "--autotune":
+        elif flags[i] in ("--autotune", "--direct"):
             i += 1
         elif flags[i] == "--max-occ":
             oflags += ["-m", flags[i + 1]]
