@@ -102,7 +102,7 @@ struct JoinParams {
 
 constexpr uint32_t kSplitStream = 0xFFFFFFFFu;  // nF/nR of a whole-read stream left for join_whole
 constexpr uint32_t kSplitItemsPerLane = 4;      // work splitting only in batches this small
-constexpr uint32_t kRefillGroup = 4;            // idle lanes of a wave that trigger a refill
+constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that trigger a refill
                                                 // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
                                                 // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms;
                                                 // 2, 3, 4 and 6 within 1 %)
@@ -170,6 +170,7 @@ int batch_download(gs_batch* B, ResultView* v);
 // Tuning fields of an existing batch (the autotuner's knobs that need no new
 // work items): kernel variant, workgroup size, suffix-array walk rows.
 int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows);
+int gs_batch_set_direct_min(gs_batch* B, uint16_t direct_min);  // as gs_opts.direct_min
 
 // ceil(2^64 / d) for d >= 2 (exact quotient for every 32-bit numerator), 0 for d == 1
 uint64_t gs_reciprocal64(uint32_t d);

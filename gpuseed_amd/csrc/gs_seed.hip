@@ -1512,6 +1512,12 @@ int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows) {
   return GS_OK;
 }
 
+int gs_batch_set_direct_min(gs_batch* B, uint16_t direct_min) {
+  if (!B || (direct_min > 64 && direct_min != 0xFFFF)) return GS_EINVAL;
+  B->opts.direct_min = direct_min;
+  return GS_OK;
+}
+
 extern "C" GS_API int gs_batch_destroy(gs_batch* B) {
   if (!B) return GS_EINVAL;
   (void)hipSetDevice(B->X->device);

@@ -6,7 +6,9 @@
 //     only if the sample has such reads): 32, 64, 96, 112;
 //  2. kernel variant (8- or 16-bit size records, which sets the LDS slot and
 //     so the waves per CU) x workgroup size 64 / 128 / 256;
-//  3. suffix-array walk rows per thread: 1, 2, 4.
+//  3. suffix-array walk rows per thread: 1, 2, 4;
+//  4. with the index's direct-extension structures: single-row steps before a
+//     task reads the text, 4 (the default), 2, 8, or never.
 // Every candidate must produce the same number of seeds and positions as the
 // first run; the output of the pipeline depends on none of the knobs.
 #include <string.h>
@@ -124,6 +126,23 @@ extern "C" GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64
     }
     if (rc) break;
     cur.walk_rows = (uint16_t)br;
+    // 4. direct extension
+    if (idx->sa_full) {
+      static const uint16_t dmins[] = {4, 2, 8, 0xFFFF};
+      uint16_t bd = cur.direct_min ? cur.direct_min : 4;
+      for (uint16_t dm : dmins) {
+        if (dm == bd) continue;
+        if ((rc = gs_batch_set_direct_min(B, dm))) break;
+        double t = 0;
+        if ((rc = T.time(&t))) break;
+        if (t < best) {
+          best = t;
+          bd = dm;
+        }
+      }
+      if (rc) break;
+      cur.direct_min = bd;
+    }
   } while (0);
   gs_batch_destroy(B);
   if (rc) return rc;

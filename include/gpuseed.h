@@ -327,7 +327,7 @@ GS_API int gs_batch_replay(gs_batch* batch, gs_batch_stats* stats);
  * give the same seed and position counts (the output does not depend on the
  * tuning fields).  No counterpart in the reference. */
 typedef struct {
-  gs_opts opts;           /* base with kernel, seg_tasks, wg, walk_rows chosen */
+  gs_opts opts;           /* base with kernel, seg_tasks, wg, walk_rows, direct_min chosen */
   double seconds;         /* pipeline time with opts                           */
   double default_seconds; /* pipeline time with base's fields (0 -> defaults)  */
   int n_tried;

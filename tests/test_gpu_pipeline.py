@@ -134,6 +134,22 @@ def test_autotune_picks_valid_options(g, case):
         assert same(tuned, plain)
 
 
+def test_autotune_with_direct_extension(g, case):
+    """With the direct-extension structures the tuner also picks direct_min
+    (4, 2, 8 or never); the tuned options give the plain kernel's output."""
+    with g.Index(case["prefix"]) as X:
+        X.set_kmer_len(16)
+        X.set_direct(True)
+        opts, info = g.autotune(X, case["arrays"], reps=2, k=19, smem=True, max_occ=100)
+        say("tuned kernel=%d wg=%d walk_rows=%d direct_min=%d: %d tried" %
+            (opts.kernel, opts.wg, opts.walk_rows, opts.direct_min, info.n_tried))
+        assert opts.direct_min in (2, 4, 8, g.NO_DIRECT)
+        assert info.n_tried >= 12
+        tuned = g.seed(X, arrays=case["arrays"], opts=opts)
+        plain = g.seed(X, arrays=case["arrays"], k=19, smem=True, max_occ=100, kernel=0)
+        assert same(tuned, plain)
+
+
 def test_large_outputs_download_in_chunks(g, tmp_path):
     """Positions of more than 64 MB go through the staged device-to-host copy
     in several pinned chunks (gs_seed, gs_batch_fetch); they equal what a
