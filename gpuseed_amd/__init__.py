@@ -141,6 +141,7 @@ def lib():
         L.gs_index_read_kext.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_set_kmer_len.argtypes = [C.c_void_p, C.c_int]
         L.gs_index_set_direct.argtypes = [C.c_void_p, C.c_int]
+        L.gs_index_locate.argtypes = [C.c_void_p, u32p, C.c_uint64, u32p]
         L.gs_index_read_direct.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p,
                                            C.POINTER(C.c_uint8)]
         L.gs_seed.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint64, C.POINTER(Opts),
@@ -339,6 +340,14 @@ class Index:
         _check("gs_index_read_direct", lib().gs_index_read_direct(self.h, first, count, _ptr(sa, C.c_uint32),
                                                                   _ptr(isa, C.c_uint32), _ptr(text, C.c_uint8)))
         return sa, isa, text
+
+    def locate(self, rows):
+        """Text positions of suffix-array rows as the reference's walk reports them."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint32)
+        out = np.empty(len(rows), dtype=np.uint32)
+        _check("gs_index_locate", lib().gs_index_locate(self.h, _ptr(rows, C.c_uint32), len(rows),
+                                                        _ptr(out, C.c_uint32)))
+        return out
 
     def read_kmer(self, first, count):
         out = np.empty((count, 2), dtype=np.uint32)

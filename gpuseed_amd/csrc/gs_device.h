@@ -18,6 +18,7 @@ namespace gs {
 
 constexpr uint32_t kBlockBases = 96;
 constexpr int kPrecalc = 13;  // pre_calc_seed_len (seed_gen.cu:2822)
+constexpr uint32_t kMaxQuirk = 64;  // rows of the primary+1 walk kept for locating from the full suffix array
 
 // floor(i / 96) for any 32-bit i: (i >> 5) / 3 via the exact multiply-high
 // reciprocal of 3.
@@ -48,6 +49,11 @@ struct DevIndex {
   const uint32_t* sa_full;
   const uint32_t* isa;
   const uint4* text;
+  // with sa_full: the rows whose reference walk passes row primary+1, each
+  // with the position that walk reports (at most kMaxQuirk; none if the
+  // list did not fit, and then positions come from the walk)
+  const uint2* quirk;
+  uint32_t n_quirk, locate_direct;
   uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)
   uint32_t L2[5];

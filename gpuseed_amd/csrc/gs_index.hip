@@ -36,6 +36,9 @@ DevIndex gs_index::dev() const {
   d.sa_full = sa_full;
   d.isa = isa;
   d.text = reinterpret_cast<const uint4*>(text);
+  d.quirk = quirk;
+  d.n_quirk = n_quirk;
+  d.locate_direct = sa_full && n_quirk <= gs::kMaxQuirk ? 1u : 0u;
   d.primary = primary;
   d.seq_len = seq_len;
   d.sa_intv = sa_intv;
@@ -324,8 +327,42 @@ static void free_direct(gs_index* X) {
   (void)hipFree(X->sa_full);
   (void)hipFree(X->isa);
   (void)hipFree(X->text);
+  (void)hipFree(X->quirk);
   X->sa_full = X->isa = X->text = nullptr;
+  X->quirk = nullptr;
+  X->n_quirk = 0;
   X->text_words = 0;
+}
+
+// The reference's walk (bwt_inv_psi_gpu, seed_gen.cu:455-484) sends row
+// primary+1 to row 0 instead of to its true LF row, so a row whose walk
+// reaches primary+1 before any sampled row reports, instead of its text
+// position, the number of steps it took to get there (sa[0] = 0xFFFFFFFF
+// plus the steps, seed_gen.cu:668-684).  Those are the rows of text positions
+// P, P+1, ... (P = the position of row primary+1) up to the first sampled
+// row; the list lets locate_rows read every other position from the full
+// suffix array in one load.
+static int find_quirk_rows(gs_index* X) {
+  X->n_quirk = 0;
+  const uint32_t q = X->primary + 1;
+  if (q > X->seq_len || q % X->sa_intv == 0) return GS_OK;  // no such row, or sampled: no detour
+  uint32_t P = 0;
+  GS_HIP(hipMemcpy(&P, X->sa_full + q, 4, hipMemcpyDeviceToHost));
+  const uint64_t avail = (uint64_t)X->seq_len + 1 - P;
+  const uint32_t m = (uint32_t)(avail < gs::kMaxQuirk + 1 ? avail : gs::kMaxQuirk + 1);
+  std::vector<uint32_t> rows(m);
+  GS_HIP(hipMemcpy(rows.data(), X->isa + P, (size_t)m * 4, hipMemcpyDeviceToHost));
+  std::vector<uint2> list;
+  for (uint32_t j = 0; j < m && rows[j] % X->sa_intv != 0; ++j) list.push_back(make_uint2(rows[j], j));
+  if (list.size() > gs::kMaxQuirk) {
+    X->n_quirk = gs::kMaxQuirk + 1;  // does not fit: locate walks
+    return GS_OK;
+  }
+  X->n_quirk = (uint32_t)list.size();
+  if (list.empty()) return GS_OK;
+  GS_HIP(hipMalloc(&X->quirk, list.size() * sizeof(uint2)));
+  GS_HIP(hipMemcpy(X->quirk, list.data(), list.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  return GS_OK;
 }
 
 // Direct extension: once a task's interval has a single row, the seeding
@@ -360,6 +397,7 @@ extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
   }
   (void)hipFree(text8);
+  if (!rc) rc = find_quirk_rows(X);
   if (rc) {
     if (!X->text_words) {  // not counted yet
       (void)hipFree(X->sa_full);

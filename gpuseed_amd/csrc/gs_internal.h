@@ -33,6 +33,10 @@ struct gs_index {
   uint32_t* isa = nullptr;
   uint32_t* text = nullptr;
   uint64_t text_words = 0;
+  // rows whose reference walk passes row primary+1 and their reported
+  // positions (locate from the full suffix array); n_quirk > kMaxQuirk: walk
+  uint2* quirk = nullptr;
+  uint32_t n_quirk = 0;
   uint64_t device_bytes = 0;
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0, t_direct = 0;
   int sa_rounds = 0;

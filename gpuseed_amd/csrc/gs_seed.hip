@@ -621,6 +621,24 @@ __global__ void __launch_bounds__(256) locate_rows(const DevIndex X, uint32_t* _
                                                    uint64_t n) {
   const uint64_t T = (uint64_t)gridDim.x * 256;
   const uint64_t t = blockIdx.x * 256ull + threadIdx.x;
+  if (X.locate_direct) {
+    // one load from the full suffix array per row; the rows whose reference
+    // walk detours through row primary+1 report what that walk reports
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint64_t i = t + j * T;
+      if (i >= n) continue;
+      const uint32_t r = rows[i];
+      uint32_t pos = 0xFFFFFFFFu;  // rows past the last one (MEM row skip) and row 0 ($) have none
+      if (r != 0 && r <= X.seq_len) {
+        pos = X.sa_full[r];
+        for (uint32_t q = 0; q < X.n_quirk; ++q)
+          if (X.quirk[q].x == r) pos = X.quirk[q].y;
+      }
+      rows[i] = pos;
+    }
+    return;
+  }
   uint32_t row[R], itr[R];
   uint32_t live = 0;
 #pragma unroll
@@ -1510,6 +1528,28 @@ int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows) {
   B->opts.wg = (uint16_t)wg;
   B->opts.walk_rows = (uint16_t)walk_rows;
   return GS_OK;
+}
+
+// Text positions of suffix-array rows as the reference's walk reports them
+// (locate_seeds_gpu, seed_gen.cu:668-684), with the same kernel as the
+// pipeline (from the full suffix array when the index has it).
+extern "C" GS_API int gs_index_locate(gs_index* X, const uint32_t* rows, uint64_t n, uint32_t* out) {
+  if (!X || (n && (!rows || !out))) return GS_EINVAL;
+  if (!n) return GS_OK;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  uint32_t* d = nullptr;
+  if (hipMalloc(&d, n * 4) != hipSuccess) return GS_ENOMEM;
+  int rc = GS_OK;
+  if (hipMemcpyAsync(d, rows, n * 4, hipMemcpyHostToDevice, X->stream) != hipSuccess) rc = GS_EHIP;
+  if (!rc) {
+    hipLaunchKernelGGL(locate_rows<1>, dim3(grid_for(n, 256)), dim3(256), 0, X->stream, X->dev(), d, n);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, X->stream) != hipSuccess ||
+        hipStreamSynchronize(X->stream) != hipSuccess)
+      rc = GS_EHIP;
+  }
+  (void)hipFree(d);
+  return rc;
 }
 
 int gs_batch_set_direct_min(gs_batch* B, uint16_t direct_min) {

@@ -198,6 +198,12 @@ GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
  * and reading the row where the match ends, in place of one backward step
  * per base.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_direct(gs_index* idx, int on);
+/* Text positions of n suffix-array rows as the reference's walk reports
+ * them, row primary+1 behaviour included (locate_seeds_gpu with
+ * bwt_inv_psi_gpu, seed_gen.cu:455-484, 668-684); rows past seq_len and row
+ * 0 give 0xFFFFFFFF.  From the full suffix array when the index has the
+ * direct-extension structures, else by the sampled-suffix-array walk. */
+GS_API int gs_index_locate(gs_index* idx, const uint32_t* rows, uint64_t n, uint32_t* out);
 /* Rows [first, first+count) of the full suffix array and positions
  * [first, first+count) of its inverse and of the text (one code 0..3 per
  * byte); any output may be NULL (tests). */

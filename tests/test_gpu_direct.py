@@ -58,6 +58,32 @@ def test_structures_of_a_gpu_built_index(g):
         assert "".join("ACGT"[c] for c in t[:len(text)]) == text
 
 
+@pytest.mark.parametrize("text,sa_intv", [("AC", 4), ("ACGTTGCA", 3), (None, 7), (None, 4), (None, 2)])
+def test_locate_every_row(g, tmp_path, text, sa_intv):
+    """Every row's reported position, by the sampled-suffix-array walk and
+    from the full suffix array, equals the oracle's (the reference's walk,
+    whose detour through row primary+1 gives some rows another value)."""
+    if text is None:
+        rng = random.Random(sa_intv)
+        text = repetitive_text(rng, 3000 + 500 * sa_intv)
+    n = len(text)
+    prefix = build_index(tmp_path, text, sa_intv)
+    ox = ob.Index(prefix)
+    bi = brute.BruteIndex(text, sa_intv)
+    rows = np.arange(n + 3, dtype=np.uint32)  # rows past seq_len have no position
+    want = [ox.locate(int(r)) for r in rows]
+    detour = sum(1 for r in range(1, n + 1) if want[r] != bi.sa[r])
+    say("n=%d sa_intv=%d: %d rows take the detour" % (n, sa_intv, detour))
+    with g.Index(prefix) as X:
+        walk = X.locate(rows)
+        X.set_direct(True)
+        direct = X.locate(rows)
+    assert walk.tolist() == want
+    assert direct.tolist() == want
+    if text == "AC":
+        assert detour == 1  # row 2 reports 0 instead of 1 (seed_gen.cu:455-457)
+
+
 def edge_reads(text):
     """Reads that match at the very start and end of the text (a run reaching
     text position 0), on both strands, with and without an N inside."""
