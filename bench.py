@@ -49,13 +49,14 @@ def cpu_model():
     return "unknown"
 
 
-def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0):
+def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0, prune_sat=0):
     """Memory operations per read of the device algorithm, counted by the
     oracle on the first `sample` reads: 13-mer table lookups (8 bytes),
     extended-table lookups (16 bytes: interval and presence bits), occ
     entries (16 bytes) and direct-extension reads (4-byte suffix-array and
     inverse words, 16-byte text chunks), with distance-1 pruning, the SMEM
-    read-start stop and tasks that cannot reach k bases ending at once."""
+    read-start stop and tasks that cannot reach k bases ending at once; size
+    records saturate at prune_sat like the kernel's (0: exact)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     buf, offs, lens = arrays
@@ -64,7 +65,8 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, 
     sub = (buf[:end], offs[:n], lens[:n])
     ox = ob.Index(prefix)
     c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
-                       ext_len=ext_len if ext_len > 13 else 0, miss_skip=1, direct_min=direct_min)[4]
+                       ext_len=ext_len if ext_len > 13 else 0, miss_skip=1, direct_min=direct_min,
+                       prune_sat=prune_sat)[4]
     ox.close()
     table = (c["w_tasks"] + c["w_ext"]) / n
     occ = c["w_blocks96"] / n
@@ -277,7 +279,8 @@ def main():
                     help="no direct-extension structures (full suffix array, inverse, text)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", type=int, default=1, help="batched kernel variant (1: 8-bit, 2: 16-bit size records)")
+    ap.add_argument("--kernel", type=int, default=3,
+                    help="batched kernel variant (1: 8-bit, 2: 16-bit, 3: 4-bit size records)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--no-graph", action="store_true", help="skip the host-to-host (HIP graph) measurement")
     ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
@@ -437,7 +440,7 @@ def main():
     wm = None
     if not args.no_cpu:
         wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem,
-                        direct_min=4 if direct else 0)
+                        direct_min=4 if direct else 0, prune_sat={1: 255, 2: 65535, 3: 15}.get(args.kernel, 0))
         log("work model: %.1f table + %.1f occ + %.1f direct requests per read (%.0f bytes)" %
             (wm["table_per_read"], wm["occ_per_read"], wm["direct_per_read"], wm["bytes_per_read"]))
     serving = None

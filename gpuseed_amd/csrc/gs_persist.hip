@@ -8,7 +8,7 @@
 //  * reads are pre-packed to 4-bit codes of both virtual strands by
 //    pack_codes, so a refill is a few 16-byte loads and LDS stores;
 //  * a lane's LDS slot holds its item's codes (4 bits per base) and its size
-//    record (one AT per position), indexed directly in LDS;
+//    record (4, 8 or 16 bits per position), indexed directly in LDS;
 //  * the loop body is one memory operation (an interval-table entry, from
 //    the extended table when the index has one, or one or two occ entries)
 //    followed by a short processing step; the rare transitions (windows with
@@ -31,9 +31,15 @@
 //    yet checks again next iteration, and a lane leaves once the main queue
 //    is exhausted, no piece is left to claim and no item is running anywhere
 //    (so no piece can appear any more).
+#include <type_traits>
+
 #include "gs_internal.h"
 
 namespace {
+
+// tag type of 4-bit size records (seed_persistent's record width is taken
+// from its first template argument: Nib, uint8_t or uint16_t)
+struct Nib {};
 
 __device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3) {
   return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
@@ -52,14 +58,37 @@ __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
 template <typename AT, int LMAX, int BS>
 __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   static_assert(LMAX % 32 == 0, "LMAX must be a multiple of 32");
+  // size records of RB bits per position: 4 (Nib, two per byte, low nibble
+  // first), 8 or 16; a record saturates at 2^RB - 1 and then never prunes
+  constexpr int RB = std::is_same<AT, Nib>::value ? 4 : 8 * (int)sizeof(AT);
   constexpr int CODE_BYTES = LMAX / 2;
-  constexpr int A_BYTES = LMAX * (int)sizeof(AT);
+  constexpr int A_BYTES = LMAX * RB / 8;
   constexpr int SLOT = CODE_BYTES + A_BYTES;
   static_assert(SLOT % 16 == 0, "slot must be 16-byte aligned");
-  constexpr uint32_t SAT = (uint32_t)(AT)(~(AT)0);
+  constexpr uint32_t SAT = (1u << RB) - 1u;
   __shared__ __align__(16) uint8_t lds[BS * SLOT];
   const uint32_t sbase = threadIdx.x * SLOT;
-  AT* const A = reinterpret_cast<AT*>(lds + sbase + CODE_BYTES);
+  // record j of the slot starting at byte `slot`
+  auto rec_get = [&](uint32_t slot, int32_t j) __attribute__((always_inline)) -> uint32_t {
+    uint8_t* a = lds + slot + CODE_BYTES;
+    if constexpr (RB == 4)
+      return (a[j >> 1] >> ((j & 1) << 2)) & 15u;
+    else if constexpr (RB == 8)
+      return a[j];
+    else
+      return reinterpret_cast<uint16_t*>(a)[j];
+  };
+  auto rec_set = [&](uint32_t slot, int32_t j, uint32_t v) __attribute__((always_inline)) {
+    uint8_t* a = lds + slot + CODE_BYTES;
+    if constexpr (RB == 4) {
+      const uint32_t sh = (j & 1) << 2;
+      a[j >> 1] = (uint8_t)((a[j >> 1] & ~(15u << sh)) | (v << sh));
+    } else if constexpr (RB == 8) {
+      a[j] = (uint8_t)v;
+    } else {
+      reinterpret_cast<uint16_t*>(a)[j] = (uint16_t)v;
+    }
+  };
   const int lane = threadIdx.x & 63;
   const uint32_t k = P.k;
   const uint32_t L2_0 = P.X.L2[0], L2_1 = P.X.L2[1], L2_2 = P.X.L2[2], L2_3 = P.X.L2[3];
@@ -173,8 +202,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const int32_t j = pos - w0;
     if (j < 0) return false;
     const uint32_t sv = size >= SAT ? SAT : size;
-    if (size < SAT && (uint32_t)A[j] == sv) return true;
-    A[j] = (AT)sv;
+    if (size < SAT && rec_get(sbase, j) == sv) return true;
+    rec_set(sbase, j, sv);
     return false;
   };
   auto finish_task = [&]() __attribute__((always_inline)) {
@@ -220,7 +249,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   auto try_direct = [&]() __attribute__((always_inline)) {
     if (P.direct_min && s == 1u && ones >= P.direct_min) {
       const int32_t j = x - w0;  // runs stay inside the LDS window
-      if (j >= 0 && (uint32_t)A[j] != 1u) phase = 3;
+      if (j >= 0 && rec_get(sbase, j) != 1u) phase = 3;
     }
   };
   // how a run ends: oc 1 the next step empties, 2 the read start, 3 an N at x;
@@ -251,7 +280,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     }
     const uint32_t c0 = (uint32_t)w0 >> 5, c1 = top >> 5;
     for (uint32_t c = c0; c < c1; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * (c - c0)) = src[c];
-    const uint32_t za = ((top - (uint32_t)w0) * (uint32_t)sizeof(AT) + 15) >> 4;
+    const uint32_t za = ((top - (uint32_t)w0) * (uint32_t)RB / 8 + 15) >> 4;
     for (uint32_t c = 0; c < za; ++c)
       *reinterpret_cast<uint4*>(lds + sbase + CODE_BYTES + 16 * c) = make_uint4(0, 0, 0, 0);
     out = P.cand + 2 * P.task_off[r];
@@ -497,7 +526,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         uint32_t cc = 0, rec = 0, tb = 0;
         if (valid) {
           cc = (lds[rs + (j >> 1)] >> ((j & 1) << 2)) & 15u;
-          rec = (uint32_t)reinterpret_cast<const AT*>(lds + rs + CODE_BYTES)[j];
+          rec = rec_get(rs, j);
           const uint32_t t = tp - (uint32_t)lane;
           tb = (pick4((t >> 4) & 3u, t0, t1, t2, t3) >> ((t & 15u) << 1)) & 3u;
         }
@@ -508,7 +537,20 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         const uint32_t fp = prn ? (uint32_t)__ffsll((unsigned long long)prn) - 1u : 64u;
         uint32_t g = fb < fp ? fb : fp;  // the first event, 64 if none
         g = g < (uint32_t)m ? g : (uint32_t)m;
-        if ((uint32_t)lane < g) reinterpret_cast<AT*>(lds + rs + CODE_BYTES)[j] = (AT)1;
+        if constexpr (RB == 4) {
+          // two positions per byte: the lane of a byte's higher position (or
+          // the first lane, at an even top position) writes the whole byte,
+          // keeping the nibble of a position outside the matched range
+          if ((uint32_t)lane < g && ((j & 1) || lane == 0)) {
+            uint8_t* a = lds + rs + CODE_BYTES + (j >> 1);
+            if ((j & 1) && (uint32_t)lane + 1 < g)
+              *a = 0x11;  // positions j and j - 1 both matched
+            else
+              *a = (uint8_t)((*a & ~(15u << ((j & 1) << 2))) | (1u << ((j & 1) << 2)));
+          }
+        } else {
+          if ((uint32_t)lane < g) rec_set(rs, j, 1u);
+        }
         int kind = 0;
         if (fp < fb)
           kind = 4;
@@ -841,6 +883,9 @@ int persistent_grid(int variant, int wg, unsigned* grid) {
     case 1064: return grid_t<uint16_t, L, 64>(grid);
     case 1128: return grid_t<uint16_t, L, 128>(grid);
     case 1256: return grid_t<uint16_t, L, 256>(grid);
+    case 2064: return grid_t<Nib, L, 64>(grid);
+    case 2128: return grid_t<Nib, L, 128>(grid);
+    case 2256: return grid_t<Nib, L, 256>(grid);
     default: return GS_EINVAL;
   }
 }
@@ -861,6 +906,9 @@ int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid
     case 1064: return launch_t<uint16_t, L, 64>(P, grid, st);
     case 1128: return launch_t<uint16_t, L, 128>(P, grid, st);
     case 1256: return launch_t<uint16_t, L, 256>(P, grid, st);
+    case 2064: return launch_t<Nib, L, 64>(P, grid, st);
+    case 2128: return launch_t<Nib, L, 128>(P, grid, st);
+    case 2256: return launch_t<Nib, L, 256>(P, grid, st);
     default: return GS_EINVAL;
   }
 }

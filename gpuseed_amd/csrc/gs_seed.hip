@@ -811,7 +811,7 @@ int gs::plan_check(const gs_opts* o) {
   if (o->seg_tasks > gs::kSegTasksMax) return GS_EINVAL;
   if (o->wg != 0 && o->wg != 64 && o->wg != 128 && o->wg != 256) return GS_EINVAL;
   if (o->walk_rows != 0 && o->walk_rows != 1 && o->walk_rows != 2 && o->walk_rows != 4) return GS_EINVAL;
-  if (o->kernel < 0 || o->kernel > 2) return GS_EINVAL;
+  if (o->kernel < 0 || o->kernel > 3) return GS_EINVAL;
   if (o->split_tasks > 64 && o->split_tasks != 0xFFFF) return GS_EINVAL;
   if (o->split_work == 0xFFFF) return GS_EINVAL;
   if (o->direct_min > 64 && o->direct_min != 0xFFFF) return GS_EINVAL;
@@ -974,7 +974,7 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   if (split_on && P.opts.split_tasks == 0) {
     unsigned grid = 0;
     const int wg = P.opts.wg ? P.opts.wg : 128;
-    if (gs::persistent_grid(P.opts.kernel == 2 ? 1 : 0, wg, &grid) != GS_OK || !grid ||
+    if (gs::persistent_grid(P.opts.kernel >= 1 ? P.opts.kernel - 1 : 0, wg, &grid) != GS_OK || !grid ||
         B->n_items > (uint64_t)gs::kSplitItemsPerLane * grid * (uint64_t)wg)
       split_on = false;
   }
@@ -1147,7 +1147,7 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   J.pad = 0;
   int rc = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
                                  B->codes.as<uint4>(), st);
-  if (!rc) rc = gs::launch_persistent(PP, kernel == 2 ? 1 : 0, B->opts.wg, pgrid, st);
+  if (!rc) rc = gs::launch_persistent(PP, kernel - 1, B->opts.wg, pgrid, st);
   if (!rc) rc = gs::launch_join(J, st);
   return rc;
 }
@@ -1190,7 +1190,7 @@ static int enqueue_walk(gs_batch* B, const SeedParams& S, const PostParams& Q, h
 static int persistent_grid_for(gs_batch* B, int kernel, unsigned* grid) {
   *grid = 0;
   if (kernel == 0 || !B->n_reads) return GS_OK;
-  return gs::persistent_grid(kernel == 2 ? 1 : 0, B->opts.wg, grid);
+  return gs::persistent_grid(kernel - 1, B->opts.wg, grid);
 }
 
 // Runs the whole device pipeline on an uploaded batch; results stay in HBM.
@@ -1200,7 +1200,7 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   hipStream_t st = B->stream;
   const int kernel = kernel_override >= 0 ? kernel_override : B->opts.kernel;
-  if (kernel < 0 || kernel > 2) return GS_EINVAL;
+  if (kernel < 0 || kernel > 3) return GS_EINVAL;
   unsigned pgrid = 0;
   int rc = persistent_grid_for(B, kernel, &pgrid);
   if (rc) return rc;
@@ -1521,7 +1521,7 @@ int gs::batch_download(gs_batch* B, ResultView* v) {
 }
 
 int gs_batch_set_tuning(gs_batch* B, int kernel, int wg, int walk_rows) {
-  if (!B || kernel < 0 || kernel > 2 || (wg != 64 && wg != 128 && wg != 256) ||
+  if (!B || kernel < 0 || kernel > 3 || (wg != 64 && wg != 128 && wg != 256) ||
       (walk_rows != 1 && walk_rows != 2 && walk_rows != 4))
     return GS_EINVAL;
   B->opts.kernel = kernel;

@@ -351,7 +351,7 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   S->o = *o;
   if (!S->o.budget_bytes) S->o.budget_bytes = 64ull << 30;
   if (!S->o.round_bases) S->o.round_bases = 16000000ull;
-  if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 1;
+  if (S->o.opts.kernel == 0 && !S->o.exclusive) S->o.opts.kernel = 3;
   // rounds hold few work items per lane, so shorter segments of long reads
   // balance better (peak with 64: 1.31 M reads/s, 48: 1.42 M, 32: 1.48 M)
   if (!S->o.opts.seg_tasks) S->o.opts.seg_tasks = 32;

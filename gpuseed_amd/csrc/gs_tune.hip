@@ -4,7 +4,7 @@
 // Knobs, searched one after another (each step keeps the best so far):
 //  1. seg_tasks (task ends per work item of reads longer than the LDS window;
 //     only if the sample has such reads): 32, 64, 96, 112;
-//  2. kernel variant (8- or 16-bit size records, which sets the LDS slot and
+//  2. kernel variant (4-, 8- or 16-bit size records, which sets the LDS slot and
 //     so the waves per CU) x workgroup size 64 / 128 / 256;
 //  3. suffix-array walk rows per thread: 1, 2, 4;
 //  4. with the index's direct-extension structures: single-row steps before a
@@ -51,7 +51,7 @@ extern "C" GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64
   if (!idx || !base || !out || !n_reads || !bases || !offsets || !lens) return GS_EINVAL;
   memset(out, 0, sizeof(*out));
   gs_opts cur = *base;
-  if (cur.kernel == 0) cur.kernel = 1;
+  if (cur.kernel == 0) cur.kernel = 3;
   if (!cur.wg) cur.wg = 128;
   if (!cur.walk_rows) cur.walk_rows = 1;
   if (!cur.seg_tasks) cur.seg_tasks = 64;
@@ -96,7 +96,7 @@ extern "C" GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64
     // 2. size-record width x workgroup size
     static const int wgs[] = {64, 128, 256};
     int bk = cur.kernel, bw = cur.wg;
-    for (int kv = 1; kv <= 2 && !rc; ++kv)
+    for (int kv = 1; kv <= 3 && !rc; ++kv)
       for (int wg : wgs) {
         if (kv == cur.kernel && wg == cur.wg) continue;
         if ((rc = gs_batch_set_tuning(B, kv, wg, cur.walk_rows))) break;

@@ -12,7 +12,8 @@
  *
  * Long options (no counterpart in the reference):
  *   --kernel N         0 plain per-read kernel, 1 batched persistent kernel
- *                      (default), 2 batched with 16-bit size records
+ *                      with 8-bit size records, 2 with 16-bit ones, 3 (default)
+ *                      with 4-bit ones (more waves per CU)
  *   --max-occ N        seeds with more than N occurrences get no positions
  *                      (default 0: every occurrence, as the reference)
  *   --device-bases N   bases uploaded per device batch (default 256,000,000);
@@ -361,7 +362,7 @@ int main(int argc, char** argv) {
   gs_opts o;
   memset(&o, 0, sizeof o);
   o.k = 20;
-  o.kernel = 1;
+  o.kernel = 3;
   int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, direct = 0, c, li;
   uint64_t device_bases = 256000000ull;
   static const struct option longopts[] = {{"kernel", required_argument, 0, 1},
@@ -399,8 +400,8 @@ int main(int argc, char** argv) {
     fprintf(stderr, "seed_gen: -k must be at least 14 (seeds start from a 13-base table lookup)\n");
     return 1;
   }
-  if (o.kernel < 0 || o.kernel > 2) {
-    fprintf(stderr, "seed_gen: --kernel must be 0, 1 or 2\n");
+  if (o.kernel < 0 || o.kernel > 3) {
+    fprintf(stderr, "seed_gen: --kernel must be 0, 1, 2 or 3\n");
     return 1;
   }
   if (kmer_len < 13 || kmer_len > 16) {

@@ -238,7 +238,9 @@ typedef struct {
                        many bases (seed_gen.cu:2814, 2955); 0 = 1,000,000     */
   int kernel;       /* 0: plain kernel, one stream per lane (the drop-in);
                        1: batched persistent kernel, 8-bit size records;
-                       2: batched persistent kernel, 16-bit size records      */
+                       2: batched persistent kernel, 16-bit size records;
+                       3: batched persistent kernel, 4-bit size records (a
+                          smaller LDS slot, so more waves per CU)             */
   uint32_t seg_tasks; /* batched kernels: reads longer than 160 bases are cut
                        into work items of this many task ends, 1..112
                        (0 = 64)                                               */
@@ -359,7 +361,7 @@ GS_API int gs_autotune(gs_index* idx, const char* bases, const uint64_t* offsets
 typedef struct gs_server gs_server;
 
 typedef struct {
-  gs_opts opts;          /* seeding options of every request; kernel 0 means 1
+  gs_opts opts;          /* seeding options of every request; kernel 0 means 3
                             unless exclusive, seg_tasks 0 means 32            */
   uint64_t budget_bytes; /* device memory of the rounds in flight (0 = 64 GiB) */
   uint64_t round_bases;  /* bases per round at most (0 = 16,000,000)         */

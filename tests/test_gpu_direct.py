@@ -110,7 +110,7 @@ def test_seeds_unchanged(g, tmp_path, smem, k):
     with g.Index(prefix) as X:
         X.set_kmer_len(16)
         X.set_direct(True)
-        for kernel in (1, 2):
+        for kernel in (1, 2, 3):
             for dmin in (1, 3, 8):
                 say("smem=%d k=%d kernel=%d direct_min=%d" % (smem, k, kernel, dmin))
                 got = g.seed(X, reads, k=k, smem=smem, kernel=kernel, direct_min=dmin).as_lists()
@@ -124,7 +124,7 @@ def test_seeds_unchanged(g, tmp_path, smem, k):
 
 
 @pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
-@pytest.mark.parametrize("kernel,sat", [(1, 255), (2, 65535)])
+@pytest.mark.parametrize("kernel,sat", [(1, 255), (2, 65535), (3, 15)])
 def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, smem, k, kernel, sat):
     """Reads held whole in the kernel's LDS window, no work splitting:
     table lookups, occ entries and direct reads (suffix array, text chunks,

@@ -91,7 +91,7 @@ def test_seeds_unchanged_by_the_extended_table(g, tmp_path, length):
 
 @pytest.mark.parametrize("length", [13, 16])
 @pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
-@pytest.mark.parametrize("kernel,sat", [(1, 255), (2, 65535)])
+@pytest.mark.parametrize("kernel,sat", [(1, 255), (2, 65535), (3, 15)])
 def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k, kernel, sat):
     """The batched kernel's own memory-operation counters equal the oracle's
     count of the same algorithm (distance-1 pruning with size records that

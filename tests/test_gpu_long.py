@@ -71,7 +71,7 @@ def test_k_above_the_window(g, long_case, smem):
     ox.close()
     want = [[(x, y, l, u, list(p)) for (x, y, l, u, p) in r] for r in want]
     with g.Index(long_case["prefix"]) as X:
-        for kernel in (0, 1):
+        for kernel in (0, 1, 3):
             got = g.seed(X, reads, k=k, smem=smem, kernel=kernel).as_lists()
             bad = [i for i in range(len(want)) if got[i] != want[i]]
             assert not bad, "kernel %d read %d (len %d)" % (kernel, bad[0], len(reads[bad[0]]))

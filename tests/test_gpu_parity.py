@@ -88,7 +88,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("ci", range(len(CASES)))
 @pytest.mark.parametrize("smem", [True, False])
 def test_tiny_cases_match_oracle_and_brute_force(g, tmp_path, ci, smem, kernel):
@@ -111,7 +111,7 @@ def test_tiny_cases_match_oracle_and_brute_force(g, tmp_path, ci, smem, kernel):
         assert a == b, "read %d %r" % (i, reads[i])
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1, 3])
 def test_mem_batch_lookahead_on_device(g, tmp_path, kernel):
     rng = random.Random(99)
     text = random_text(rng, 3000)
@@ -150,7 +150,7 @@ def config1(g, tmp_path_factory):
     return dict(prefix=prefix, reads=reads, arrays=(buf, offs, lens), dir=str(d))
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1, 3])
 @pytest.mark.parametrize("smem,k", [(False, 20), (True, 20), (True, 19)])
 def test_config1_records_and_text_match_oracle(g, config1, smem, k, kernel):
     """Configuration 1 (1 Mbp reference, 10,000 x 100 bp reads), all records

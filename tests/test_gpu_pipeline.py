@@ -60,7 +60,7 @@ def test_tuning_fields_do_not_change_output(g, case, smem, k):
         assert ok_ref, "plain kernel differs from the oracle"
         bad = []
         for seg_tasks in (64, 48):
-            for kernel in (1, 2):
+            for kernel in (1, 2, 3):
                 for wg in (128, 64, 256):
                     for walk_rows in (1, 2, 4):
                         out = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=kernel, max_occ=50, wg=wg,
@@ -124,7 +124,7 @@ def test_autotune_picks_valid_options(g, case):
         say("tuned kernel=%d wg=%d walk_rows=%d seg_tasks=%d: %.3f ms (default %.3f ms, %d tried)" %
             (opts.kernel, opts.wg, opts.walk_rows, opts.seg_tasks, info.seconds * 1e3,
              info.default_seconds * 1e3, info.n_tried))
-        assert opts.kernel in (1, 2) and opts.wg in (64, 128, 256) and opts.walk_rows in (1, 2, 4)
+        assert opts.kernel in (1, 2, 3) and opts.wg in (64, 128, 256) and opts.walk_rows in (1, 2, 4)
         assert opts.seg_tasks in (32, 64, 96, 112)
         assert opts.k == 19 and opts.smem == 1 and opts.max_occ == 100
         # the two times come from separate measurements: allow timing noise
