@@ -88,7 +88,8 @@ def test_work_splitting_does_not_change_output(g, case, smem, k):
 
 def test_bad_tuning_fields_are_refused(g, case):
     with g.Index(case["prefix"]) as X:
-        for kw in (dict(wg=96), dict(walk_rows=3), dict(kernel=3), dict(split_tasks=65), dict(split_work=0xFFFF)):
+        for kw in (dict(wg=96), dict(walk_rows=3), dict(kernel=4), dict(split_tasks=65), dict(split_work=0xFFFF),
+                   dict(direct_min=65)):
             with pytest.raises(g.GsError):
                 g.seed(X, arrays=case["arrays"], k=19, smem=True, **kw)
 
