@@ -1285,7 +1285,7 @@ static void free_host_results(gs_batch* B) {
 extern "C" GS_API int gs_batch_capture(gs_batch* B, int kernel) {
   if (!B || !B->ran) return GS_EINVAL;
   const int kern = kernel >= 0 ? kernel : B->opts.kernel;
-  if (kern < 0 || kern > 2) return GS_EINVAL;
+  if (kern < 0 || kern > 3) return GS_EINVAL;
   gs_index* X = B->X;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   hipStream_t st = B->stream;

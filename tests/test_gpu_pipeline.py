@@ -94,11 +94,15 @@ def test_bad_tuning_fields_are_refused(g, case):
                 g.seed(X, arrays=case["arrays"], k=19, smem=True, **kw)
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2])
-def test_graph_replays_equal_eager_runs(g, case, kernel):
+@pytest.mark.parametrize("kernel,tables", [(0, False), (1, False), (2, False), (3, False), (3, True)])
+def test_graph_replays_equal_eager_runs(g, case, kernel, tables):
     """Every replay, with eager work and synchronisations between replays,
-    gives the eager result."""
+    gives the eager result (tables: the index also has the 16-mer table with
+    presence bits and the direct-extension structures)."""
     with g.Index(case["prefix"]) as X:
+        if tables:
+            X.set_kmer_len(16)
+            X.set_direct(True)
         other = g.Batch(X, case["arrays"], k=20, smem=False, kernel=1)
         b = g.Batch(X, case["arrays"], k=19, smem=True, kernel=kernel, max_occ=100)
         b.run()
