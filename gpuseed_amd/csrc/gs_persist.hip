@@ -46,6 +46,7 @@ __device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, 
 }
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kBad13 = (1u << gs::kPrecalc) - 1u;  // the 13-mer window's codes in a 19-code window
 
 __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -101,7 +102,13 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
 
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
-  uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0, p = 0, nbad = 0;
+  uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0;
+  // the 19 codes left of the task end e: code(e-1-j) at bits 2j of w19 for
+  // j = 0..18, and bit j of bad19 set if that code is above 3 or its position
+  // is left of the read start; the 13-mer pattern is the low 26 bits, an
+  // extended window and its 3 presence bases the bits above
+  uint64_t w19 = 0;
+  uint32_t bad19 = 0;
   uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
   int32_t e = 0, x = 0, e_min = 0, w0 = 0;
   uint32_t t0 = 0, first_b = 0, flags = 0, work = 0;
@@ -151,9 +158,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   };
   auto advance = [&]() __attribute__((always_inline)) -> bool {
     if (e - 1 < e_min) return false;
-    const uint32_t cn = code(e - 14), co = code(e - 1);
-    nbad = nbad + (cn > 3 ? 1u : 0u) - (co > 3 ? 1u : 0u);
-    p = (p >> 2) | ((cn & 3u) << 24);
+    const int32_t in = e - 20;  // enters the window as position e' - 19 of the next end e' = e - 1
+    const uint32_t cn = in >= 0 ? code(in) : 15u;
+    w19 = (w19 >> 2) | ((uint64_t)(cn & 3u) << 36);
+    bad19 = (bad19 >> 1) | ((cn > 3 ? 1u : 0u) << 18);
     --e;
     return true;
   };
@@ -194,7 +202,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         end_stream();
         return;
       }
-    } while (nbad != 0);
+    } while (bad19 & kBad13);
   };
   // size record at pos (positions left of the window keep none: pruning is
   // only an early exit, so a missing record never changes the output)
@@ -292,19 +300,20 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     link = link0;  // set before the item can end (a piece whose windows all hold an N ends here)
     e = (int32_t)(L - t0);
     e_min = e - (int32_t)tcount + 1;
-    p = 0;
-    nbad = 0;
-    for (int j = 0; j < gs::kPrecalc; ++j) {
-      const uint32_t c = code(e - 1 - j);
-      nbad += c > 3 ? 1u : 0u;
-      p |= (c & 3u) << (2 * j);
+    w19 = 0;
+    bad19 = 0;
+    for (int j = 0; j < 19; ++j) {
+      const int32_t pos = e - 1 - j;
+      const uint32_t c = pos >= 0 ? code(pos) : 15u;
+      bad19 |= (c > 3 ? 1u : 0u) << j;
+      w19 |= (uint64_t)(c & 3u) << (2 * j);
     }
     have = true;
     ++started;
     phase = 0;
     ext_miss = false;
     ones = 0;
-    if (nbad != 0) next_task();  // no work done yet, so this never splits
+    if (bad19 & kBad13) next_task();  // no work done yet, so this never splits
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
     item = it;
@@ -440,27 +449,18 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const uint4* a2 = nullptr;
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
-    uint32_t tlen = gs::kPrecalc, tidx = p;  // table lookup: pattern length and entry
+    uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)w19 & 0x3FFFFFFu;  // table lookup: pattern length and entry
     uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
     if (op == 1) {
-      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (p >> 1);  // two 8-byte entries per load
-      if (P.X.ext_len && !ext_miss && e >= (int32_t)P.X.ext_len) {
-        uint32_t q = p, ok = 1;
-        for (uint32_t j = gs::kPrecalc; j < P.X.ext_len; ++j) {
-          const uint32_t c = code(e - 1 - (int32_t)j);
-          ok &= c <= 3 ? 1u : 0u;
-          q |= (c & 3u) << (2 * j);
-        }
-        if (ok) {
-          tlen = P.X.ext_len;
-          tidx = q;
-          a1 = P.X.kext + q;
-          if (k >= tlen + 3 && e >= (int32_t)tlen + 3) {
-            const uint32_t c1 = code(e - (int32_t)tlen - 1), c2 = code(e - (int32_t)tlen - 2),
-                           c3 = code(e - (int32_t)tlen - 3);
-            if ((c1 | c2 | c3) <= 3) pbit = c1 + 4 * c2 + 16 * c3;
-          }
-        }
+      a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (tidx >> 1);  // two 8-byte entries per load
+      const uint32_t el = P.X.ext_len;
+      // the extended window: codes 13 .. el-1 clean and inside the read
+      if (el && !ext_miss && !(bad19 & (((1u << el) - 1u) & ~kBad13))) {
+        tlen = el;
+        tidx = (uint32_t)(w19 & ((1ull << (2 * el)) - 1ull));
+        a1 = P.X.kext + tidx;
+        // presence bit c1 + 4*c2 + 16*c3 of the next three bases, if clean
+        if (k >= tlen + 3 && !(bad19 & (7u << el))) pbit = (uint32_t)(w19 >> (2 * el)) & 63u;
       }
       ++n_table;
       ++work;
