@@ -45,11 +45,17 @@ def test_open_reads_the_same_index(both):
         assert np.array_equal(A.read_occ(0, n), B.read_occ(0, n))
 
 
-@pytest.mark.parametrize("smem,k,kernel", [(True, 19, 1), (False, 20, 1), (True, 19, 0)])
-def test_seeds_equal_fork_files_and_oracle(both, smem, k, kernel):
+@pytest.mark.parametrize("smem,k,kernel,tables", [(True, 19, 1, False), (False, 20, 1, False), (True, 19, 0, False),
+                                                  (True, 19, 3, True), (False, 20, 3, True)])
+def test_seeds_equal_fork_files_and_oracle(both, smem, k, kernel, tables):
+    """tables: the upstream-layout index also gets the 16-mer table with
+    presence bits and the direct-extension structures, derived from it."""
     import gpuseed_amd as g
-    say("bwa 0.7 index smem=%d k=%d kernel=%d" % (smem, k, kernel))
+    say("bwa 0.7 index smem=%d k=%d kernel=%d tables=%d" % (smem, k, kernel, tables))
     with g.Index(both["fork"]) as A, g.Index(both["b07"]) as B:
+        if tables:
+            B.set_kmer_len(16)
+            B.set_direct(True)
         a = g.seed(A, both["reads"], k=k, smem=smem, kernel=kernel).as_lists()
         b = g.seed(B, both["reads"], k=k, smem=smem, kernel=kernel).as_lists()
     assert a == b
