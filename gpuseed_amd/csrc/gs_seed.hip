@@ -824,9 +824,24 @@ void gs::plan_reset(BatchPlan& P, const gs_opts& o) {
   P.seg_tasks = o.seg_tasks ? o.seg_tasks : 64u;
 }
 
+// room for `add` more elements, growing geometrically so that many small
+// additions (a server round's chunks) stay linear
+template <typename V>
+static void grow(V& v, size_t add) {
+  const size_t need = v.size() + add;
+  if (need > v.capacity()) v.reserve(std::max(need, 2 * v.capacity()));
+}
+
 int gs::plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const uint32_t* lens, uint64_t r0,
                  uint64_t r1) {
   if (r1 <= r0) return GS_OK;
+  const size_t nr = (size_t)(r1 - r0);
+  grow(P.offs, nr);
+  grow(P.lens, nr);
+  grow(P.task_off, nr + 1);
+  grow(P.rbatch, nr);
+  grow(P.code_off, nr + 1);
+  grow(P.items, 2 * nr);  // one item per strand of every read that fits the LDS window
   const uint32_t k = (uint32_t)P.opts.k;
   const uint32_t batch_bases = P.opts.batch_bases ? P.opts.batch_bases : 1000000u;
   uint64_t lo = UINT64_MAX, hi = 0;
