@@ -1431,7 +1431,7 @@ static int staged_d2h(gs_batch* B, void* dst, const void* src, size_t bytes) {
     const size_t off = c * ch, len = std::min(ch, bytes - off);
     char* d = static_cast<char*>(dst) + off;
     const char* sp = stage[c & 1];
-    constexpr size_t kThreads = 4;
+    constexpr size_t kThreads = 8;
     const size_t part = (len + kThreads - 1) / kThreads;
     std::vector<std::thread> th;
     size_t done = part;  // bytes the helper threads do not cover
@@ -1485,13 +1485,12 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   }
   hipEvent_t e0 = B->ev[5], e1 = B->ev[6];
   int rc = GS_OK;
-  if (hipEventRecord(e0, st) != hipSuccess ||
-      (n && (hipMemcpyAsync(out->read_nseeds, B->ns.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-             hipMemcpyAsync(out->read_npos, B->np.p, 4 * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
-      hipMemcpyAsync(out->read_seed_off, B->soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(out->read_pos_off, B->poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    rc = GS_EHIP;
+  if (hipEventRecord(e0, st) != hipSuccess) rc = GS_EHIP;
+  // per-read arrays, then seeds and positions, each through the pinned chunks
+  if (!rc && n) rc = staged_d2h(B, out->read_nseeds, B->ns.p, 4 * n);
+  if (!rc && n) rc = staged_d2h(B, out->read_npos, B->np.p, 4 * n);
+  if (!rc) rc = staged_d2h(B, out->read_seed_off, B->soff.p, 8 * nr1);
+  if (!rc) rc = staged_d2h(B, out->read_pos_off, B->poff.p, 8 * nr1);
   if (!rc && B->n_seeds) rc = staged_d2h(B, out->seeds, B->seeds.p, sizeof(gs_seed_rec) * B->n_seeds);
   if (!rc && B->n_pos) rc = staged_d2h(B, out->pos, B->rows.p, 4 * B->n_pos);
   if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) rc = GS_EHIP;
