@@ -80,13 +80,15 @@ typedef struct {
    * 0: exact records. */
   uint32_t prune_sat;
   /* direct_min: work model of the device's direct extension.  A task whose
-   * interval has a single row, with at least direct_min positions left to
-   * its left and no size record of 1 at the next one, reads the reference
-   * text instead of stepping: one suffix-array read (the text position), one
-   * 16-byte read per 64 text bases compared, and one inverse-suffix-array read
-   * for the row where the run ends, skipped when the task emits nothing
-   * there or is pruned inside the run.  Counted in w_direct; the steps it
-   * replaces are not counted in w_blocks64/96.  0: off. */
+   * interval has kept a single row for direct_min backward steps, with no
+   * size record of 1 at the next position, reads the reference text instead
+   * of stepping: one suffix-array read (the text position), one 16-byte read
+   * per 64-base text chunk compared (w_text), and one inverse-suffix-array
+   * read for the row where the run ends, skipped when the task emits nothing
+   * there or is pruned inside the run; an N ends the run (the row is read
+   * back and backward steps go on).  Counted in w_direct; the steps it
+   * replaces are not counted in w_blocks64/96.  The model holds for reads
+   * within the device's 160-base LDS window.  0: off. */
   uint32_t direct_min;
   /* miss_skip: with ext_len <= k, a task whose clean ext_len-base window does
    * not occur ends at once instead of falling back to the 13-mer table.  Its
