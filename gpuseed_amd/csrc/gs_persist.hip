@@ -10,10 +10,15 @@
 //  * a lane's LDS slot holds its item's codes (4 bits per base) and its size
 //    record (4, 8 or 16 bits per position), indexed directly in LDS;
 //  * the loop body is one memory operation (an interval-table entry, from
-//    the extended table when the index has one, or one or two occ entries)
-//    followed by a short processing step; the rare transitions (windows with
-//    a code > 3, N-transparent positions, reaching the read start) are
-//    resolved inside the processing step;
+//    the extended table when the index has one, or one or two occ entries,
+//    or a direct-extension read: a suffix-array word, a 64-base text chunk,
+//    an inverse-suffix-array word) followed by a short processing step; the
+//    rare transitions (windows with a code > 3, N-transparent positions,
+//    reaching the read start) are resolved inside the processing step;
+//  * tasks that cannot reach k bases (absent clean 16- or 19-base window,
+//    from the extended table's interval and presence bits) end after their
+//    lookup; each task end keeps its 19 codes in a rolling register window;
+//  * text chunks are compared by the whole wave, one loading lane at a time;
 //  * work splitting: the cost of a stream is heavy-tailed (in young repeat
 //    families interval sizes shrink at every step, so distance-1 pruning
 //    never stops a task and each task walks far).  Once the main queue is
@@ -21,7 +26,8 @@
 //    split_work memory operations since it started or last split, and still
 //    has at least 2 * split_tasks task ends to do, hands half of them to an
 //    overflow queue, which idle lanes take (a piece starts without size
-//    records, so splitting while every lane is busy would only add work); the join kernels put the pieces back in task order and settle the
+//    records, so splitting while every lane is busy would only add work);
+//    the join kernels put the pieces back in task order and settle the
 //    SMEM filter at their boundaries, as for the segments of long reads.
 //    A split keeps the upper half of the remaining task ends in the lane
 //    (with its size records) and pushes the lower half as one piece, which
