@@ -167,7 +167,8 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
     """The device kernels' shortcuts (a strand stops once a task reaches the
     read start in SMEM mode; an extended interval table answers the first
     14..16 bases; with k at least the table's length, a task whose clean
-    window is absent ends at once) change only the counted work, never the
+    window is absent ends at once; direct reads of the text replace the
+    steps of single-row runs) change only the counted work, never the
     seeds."""
     import gpuseed_amd
     rp, qp = gpuseed_amd.preset(1)
@@ -180,9 +181,10 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
     for smem, k in ((True, 19), (False, 20), (True, 16), (False, 16), (True, 15)):
         base = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4)
         noskip = {}
-        for stop, ext, skip in ((1, 0, 0), (0, 16, 0), (1, 14, 0), (1, 16, 0), (0, 16, 1), (1, 14, 1), (1, 16, 1)):
+        for stop, ext, skip, dmin in ((1, 0, 0, 0), (0, 16, 0, 0), (1, 14, 0, 0), (1, 16, 0, 0), (0, 16, 1, 0),
+                                      (1, 14, 1, 0), (1, 16, 1, 0), (1, 16, 1, 4), (1, 0, 0, 1)):
             got = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=stop, ext_len=ext,
-                                 miss_skip=skip)
+                                 miss_skip=skip, direct_min=dmin)
             for a, b in zip(base[:4], got[:4]):
                 assert (a == b).all(), (smem, k, stop, ext, skip)
             cb, cg = base[4], got[4]
@@ -190,6 +192,9 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
                 assert cg["w_blocks96"] < cb["w_blocks96"]
             if ext:
                 assert cg["w_ext"] > 0 and cg["w_steps"] < cb["w_steps"]
+            if dmin:  # direct reads replace steps; the output is unchanged (checked above)
+                assert cg["w_direct"] > 0 and cg["w_text"] > 0 and cg["w_direct"] > cg["w_text"]
+                continue
             if not skip:
                 noskip[(stop, ext)] = cg
             elif k >= ext:  # absent windows end their tasks: fewer 13-mer lookups
