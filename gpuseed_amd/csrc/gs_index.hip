@@ -170,9 +170,9 @@ __global__ void kmer_presence(const DevIndex X, const uint2* __restrict__ level,
 // position and the base it precedes.  Every row is visited by exactly one
 // thread; positions fall at every step, so each walk ends.  The bounds
 // checks only guard against an inconsistent index.
-__global__ void direct_chains(const DevIndex X, uint64_t n_sa, uint32_t* __restrict__ sa_full,
+__global__ void direct_chains(const DevIndex X, uint64_t first, uint64_t n_sa, uint32_t* __restrict__ sa_full,
                               uint32_t* __restrict__ isa, uint8_t* __restrict__ text8) {
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (i >= n_sa) return;
   const uint32_t n = X.seq_len;
   uint32_t row = (uint32_t)i * X.sa_intv;
@@ -391,8 +391,12 @@ extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
     X->text_words = words;
     X->device_bytes += 2 * rows * 4 + words * 4;
     const DevIndex D = X->dev();
-    hipLaunchKernelGGL(direct_chains, dim3(grid_for(X->n_sa, 256)), dim3(256), 0, X->stream, D, X->n_sa,
-                       X->sa_full, X->isa, text8);
+    const uint64_t step = 1ull << 30;  // a dispatch's work-item count is a 32-bit quantity
+    for (uint64_t first = 0; first < X->n_sa; first += step) {
+      const uint64_t m = X->n_sa - first < step ? X->n_sa - first : step;
+      hipLaunchKernelGGL(direct_chains, dim3(grid_for(m, 256)), dim3(256), 0, X->stream, D, first, X->n_sa,
+                         X->sa_full, X->isa, text8);
+    }
     hipLaunchKernelGGL(pack_text, dim3(grid_for(words, 256)), dim3(256), 0, X->stream, text8, n, X->text, words);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
   }
