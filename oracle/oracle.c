@@ -605,7 +605,7 @@ static void emit_seed(rout_t* ro, uint32_t* cap_seeds, uint32_t* cap_rows, or_se
     while (ro->nrows + nrows > *cap_rows) *cap_rows = *cap_rows ? 2 * *cap_rows : 64;
     ro->rows = (uint32_t*)realloc(ro->rows, sizeof(uint32_t) * *cap_rows);
   }
-  memcpy(ro->rows + ro->nrows, rows, sizeof(uint32_t) * nrows);
+  if (nrows) memcpy(ro->rows + ro->nrows, rows, sizeof(uint32_t) * nrows);
   ro->nrows += nrows;
 }
 
@@ -799,8 +799,8 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
       while (res->n_pos + ro->nrows > cap_pos) cap_pos *= 2;
       res->seeds = (or_seed*)realloc(res->seeds, sizeof(or_seed) * cap_seeds);
       res->pos = (uint32_t*)realloc(res->pos, sizeof(uint32_t) * cap_pos);
-      memcpy(res->seeds + res->n_seeds, ro->seeds, sizeof(or_seed) * ro->nseeds);
-      memcpy(res->pos + res->n_pos, ro->rows, sizeof(uint32_t) * ro->nrows);
+      if (ro->nseeds) memcpy(res->seeds + res->n_seeds, ro->seeds, sizeof(or_seed) * ro->nseeds);
+      if (ro->nrows) memcpy(res->pos + res->n_pos, ro->rows, sizeof(uint32_t) * ro->nrows);
       res->n_seeds += ro->nseeds;
       res->n_pos += ro->nrows;
       free(ro->seeds);

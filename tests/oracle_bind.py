@@ -2,7 +2,8 @@
 
 The oracle is test infrastructure: the checker for the HIP path and the CPU
 baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use
-it.
+it.  GS_ORACLE_LIB names another build of the same library, e.g. the one
+`make -C oracle sanitize` writes with AddressSanitizer and UBSan.
 """
 import ctypes as C
 import os
@@ -10,7 +11,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+LIB_PATH = os.environ.get("GS_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 
 class OrOpts(C.Structure):
