@@ -1188,17 +1188,24 @@ static int enqueue_write(gs_batch* B, const PostParams& Q, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
-static int enqueue_walk(gs_batch* B, const SeedParams& S, const PostParams& Q, hipStream_t st) {
-  if (B->n_pos) {
-    const unsigned R = B->opts.walk_rows ? B->opts.walk_rows : 1u;
-    const unsigned grid = grid_for((B->n_pos + R - 1) / R, 256);
+// locate_rows over rows[0, n), R rows per thread, in launches of at most
+// 2^30 rows (a dispatch's work-item count is a 32-bit quantity).
+static void launch_locate(const DevIndex& X, uint32_t* rows, uint64_t n, unsigned R, hipStream_t st) {
+  const uint64_t step = 1ull << 30;
+  for (uint64_t first = 0; first < n; first += step) {
+    const uint64_t m = n - first < step ? n - first : step;
+    const unsigned grid = grid_for((m + R - 1) / R, 256);
     if (R == 4)
-      hipLaunchKernelGGL(locate_rows<4>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+      hipLaunchKernelGGL(locate_rows<4>, dim3(grid), dim3(256), 0, st, X, rows + first, m);
     else if (R == 2)
-      hipLaunchKernelGGL(locate_rows<2>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+      hipLaunchKernelGGL(locate_rows<2>, dim3(grid), dim3(256), 0, st, X, rows + first, m);
     else
-      hipLaunchKernelGGL(locate_rows<1>, dim3(grid), dim3(256), 0, st, S.X, Q.rows, B->n_pos);
+      hipLaunchKernelGGL(locate_rows<1>, dim3(grid), dim3(256), 0, st, X, rows + first, m);
   }
+}
+
+static int enqueue_walk(gs_batch* B, const SeedParams& S, const PostParams& Q, hipStream_t st) {
+  if (B->n_pos) launch_locate(S.X, Q.rows, B->n_pos, B->opts.walk_rows ? B->opts.walk_rows : 1u, st);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
@@ -1556,7 +1563,7 @@ extern "C" GS_API int gs_index_locate(gs_index* X, const uint32_t* rows, uint64_
   int rc = GS_OK;
   if (hipMemcpyAsync(d, rows, n * 4, hipMemcpyHostToDevice, X->stream) != hipSuccess) rc = GS_EHIP;
   if (!rc) {
-    hipLaunchKernelGGL(locate_rows<1>, dim3(grid_for(n, 256)), dim3(256), 0, X->stream, X->dev(), d, n);
+    launch_locate(X->dev(), d, n, 1, X->stream);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, X->stream) != hipSuccess ||
         hipStreamSynchronize(X->stream) != hipSuccess)
