@@ -575,21 +575,26 @@ extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) 
 // Test access to the device structures: copies occ entries [first, first+count)
 // of block-major order and table entries to host buffers.
 extern "C" GS_API int gs_index_read_occ(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out4) {
-  if (!X || !out4 || first + count > (X->n_blocks + 1) * 4) return GS_EINVAL;
+  if (!X || !out4) return GS_EINVAL;
+  const uint64_t entries = ((uint64_t)X->n_blocks + 1) * 4;
+  if (first > entries || count > entries - first) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   GS_HIP(hipMemcpy(out4, X->occ + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
   return GS_OK;
 }
 
 extern "C" GS_API int gs_index_read_kext(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out4) {
-  if (!X || !out4 || !X->kext || first + count > (1ull << (2 * X->ext_len))) return GS_EINVAL;
+  if (!X || !out4 || !X->kext) return GS_EINVAL;
+  const uint64_t entries = 1ull << (2 * X->ext_len);
+  if (first > entries || count > entries - first) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   GS_HIP(hipMemcpy(out4, X->kext + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
   return GS_OK;
 }
 
 extern "C" GS_API int gs_index_read_kmer(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out2) {
-  if (!X || !out2 || first + count > (1ull << (2 * gs::kPrecalc))) return GS_EINVAL;
+  const uint64_t entries = 1ull << (2 * gs::kPrecalc);
+  if (!X || !out2 || first > entries || count > entries - first) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   GS_HIP(hipMemcpy(out2, X->kmer + first, count * sizeof(uint2), hipMemcpyDeviceToHost));
   return GS_OK;
