@@ -93,3 +93,39 @@ def test_seed_rejects_bad_options(built):
     rc = g.lib().gs_seed(None, C.cast(buf.ctypes.data, C.c_char_p), None, None, 0, C.byref(o),
                          C.byref(res))
     assert rc == g.GS_EINVAL
+
+
+def test_null_handles_are_rejected(built):
+    """Every entry point that takes an index, batch or server handle returns
+    GS_EINVAL for a null one (checked before any device call)."""
+    import gpuseed_amd as g
+    L = C.CDLL(os.path.join(ROOT, "gpuseed_amd", "libgpuseed.so"))
+    vp, u32, u64, i = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    scratch = (C.c_uint8 * 4096)()
+    p = C.cast(scratch, vp)
+    calls = {
+        "gs_index_close": ([vp], [None]),
+        "gs_index_get_info": ([vp, vp], [None, p]),
+        "gs_index_read_occ": ([vp, u64, u64, vp], [None, 0, 1, p]),
+        "gs_index_read_kmer": ([vp, u64, u64, vp], [None, 0, 1, p]),
+        "gs_index_read_kext": ([vp, u64, u64, vp], [None, 0, 1, p]),
+        "gs_index_set_kmer_len": ([vp, i], [None, 14]),
+        "gs_index_set_direct": ([vp, i], [None, 1]),
+        "gs_index_locate": ([vp, vp, u64, vp], [None, p, 1, p]),
+        "gs_index_read_direct": ([vp, u32, u32, vp, vp, vp], [None, 0, 1, p, p, p]),
+        "gs_batch_create": ([vp, vp, vp, vp, u64, vp, vp], [None, p, p, p, 0, p, p]),
+        "gs_batch_run": ([vp, i, vp], [None, -1, None]),
+        "gs_batch_fetch": ([vp, vp], [None, p]),
+        "gs_batch_destroy": ([vp], [None]),
+        "gs_batch_capture": ([vp, i], [None, 3]),
+        "gs_batch_replay": ([vp, vp], [None, None]),
+        "gs_server_create": ([vp, vp, vp], [None, p, p]),
+        "gs_server_submit": ([vp, vp, vp, vp, u64, vp], [None, p, p, p, 0, p]),
+        "gs_server_wait": ([vp, u64, vp, vp], [None, 0, p, None]),
+        "gs_server_get_stats": ([vp, vp], [None, p]),
+        "gs_server_destroy": ([vp], [None]),
+    }
+    for name, (types, args) in calls.items():
+        fn = getattr(L, name)
+        fn.argtypes, fn.restype = types, C.c_int
+        assert fn(*args) == g.GS_EINVAL, name
