@@ -162,6 +162,7 @@ def lib():
         L.gs_bwa_ref_contig.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), u64p, u64p]
         L.gs_bwa_ref_hole.argtypes = [C.c_void_p, C.c_int, u64p, u64p, C.c_char_p]
         L.gs_bwa_ref_locate.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), u64p]
+        L.gs_set_launch_chunk.argtypes = [C.c_uint64]
         _lib = L
     return _lib
 
@@ -269,6 +270,12 @@ class BwaRef:
         return c.value, off.value
 
 
+def set_launch_chunk(items):
+    """Work-items per launch of the chunked kernels (test support; 0 = the
+    default 2^30)."""
+    _check("gs_set_launch_chunk", lib().gs_set_launch_chunk(items))
+
+
 def roofline_random(buffer_bytes, line_bytes=16, mode=0, inflight=1, grid=8192, iters=256, reps=3):
     r = RooflineResult()
     _check("gs_roofline_random", lib().gs_roofline_random(buffer_bytes, line_bytes, mode, inflight,
@@ -340,6 +347,14 @@ class Index:
         _check("gs_index_read_direct", lib().gs_index_read_direct(self.h, first, count, _ptr(sa, C.c_uint32),
                                                                   _ptr(isa, C.c_uint32), _ptr(text, C.c_uint8)))
         return sa, isa, text
+
+    def read_text(self, first, count):
+        """Text codes 0..3 over [first, first+count) as the direct-extension
+        structures hold them (derived from the index by walking it)."""
+        text = np.empty(count, dtype=np.uint8)
+        _check("gs_index_read_direct", lib().gs_index_read_direct(self.h, first, count, None, None,
+                                                                  _ptr(text, C.c_uint8)))
+        return text
 
     def locate(self, rows):
         """Text positions of suffix-array rows as the reference's walk reports them."""
@@ -532,6 +547,7 @@ def _bind_server(L):
     L.gs_server_clock.argtypes = [C.c_void_p]
     L.gs_server_clock.restype = C.c_double
     L.gs_server_destroy.argtypes = [C.c_void_p]
+    L.gs_server_inject_failure.argtypes = [C.c_void_p, C.c_uint64, C.c_int]
     L._server_bound = True
 
 
@@ -574,6 +590,10 @@ class Server:
 
     def clock(self):
         return lib().gs_server_clock(self.h)
+
+    def inject_failure(self, round_no, rc):
+        """Test support: the round_no-th round taken fails with rc."""
+        _check("gs_server_inject_failure", lib().gs_server_inject_failure(self.h, round_no, rc))
 
     def stats(self):
         st = ServerStats()

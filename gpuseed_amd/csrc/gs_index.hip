@@ -118,9 +118,9 @@ __global__ void repack_occ96(const uint32_t* __restrict__ bwt, uint32_t n, uint6
 // after i characters depends on the low 2i bits of p; character i is applied
 // next; an empty interval is carried unchanged (the break at 2666-2668).
 __global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uint2* __restrict__ next,
-                           uint32_t level, uint64_t first, uint64_t count) {
+                           uint32_t level, uint64_t first, uint64_t end) {
   const uint64_t t = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (t >= count) return;
+  if (t >= end) return;
   const uint64_t q = t & ((1ull << (2 * level)) - 1);
   const uint32_t c = (uint32_t)(t >> (2 * level));
   uint32_t l = 0, u = X.seq_len;
@@ -137,9 +137,9 @@ __global__ void kmer_level(const DevIndex X, const uint2* __restrict__ prev, uin
 // left extensions c3.c2.c1.P of pattern P occur (bit c1 + 4*c2 + 16*c3),
 // found by a depth-first search that stops at empty intervals.
 __global__ void kmer_presence(const DevIndex X, const uint2* __restrict__ level, uint4* __restrict__ out,
-                              uint64_t first, uint64_t count) {
+                              uint64_t first, uint64_t end) {
   const uint64_t t = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (t >= count) return;
+  if (t >= end) return;
   const uint2 iv = level[t];
   uint64_t mask = 0;
   if (iv.x <= iv.y) {
@@ -168,19 +168,33 @@ __global__ void kmer_presence(const DevIndex X, const uint2* __restrict__ level,
 // (the textbook one: the row of the suffix one position to the left) until
 // it reaches position 0 or another sampled row, recording each row's
 // position and the base it precedes.  Every row is visited by exactly one
-// thread; positions fall at every step, so each walk ends.  The bounds
-// checks only guard against an inconsistent index.
-__global__ void direct_chains(const DevIndex X, uint64_t first, uint64_t n_sa, uint32_t* __restrict__ sa_full,
-                              uint32_t* __restrict__ isa, uint8_t* __restrict__ text8) {
+// thread; positions fall at every step, so each walk ends.  The walk checks
+// the index against itself: a walk that lands on a sampled row must arrive
+// with the position the sampled array holds there, the walk that reaches
+// position 0 must be at row primary, and the walks together must visit
+// n + 1 rows.  err[0] collects bit 1 for a wrong landing, bit 2 for a row or
+// position out of range; err[1..2] count the rows visited (64 bits).
+__global__ void direct_chains(const DevIndex X, uint64_t first, uint64_t end, uint32_t* __restrict__ sa_full,
+                              uint32_t* __restrict__ isa, uint8_t* __restrict__ text8,
+                              uint32_t* __restrict__ err) {
   const uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i >= n_sa) return;
+  if (i >= end) return;
   const uint32_t n = X.seq_len;
   uint32_t row = (uint32_t)i * X.sa_intv;
   uint32_t pos = i ? X.sa[i] : n;
-  while (row <= n && pos <= n) {
+  uint32_t visited = 0, bad = 0;
+  while (true) {
+    if (row > n || pos > n) {
+      bad |= 2u;
+      break;
+    }
     sa_full[row] = pos;
     isa[pos] = row;
-    if (pos == 0) break;  // row primary: the suffix at position 0, preceded by $
+    ++visited;
+    if (pos == 0) {  // the suffix at position 0, preceded by $: row primary
+      if (row != X.primary) bad |= 1u;
+      break;
+    }
     const uint32_t kk = row - (row > X.primary ? 1u : 0u);
     const uint32_t b = gs::div96(kk), bit = kk - b * gs::kBlockBases;
     const uint4* p = X.occ + (size_t)b * 4;
@@ -199,8 +213,14 @@ __global__ void direct_chains(const DevIndex X, uint64_t first, uint64_t n_sa, u
     text8[pos - 1] = (uint8_t)c;
     row = X.L2[c] + gs::rank_in_entry(e, bit);
     --pos;
-    if (row - gs::div_sa(X, row) * X.sa_intv == 0) break;  // another thread's start
+    const uint32_t j = gs::div_sa(X, row);
+    if (row - j * X.sa_intv == 0) {  // another thread's start: it must hold this position
+      if (row > n || X.sa[j] != pos || row == 0) bad |= 1u;
+      break;
+    }
   }
+  if (bad) atomicOr(err, bad);
+  atomicAdd(reinterpret_cast<unsigned long long*>(err + 2), (unsigned long long)visited);
 }
 
 // 16 bases per word, base i at bits 2*(i & 15); words past the text are 0
@@ -218,18 +238,33 @@ __global__ void pack_text(const uint8_t* __restrict__ text8, uint64_t n, uint32_
 
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// One table level in launches of at most 2^30 work-items (a dispatch's
-// work-item count is a 32-bit quantity; level 16 has 2^32 entries).
+// One table level in launches of at most launch_chunk() work-items (level
+// 16 has 2^32 entries).
 void launch_level(const DevIndex& D, const uint2* prev, uint2* next, uint32_t level, uint64_t cnt,
                   hipStream_t st) {
-  const uint64_t step = 1ull << 30;
+  const uint64_t step = gs::launch_chunk();
   for (uint64_t first = 0; first < cnt; first += step) {
     const uint64_t n = cnt - first < step ? cnt - first : step;
-    hipLaunchKernelGGL(kmer_level, dim3(grid_for(n, 256)), dim3(256), 0, st, D, prev, next, level, first, cnt);
+    hipLaunchKernelGGL(kmer_level, dim3(grid_for(n, 256)), dim3(256), 0, st, D, prev, next, level, first,
+                       first + n);
   }
 }
 
+std::atomic<uint64_t> g_launch_chunk{1ull << 30};
+
 }  // namespace
+
+uint64_t gs::launch_chunk() { return g_launch_chunk.load(std::memory_order_relaxed); }
+
+// Test support: smaller launches make every chunked loop run past its first
+// chunk on a small index (0 restores 2^30).  No counterpart in the reference.
+extern "C" GS_API int gs_set_launch_chunk(uint64_t items) {
+  if (items > (1ull << 30)) return GS_EINVAL;
+  g_launch_chunk.store(items ? items : (1ull << 30), std::memory_order_relaxed);
+  return GS_OK;
+}
+
+void gs::index_acquire(gs_index* X) { X->refs.fetch_add(1, std::memory_order_relaxed); }
 
 int gs_build_kmer_table(gs_index* X) {
   const uint64_t full = 1ull << (2 * gs::kPrecalc);
@@ -266,9 +301,10 @@ int gs_build_kmer_table(gs_index* X) {
 // when the pattern is absent with k < len or its window holds a code above 3.
 extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
   if (!X || len < gs::kPrecalc || len > 16) return GS_EINVAL;
+  if (X->servers.load() > 0) return GS_EINVAL;  // their workers may be reading the table
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   if (X->kext) {
-    GS_HIP(hipStreamSynchronize(X->stream));
+    GS_HIP(hipDeviceSynchronize());  // batches run on streams of their own
     (void)hipFree(X->kext);
     X->device_bytes -= (1ull << (2 * X->ext_len)) * sizeof(uint4);
     X->kext = nullptr;
@@ -300,11 +336,11 @@ extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
   const uint64_t cnt = 1ull << (2 * len);
   if (!rc && hipMalloc(&table, cnt * sizeof(uint4)) != hipSuccess) rc = GS_ENOMEM;
   if (!rc) {
-    const uint64_t step = 1ull << 30;  // a dispatch's work-item count is a 32-bit quantity
+    const uint64_t step = gs::launch_chunk();
     for (uint64_t first = 0; first < cnt; first += step) {
       const uint64_t n = cnt - first < step ? cnt - first : step;
       hipLaunchKernelGGL(kmer_presence, dim3(grid_for(n, 256)), dim3(256), 0, X->stream, D,
-                         levels[len - 14], table, first, cnt);
+                         levels[len - 14], table, first, first + n);
     }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
   }
@@ -373,9 +409,10 @@ static int find_quirk_rows(gs_index* X) {
 // itself (direct_chains), so they describe exactly its text.
 extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
   if (!X) return GS_EINVAL;
+  if (X->servers.load() > 0) return GS_EINVAL;  // their workers may be reading the structures
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   if (X->sa_full) {
-    GS_HIP(hipStreamSynchronize(X->stream));
+    GS_HIP(hipDeviceSynchronize());  // batches run on streams of their own
     free_direct(X);
   }
   if (!on) return GS_OK;
@@ -383,23 +420,35 @@ extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
   const uint64_t n = X->seq_len, rows = n + 1;
   const uint64_t words = 4 * ((n + 63) / 64 + 1);  // whole 64-base chunks plus one
   uint8_t* text8 = nullptr;
+  uint32_t* d_err = nullptr;  // [0] error bits, [2..3] rows visited
   int rc = GS_OK;
   if (hipMalloc(&X->sa_full, rows * 4) != hipSuccess || hipMalloc(&X->isa, rows * 4) != hipSuccess ||
-      hipMalloc(&X->text, words * 4) != hipSuccess || hipMalloc(&text8, n + 1) != hipSuccess)
+      hipMalloc(&X->text, words * 4) != hipSuccess || hipMalloc(&text8, n + 1) != hipSuccess ||
+      hipMalloc(&d_err, 16) != hipSuccess)
     rc = GS_ENOMEM;
   if (!rc) {
     X->text_words = words;
     X->device_bytes += 2 * rows * 4 + words * 4;
     const DevIndex D = X->dev();
-    const uint64_t step = 1ull << 30;  // a dispatch's work-item count is a 32-bit quantity
-    for (uint64_t first = 0; first < X->n_sa; first += step) {
+    uint32_t h_err[4] = {0, 0, 0, 0};
+    if (hipMemsetAsync(d_err, 0, 16, X->stream) != hipSuccess) rc = GS_EHIP;
+    const uint64_t step = gs::launch_chunk();
+    for (uint64_t first = 0; !rc && first < X->n_sa; first += step) {
       const uint64_t m = X->n_sa - first < step ? X->n_sa - first : step;
-      hipLaunchKernelGGL(direct_chains, dim3(grid_for(m, 256)), dim3(256), 0, X->stream, D, first, X->n_sa,
-                         X->sa_full, X->isa, text8);
+      hipLaunchKernelGGL(direct_chains, dim3(grid_for(m, 256)), dim3(256), 0, X->stream, D, first, first + m,
+                         X->sa_full, X->isa, text8, d_err);
     }
-    hipLaunchKernelGGL(pack_text, dim3(grid_for(words, 256)), dim3(256), 0, X->stream, text8, n, X->text, words);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(X->stream) != hipSuccess) rc = GS_EHIP;
+    if (!rc)
+      hipLaunchKernelGGL(pack_text, dim3(grid_for(words, 256)), dim3(256), 0, X->stream, text8, n, X->text, words);
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(h_err, d_err, 16, hipMemcpyDeviceToHost, X->stream) != hipSuccess ||
+                hipStreamSynchronize(X->stream) != hipSuccess))
+      rc = GS_EHIP;
+    // every row once, every landing on the position the sampled array holds
+    const uint64_t visited = (uint64_t)h_err[2] | ((uint64_t)h_err[3] << 32);
+    if (!rc && (h_err[0] != 0 || visited != rows)) rc = GS_EFORMAT;
   }
+  (void)hipFree(d_err);
   (void)hipFree(text8);
   if (!rc) rc = find_quirk_rows(X);
   if (rc) {
@@ -437,6 +486,7 @@ extern "C" GS_API int gs_index_read_direct(const gs_index* X, uint32_t first, ui
 static void free_index(gs_index* X) {
   if (!X) return;
   if (X->work) gs::batch_free(X->work);
+  X->work = nullptr;
   (void)hipFree(X->occ);
   (void)hipFree(X->sa);
   (void)hipFree(X->kmer);
@@ -543,9 +593,23 @@ extern "C" GS_API int gs_index_open(const char* prefix, int device, gs_index** o
   return rc;
 }
 
+void gs::index_release(gs_index* X) {
+  if (X->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+    (void)hipSetDevice(X->device);
+    free_index(X);
+  }
+}
+
+// Drops the caller's reference: the index is freed now, or with the last
+// batch or server made on it.  gs_seed's own batch goes at once.
 extern "C" GS_API int gs_index_close(gs_index* X) {
   if (!X) return GS_EINVAL;
-  free_index(X);
+  if (X->work) {
+    (void)hipSetDevice(X->device);
+    gs::batch_free(X->work);
+    X->work = nullptr;
+  }
+  gs::index_release(X);
   return GS_OK;
 }
 

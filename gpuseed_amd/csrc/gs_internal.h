@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <utility>
 #include <vector>
 
@@ -41,8 +42,24 @@ struct gs_index {
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0, t_direct = 0;
   int sa_rounds = 0;
   gs_batch* work = nullptr;  // gs_seed's batch, kept for the next call (its buffers only grow)
+  // Lifetime: the caller's handle holds one reference, every batch and
+  // server made on the index one more; gs_index_close drops the caller's
+  // and the device structures are freed with the last one, so a batch or
+  // server that outlives the caller's handle still reads valid memory.
+  std::atomic<int> refs{1};
+  std::atomic<int> servers{0};  // live servers: their workers launch at any time
   gs::DevIndex dev() const;
 };
+
+namespace gs {
+void index_acquire(gs_index* X);
+void index_release(gs_index* X);  // frees the index with its last reference
+// Work-items per launch of the chunked kernels (table levels, presence
+// bits, direct-structure chains, suffix-array walk): a dispatch's work-item
+// count is a 32-bit quantity, so launches over more items than this are
+// split.  2^30 unless a test sets it smaller (gs_set_launch_chunk).
+uint64_t launch_chunk();
+}  // namespace gs
 
 namespace gs {
 
@@ -151,7 +168,9 @@ int plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const uin
              uint64_t r1);
 void plan_finish(BatchPlan& P);
 uint64_t plan_device_bytes(const BatchPlan& P);  // device buffers the plan needs, outputs excluded
-gs_batch* batch_new(gs_index* X);
+// hold: the batch keeps a reference to X (every batch but gs_seed's own,
+// which the index owns and frees)
+gs_batch* batch_new(gs_index* X, bool hold = true);
 int batch_load(gs_batch* B, const BatchPlan& P);
 // Grows B's device buffers for any plan of at most `bases` bases and `reads`
 // reads, so that loading such plans allocates (and synchronises) nothing.

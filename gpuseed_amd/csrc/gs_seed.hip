@@ -726,6 +726,7 @@ void launch_seed(const SeedParams& P, hipStream_t st) {
 
 struct gs_batch {
   gs_index* X = nullptr;
+  bool holds = false;  // keeps a reference to X (see batch_new)
   hipStream_t stream = nullptr;  // a batch's own stream: batches of one index may run concurrently
   gs_opts opts;
   uint64_t n_reads = 0, span = 0, tasks = 0;
@@ -796,7 +797,10 @@ static void destroy_batch(gs_batch* B) {
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
   if (B->stream) (void)hipStreamDestroy(B->stream);
+  gs_index* X = B->X;
+  const bool holds = B->holds;
   delete B;
+  if (holds) gs::index_release(X);
 }
 
 // ---- host plan of a batch -----------------------------------------------------
@@ -1024,9 +1028,11 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   return GS_OK;
 }
 
-gs_batch* gs::batch_new(gs_index* X) {
+gs_batch* gs::batch_new(gs_index* X, bool hold) {
   gs_batch* B = new gs_batch();
   B->X = X;
+  B->holds = hold;
+  if (hold) gs::index_acquire(X);
   return B;
 }
 
@@ -1189,9 +1195,9 @@ static int enqueue_write(gs_batch* B, const PostParams& Q, hipStream_t st) {
 }
 
 // locate_rows over rows[0, n), R rows per thread, in launches of at most
-// 2^30 rows (a dispatch's work-item count is a 32-bit quantity).
+// launch_chunk() rows (a dispatch's work-item count is a 32-bit quantity).
 static void launch_locate(const DevIndex& X, uint32_t* rows, uint64_t n, unsigned R, hipStream_t st) {
-  const uint64_t step = 1ull << 30;
+  const uint64_t step = gs::launch_chunk();
   for (uint64_t first = 0; first < n; first += step) {
     const uint64_t m = n - first < step ? n - first : step;
     const unsigned grid = grid_for((m + R - 1) / R, 256);
@@ -1602,7 +1608,7 @@ extern "C" GS_API int gs_seed(gs_index* X, const char* bases, const uint64_t* of
   gs::plan_finish(P);
   // the index keeps this batch for the next call: its device buffers are
   // reused (no allocation, which would synchronise the device, once they fit)
-  if (!X->work) X->work = gs::batch_new(X);
+  if (!X->work) X->work = gs::batch_new(X, false);
   gs_batch* B = X->work;
   if ((rc = gs::batch_load(B, P))) return rc;
   gs_batch_stats s;

@@ -103,6 +103,9 @@ struct gs_server {
   std::vector<gs_batch*> batches;  // one per device thread
   int running = 0;                 // rounds on the device
   bool building = false;           // the planner builds a round outside the lock
+  uint64_t rounds_taken = 0;       // ordinal of the next round a device thread takes
+  uint64_t fail_round = UINT64_MAX;  // test support: this round fails with fail_rc before its upload
+  int fail_rc = GS_OK;
   gs_server_stats st{};
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -253,6 +256,7 @@ void device_main(gs_server* S, gs_batch* B) {
     if (!S->ready) break;
     std::unique_ptr<Round> R = std::move(S->ready);
     ++S->running;
+    const int injected = S->rounds_taken++ == S->fail_round ? S->fail_rc : GS_OK;
     S->cv_plan.notify_all();
     const double t_round = S->now();
     std::vector<char> skip(R->chunks.size());  // requests that already failed keep nothing more
@@ -264,7 +268,7 @@ void device_main(gs_server* S, gs_batch* B) {
     }
     lk.unlock();
     // device work without the lock: the planner and the other workers go on
-    int rc = R->rc ? R->rc : gs::batch_load(B, R->plan);
+    int rc = R->rc ? R->rc : (injected ? injected : gs::batch_load(B, R->plan));
     const double t1 = S->now();
     gs_batch_stats bs;
     memset(&bs, 0, sizeof bs);
@@ -325,6 +329,13 @@ void device_main(gs_server* S, gs_batch* B) {
       --q->inflight;
       if (!q->done) {
         if (rc) {
+          // nothing more is planned for a failed request: it leaves the
+          // active list now, so the planner never walks it once
+          // gs_server_wait has freed it
+          q->next_read = q->n;
+          q->next_cut = q->cuts.size();
+          auto it = std::find(S->active.begin(), S->active.end(), q);
+          if (it != S->active.end()) S->active.erase(it);
           finish_request(S, q, rc);
         } else {
           q->done_reads += c.r1 - c.r0;
@@ -348,6 +359,8 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   if (rc) return rc;
   gs_server* S = new gs_server();
   S->X = X;
+  gs::index_acquire(X);
+  X->servers.fetch_add(1);
   S->o = *o;
   if (!S->o.budget_bytes) S->o.budget_bytes = 64ull << 30;
   if (!S->o.round_bases) S->o.round_bases = 16000000ull;
@@ -371,6 +384,8 @@ extern "C" GS_API int gs_server_create(gs_index* X, const gs_server_opts* o, gs_
   }
   if (rc) {
     for (gs_batch* B : S->batches) gs::batch_free(B);
+    X->servers.fetch_sub(1);
+    gs::index_release(X);
     delete S;
     return rc;
   }
@@ -520,6 +535,20 @@ extern "C" GS_API int gs_server_destroy(gs_server* S) {
   for (auto& kv : S->reqs) delete kv.second;  // submitted but never collected
   for (gs_batch* B : S->batches) gs::batch_free(B);
   const int rc = S->fatal;
+  gs_index* X = S->X;
   delete S;
+  X->servers.fetch_sub(1);
+  gs::index_release(X);
   return rc;
+}
+
+// Test support: the round_no-th round a device thread takes (from 0) fails
+// with rc before its upload, as a failed upload or launch would.  No
+// counterpart in the reference.
+extern "C" GS_API int gs_server_inject_failure(gs_server* S, uint64_t round_no, int rc) {
+  if (!S || rc >= 0) return GS_EINVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->fail_round = round_no;
+  S->fail_rc = rc;
+  return GS_OK;
 }

@@ -171,6 +171,8 @@ typedef struct {
 } gs_index_info;
 
 GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
+/* Drops the caller's handle.  Batches and servers made on the index keep it
+ * alive: its device memory is freed with the last of them. */
 GS_API int gs_index_close(gs_index* idx);
 GS_API int gs_index_get_info(const gs_index* idx, gs_index_info* info);
 /* Copies device structures back for inspection (tests): occurrence entries
@@ -189,6 +191,11 @@ GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t coun
  * back to those steps when the pattern does not occur or its window holds a
  * code above 3.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
+/* gs_index_set_kmer_len and gs_index_set_direct change structures that
+ * every batch and server of the index reads: both return GS_EINVAL while a
+ * server made on the index is alive, and wait for all device work of the
+ * process before freeing anything; the caller must not run a batch of the
+ * index on another thread during the call. */
 /* Adds (on != 0) or drops the direct-extension structures: the full suffix
  * array, its inverse and the text at 2 bits per base, derived on the device
  * from the index itself (8 * (seq_len + 1) + seq_len / 4 bytes of HBM: 25.6 GB
@@ -290,9 +297,12 @@ GS_API void gs_result_free(gs_result* res);
 /* Device-resident batches: gs_seed is gs_batch_create + gs_batch_run +
  * gs_batch_fetch.  gs_batch_create uploads the reads once; gs_batch_run runs
  * the whole device pipeline and leaves the results in HBM (it may be called
- * again, e.g. for timing); gs_batch_fetch copies them to a gs_result.  One
- * batch at a time per index (they share the index's stream).  No counterpart
- * in the reference, whose batches are host round trips (seed_gen.cu:3108-3601). */
+ * again, e.g. for timing); gs_batch_fetch copies them to a gs_result.  Each
+ * batch has a HIP stream of its own, so batches of one index may run from
+ * different threads at once; one batch must not be used by two threads at
+ * once.  A batch keeps its index alive until gs_batch_destroy.  No
+ * counterpart in the reference, whose batches are host round trips
+ * (seed_gen.cu:3108-3601). */
 typedef struct gs_batch gs_batch;
 
 typedef struct {
@@ -399,8 +409,22 @@ GS_API int gs_server_submit(gs_server* srv, const char* bases, const uint64_t* o
 GS_API int gs_server_wait(gs_server* srv, uint64_t ticket, gs_result* out, gs_request_info* info);
 GS_API int gs_server_get_stats(gs_server* srv, gs_server_stats* stats);
 GS_API double gs_server_clock(gs_server* srv);
-/* Finishes every submitted request, then stops the threads. */
+/* Finishes every submitted request, then stops the threads.  Returns the
+ * first failure of any round (that round's requests failed with it; the
+ * others completed), else GS_OK. */
 GS_API int gs_server_destroy(gs_server* srv);
+
+/* ---- test support ----------------------------------------------------------
+ * No counterpart in the reference. */
+/* Work-items per launch of the chunked kernels (table levels, presence bits,
+ * direct-structure chains, suffix-array walk), so that a small index runs
+ * every chunked loop past its first chunk; 0 restores the default 2^30.
+ * Process-wide; set it while no device work runs. */
+GS_API int gs_set_launch_chunk(uint64_t items);
+/* The round_no-th round a server's device threads take (counted from 0)
+ * fails with rc (negative) before its upload; its requests fail with rc and
+ * later requests are served normally. */
+GS_API int gs_server_inject_failure(gs_server* srv, uint64_t round_no, int rc);
 
 /* The -o text of a result (seed_gen.cu:3625-3648) written to file descriptor
  * fd, reads numbered from first_read_no. */

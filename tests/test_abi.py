@@ -124,6 +124,7 @@ def test_null_handles_are_rejected(built):
         "gs_server_wait": ([vp, u64, vp, vp], [None, 0, p, None]),
         "gs_server_get_stats": ([vp, vp], [None, p]),
         "gs_server_destroy": ([vp], [None]),
+        "gs_server_inject_failure": ([vp, u64, i], [None, 0, -3]),
     }
     for name, (types, args) in calls.items():
         fn = getattr(L, name)
