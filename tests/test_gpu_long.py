@@ -53,9 +53,10 @@ def test_segments_match_oracle(g, long_case, smem, k, seg_tasks):
     say("long reads smem=%d k=%d seg_tasks=%d" % (smem, k, seg_tasks))
     want = oracle_lists(long_case["prefix"], long_case["reads"], k, smem)
     with g.Index(long_case["prefix"]) as X:
-        got = g.seed(X, long_case["reads"], k=k, smem=smem, kernel=1, seg_tasks=seg_tasks).as_lists()
-    bad = [i for i in range(len(want)) if got[i] != want[i]]
-    assert not bad, "read %d (len %d)" % (bad[0], len(long_case["reads"][bad[0]]))
+        for kernel in (1, 3):  # 8-bit and 4-bit size records
+            got = g.seed(X, long_case["reads"], k=k, smem=smem, kernel=kernel, seg_tasks=seg_tasks).as_lists()
+            bad = [i for i in range(len(want)) if got[i] != want[i]]
+            assert not bad, "kernel %d read %d (len %d)" % (kernel, bad[0], len(long_case["reads"][bad[0]]))
 
 
 @pytest.mark.parametrize("smem", [True, False])
@@ -106,16 +107,43 @@ def test_config4_mix_batched_equals_plain_and_oracle(g, mix, smem):
     assert lens.max() > 5000 and lens.min() >= 100
     with g.Index(mix["prefix"]) as X:
         plain = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=0)
-        batched = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1)
-        b16 = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=2, seg_tasks=40)
-        nosplit = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1, split_tasks=g.NO_SPLIT)
-    for out in (batched, b16, nosplit):
-        assert np.array_equal(out.seeds, plain.seeds)
-        assert np.array_equal(out.pos, plain.pos)
-        assert np.array_equal(out.read_seed_off, plain.read_seed_off)
+        runs = {
+            "kernel 1": g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1),
+            "kernel 2 seg 40": g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=2, seg_tasks=40),
+            "kernel 1 no split": g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=1, split_tasks=g.NO_SPLIT),
+            "kernel 3": g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=3),
+            "kernel 3 seg 32": g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=3, seg_tasks=32),
+        }
+    for name, out in runs.items():
+        assert np.array_equal(out.seeds, plain.seeds), name
+        assert np.array_equal(out.pos, plain.pos), name
+        assert np.array_equal(out.read_seed_off, plain.read_seed_off), name
     ox = ob.Index(mix["prefix"])
     seeds, pos, so, po, _ = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=8)
     ox.close()
     assert np.array_equal(so, plain.read_seed_off) and np.array_equal(po, plain.read_pos_off)
     assert np.array_equal(seeds, plain.seeds.view(np.uint32).reshape(-1, 6))
     assert np.array_equal(pos, plain.pos)
+
+
+@pytest.mark.parametrize("smem", [True, False])
+def test_config4_mix_with_tables_and_direct_extension(g, mix, smem):
+    """The benchmark's configuration-4 set-up: the 16-mer table with presence
+    bits and the direct-extension structures, the default kernel (4-bit
+    records), with and without forced work splitting; bit-exact to the
+    oracle."""
+    k = 19
+    buf, offs, lens = mix["arrays"]
+    ox = ob.Index(mix["prefix"])
+    seeds, pos, so, po, _ = ob.seed_arrays(ox, buf, offs, lens, k, smem, max_occ=500, threads=8)
+    ox.close()
+    with g.Index(mix["prefix"]) as X:
+        X.set_kmer_len(16)
+        X.set_direct(True)
+        for kernel, kw in ((3, {}), (3, dict(seg_tasks=32, direct_min=1)), (1, dict(direct_min=2)),
+                           (3, dict(split_tasks=2, split_work=1, direct_min=1))):
+            say("config-4 mix with tables smem=%d kernel=%d %s" % (smem, kernel, kw))
+            out = g.seed(X, arrays=mix["arrays"], k=k, smem=smem, kernel=kernel, max_occ=500, **kw)
+            assert np.array_equal(so, out.read_seed_off) and np.array_equal(po, out.read_pos_off), (kernel, kw)
+            assert np.array_equal(seeds, out.seeds.view(np.uint32).reshape(-1, 6)), (kernel, kw)
+            assert np.array_equal(pos, out.pos), (kernel, kw)

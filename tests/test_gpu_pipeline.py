@@ -78,12 +78,37 @@ def test_work_splitting_does_not_change_output(g, case, smem, k):
     with g.Index(case["prefix"]) as X:
         ref = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=0, max_occ=50)
         for kernel, split_tasks, split_work in ((1, 1, 1), (1, 3, 1), (2, 8, 2), (1, 64, 1), (1, 8, 32),
-                                                (2, 5, 4)):
+                                                (2, 5, 4), (3, 1, 1), (3, 3, 1), (3, 8, 2), (3, 64, 1)):
             say("smem=%d kernel=%d split_tasks=%d split_work=%d" % (smem, kernel, split_tasks, split_work))
             out = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=kernel, max_occ=50,
                          split_tasks=split_tasks, split_work=split_work, seg_tasks=48)
             assert same(out, ref), (kernel, split_tasks, split_work,
                                     first_diff(out, ref, case["arrays"][2]))
+
+
+@pytest.mark.parametrize("smem,k", [(True, 19), (False, 20)])
+def test_work_splitting_with_tables_and_direct_extension(g, case, smem, k):
+    """Forced splitting on an index with the 16-mer table (presence bits)
+    and the direct-extension structures: pieces start without size records
+    and may start inside a direct run; kernels 1 and 3 against the plain
+    kernel and the oracle."""
+    import oracle_bind as ob
+    ox = ob.Index(case["prefix"])
+    seeds, pos, so, po, _ = ob.seed_arrays(ox, *case["arrays"], k=k, smem=smem, max_occ=50, threads=8)
+    ox.close()
+    with g.Index(case["prefix"]) as X:
+        X.set_kmer_len(16)
+        X.set_direct(True)
+        ref = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=0, max_occ=50)
+        assert (np.array_equal(so, ref.read_seed_off) and np.array_equal(po, ref.read_pos_off) and
+                np.array_equal(seeds, ref.seeds.view(np.uint32).reshape(-1, 6)) and np.array_equal(pos, ref.pos))
+        for kernel, split_tasks, split_work, dmin in ((3, 1, 1, 1), (3, 2, 1, 2), (1, 1, 1, 1), (3, 8, 2, 4),
+                                                      (1, 3, 1, 3)):
+            say("smem=%d kernel=%d split_tasks=%d split_work=%d direct_min=%d" %
+                (smem, kernel, split_tasks, split_work, dmin))
+            out = g.seed(X, arrays=case["arrays"], k=k, smem=smem, kernel=kernel, max_occ=50,
+                         split_tasks=split_tasks, split_work=split_work, seg_tasks=48, direct_min=dmin)
+            assert same(out, ref), (kernel, split_tasks, split_work, dmin, first_diff(out, ref, case["arrays"][2]))
 
 
 def test_bad_tuning_fields_are_refused(g, case):

@@ -129,3 +129,35 @@ def test_unknown_ticket_is_an_error(g, world):
         with g.Server(X, k=19, smem=True) as S:
             with pytest.raises(g.GsError):
                 S.wait(12345)
+
+
+def test_failed_round_fails_its_requests_only(g, world):
+    """A round that fails (injected before its upload, as a failed upload
+    would) fails the request it belongs to; that request is collected while
+    its later chunks are still planned, and the requests submitted after it
+    are served and equal gs_seed alone (MEM mode, small reference batches)."""
+    buf, offs, lens = world["buf"], world["offs"], world["lens"]
+
+    def req(a, m):
+        lo, hi = int(offs[a]), int(offs[a + m - 1] + lens[a + m - 1])
+        return buf[lo:hi].copy(), (offs[a:a + m] - np.uint64(lo)).copy(), lens[a:a + m].copy()
+
+    kw = dict(k=20, smem=False, batch_bases=20000)
+    A, B, C = req(0, 100), req(200, 30), req(400, 5)
+    assert int(A[2].sum()) > 4 * 20000  # A spans several reference batches, one per round
+    with g.Index(world["prefix"]) as X:
+        alone = [g.seed(X, arrays=r, kernel=0, **kw) for r in (B, C)]
+        S = g.Server(X, round_bases=1, round_growth=1, workers=1, **kw)
+        try:
+            S.inject_failure(2, g.GS_EHIP)
+            ta = S.submit(A)
+            with pytest.raises(g.GsError) as ei:
+                S.wait(ta)
+            assert ei.value.rc == g.GS_EHIP
+            for r, want in zip((B, C), alone):
+                out, info = S.wait(S.submit(r))
+                assert same(out, want) and info.n_reads == len(r[2])
+        finally:
+            with pytest.raises(g.GsError) as ei:  # the server reports the failed round
+                S.close()
+            assert ei.value.rc == g.GS_EHIP
