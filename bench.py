@@ -49,6 +49,15 @@ def cpu_model():
     return "unknown"
 
 
+def host_cpus():
+    """(logical CPUs of the machine, CPUs this process may run on)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return os.cpu_count() or avail, avail
+
+
 def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0, prune_sat=0):
     """Memory operations per read of the device algorithm, counted by the
     oracle on the first `sample` reads: 13-mer table lookups (8 bytes),
@@ -218,25 +227,82 @@ def serve_bench(g, X, ref, k, max_occ, args):
                          "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
 
 
-def profiled_traffic(config, kernel, kmer_len, direct, reads):
-    """HBM bytes per launch of the extension kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), scaled to `reads`
-    per launch, if it was measured on this workload; else None."""
+def profiled_counters(config, kernel, kmer_len, direct):
+    """The committed rocprofv3 counter summary of the extension kernel
+    (profiles/seed_persistent_config<N>.txt) if it was measured on this
+    workload and index shape, else None."""
     path = os.path.join(ROOT, "profiles", "seed_persistent_config%d.txt" % config)
     if not os.path.exists(path):
-        return None, None
+        return None
     kv = {}
     for line in open(path):
         parts = line.split()
         if len(parts) == 2 and not line.startswith("#"):
             kv[parts[0]] = float(parts[1])
     if kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len or kv.get("direct", 0) != direct:
+        return None
+    kv["path"] = os.path.relpath(path, ROOT)
+    return kv
+
+
+def profiled_traffic(kv, reads):
+    """HBM bytes per launch of the extension kernel from the FETCH_SIZE and
+    WRITE_SIZE counters, scaled to `reads` per launch."""
+    if not kv:
         return None, None
     per_launch = (kv["fetch_kib"] * kv["fetch_factor"] + kv["write_kib"] * kv["write_factor"]) * 1024.0
     src = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE at %d reads per launch (%s), scaled by the stream-copy "
-           "calibration%s" % (kv["reads_per_launch"], os.path.relpath(path, ROOT),
+           "calibration%s" % (kv["reads_per_launch"], kv["path"],
                               "" if kv["reads_per_launch"] == reads else " and to %d reads per launch" % reads))
     return per_launch * reads / kv["reads_per_launch"], src
+
+
+def sample_arrays(arrays, n):
+    buf, offs, lens = arrays
+    n = min(n, len(lens))
+    end = int(offs[n - 1] + lens[n - 1])
+    return buf[:end], offs[:n], lens[:n]
+
+
+def cli_end_to_end(arrays, prefix, tmpdir, n_reads, k, max_occ, kmer_len, direct, smem=True):
+    """The drop-in command line, seed_gen, on a FASTA file of the first
+    n_reads reads: index files loaded and structures built by the process
+    itself, reads parsed from the file, seeds formatted to /dev/null with -o
+    and not formatted without it.  Wall time of the process, and its own
+    summary (index ready, then seeding)."""
+    import re
+    import subprocess
+    buf, offs, lens = sample_arrays(arrays, n_reads)
+    fa = os.path.join(tmpdir, "reads.fa")
+    with open(fa, "wb") as f:
+        mv = memoryview(buf)
+        for i in range(len(lens)):
+            o = int(offs[i])
+            f.write(b">r\n")
+            f.write(mv[o:o + int(lens[i])])
+            f.write(b"\n")
+    exe = os.path.join(ROOT, "gpuseed_amd", "bin", "seed_gen")
+    out = {"reads": int(len(lens)), "bases": int(lens.sum()),
+           "command": "seed_gen %s-k %d --max-occ %d --kmer-len %d%s [-o] <prefix> reads.fa"
+                      % ("-s " if smem else "", k, max_occ, kmer_len, " --direct" if direct else "")}
+    for label, extra in (("with_o", ["-o"]), ("without_o", [])):
+        cmd = [exe] + (["-s"] if smem else []) + ["-k", str(k), "--max-occ", str(max_occ), "--kmer-len",
+                                                  str(kmer_len)] + (["--direct"] if direct else []) + extra
+        cmd += [prefix, fa]
+        t = time.perf_counter()
+        p = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+        wall = time.perf_counter() - t
+        if p.returncode != 0:
+            raise RuntimeError("seed_gen failed (%d): %s" % (p.returncode, p.stderr[-2000:]))
+        err = p.stderr
+        m = re.search(r"index ready after ([0-9.]+) s; seeding ([0-9.]+) s", err)
+        t_index, t_seed = (float(m.group(1)), float(m.group(2))) if m else (None, None)
+        r = {"wall_s": wall, "reads_per_s_wall": len(lens) / wall, "index_ready_s": t_index,
+             "seeding_s": t_seed, "reads_per_s_after_index": len(lens) / t_seed if t_seed else None}
+        out[label] = r
+        log("seed_gen %s: %.2f s wall (index ready after %s s, seeding %s s)" % (label, wall, t_index, t_seed))
+    os.unlink(fa)
+    return out
 
 
 def timed_steps(batches, warmup, kernel):
@@ -283,6 +349,11 @@ def main():
                     help="batched kernel variant (1: 8-bit, 2: 16-bit, 3: 4-bit size records)")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--no-graph", action="store_true", help="skip the host-to-host (HIP graph) measurement")
+    ap.add_argument("--no-reference-shape", action="store_true",
+                    help="skip the second timing on the reference's index shape (13-mer table, sampled "
+                         "suffix-array walk, no direct-extension structures)")
+    ap.add_argument("--cli-reads", type=int, default=2000000,
+                    help="reads of the seed_gen end-to-end measurement (0: none)")
     ap.add_argument("--profile-steps", action="store_true", help="stop after the timed steps")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement (configuration 5)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement")
@@ -373,6 +444,7 @@ def main():
     log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (resident, args.batch, len(buf) / 1e9))
     batches = [batches[i % resident] for i in range(args.warmup + args.steps)]
 
+    shape = {"kmer_table": info.kmer_len, "direct_extension": bool(direct), "device_bytes": int(info.device_bytes)}
     elapsed, stats = timed_steps(batches, args.warmup, args.kernel)
     reads = args.steps * args.batch
     value = reads / elapsed
@@ -410,6 +482,50 @@ def main():
     seed_s = sum(s.t_seed for s in stats) / len(stats)
     first_arrays, first_batch = batches[args.warmup]
     gpu_out = first_batch.fetch()
+
+    # the same resident batches on the reference's own index shape: the
+    # 13-mer table only and no direct-extension structures, so positions
+    # come from the sampled-suffix-array walk (sampling 7) inside the timed
+    # region; the output must be the same bit for bit
+    ref_shape = None
+    if not args.no_reference_shape and (info.kmer_len > 13 or direct):
+        X.set_kmer_len(13)
+        X.set_direct(False)
+        rinfo = X.info()
+        r_elapsed, r_stats = timed_steps(batches, min(1, args.warmup), args.kernel)
+        r_steps = len(batches) - min(1, args.warmup)
+        r_out = first_batch.fetch()
+        same = (np.array_equal(r_out.seeds, gpu_out.seeds) and np.array_equal(r_out.pos, gpu_out.pos) and
+                np.array_equal(r_out.read_seed_off, gpu_out.read_seed_off))
+        ref_shape = {"kmer_table": 13, "direct_extension": False, "device_bytes": int(rinfo.device_bytes),
+                     "reads_per_s": r_steps * args.batch / r_elapsed,
+                     "kernel_ms": {"seed": sum(x.t_seed for x in r_stats) / len(r_stats) * 1e3,
+                                   "count": sum(x.t_count for x in r_stats) / len(r_stats) * 1e3,
+                                   "write": sum(x.t_write for x in r_stats) / len(r_stats) * 1e3,
+                                   "locate": sum(x.t_locate for x in r_stats) / len(r_stats) * 1e3},
+                     "requests_per_read_issued": sum(x.n_table_reads + x.n_occ_reads + x.n_direct_reads
+                                                     for x in r_stats) / (len(r_stats) * args.batch),
+                     "same_output_as_headline": bool(same)}
+        log("reference-shaped index (13-mer table, suffix-array walk): %.0f reads/s, seed %.1f ms, locate %.1f ms, "
+            "%.2f GB on device, output %s" % (ref_shape["reads_per_s"], ref_shape["kernel_ms"]["seed"],
+                                              ref_shape["kernel_ms"]["locate"], rinfo.device_bytes / 1e9,
+                                              "identical" if same else "DIFFERENT"))
+        del r_out
+        if info.kmer_len > 13:
+            X.set_kmer_len(info.kmer_len)
+        if direct:
+            X.set_direct(True)
+
+    # the extension kernel's own counters on exactly the reads the work
+    # model counts (no work splitting, which the model does not describe)
+    sample_counts = None
+    if not args.no_cpu:
+        sa = sample_arrays(first_arrays, args.work_sample)
+        sb = g.Batch(X, sa, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks, split_tasks=g.NO_SPLIT)
+        st_s = sb.run(args.kernel)
+        sb.close()
+        sample_counts = dict(reads=len(sa[2]), table=st_s.n_table_reads, occ=st_s.n_occ_reads,
+                             direct=st_s.n_direct_reads)
     for _, bt in batches:
         bt.close()
     del batches
@@ -431,8 +547,11 @@ def main():
     log("random-access ceiling: %.1f G requests/s" % (r_max / 1e9))
 
     cpu = None
+    nproc, avail = host_cpus()
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every CPU this process may use, at most the 16 host cores that go
+        # with one GPU of the machine
+        threads = args.cpu_threads or min(16, avail)
         cpu = cpu_baseline(prefix, first_arrays, gpu_out, args.cpu_sample, threads, k, max_occ, smem)
         cpu["threads"] = threads
         log("cpu baseline: %.0f reads/s on %d threads; parity %d/%d reads differ"
@@ -448,6 +567,9 @@ def main():
         serving = serve_bench(g, X, serve_ref, k, max_occ, args)
         del serve_ref
     X.close()
+    cli = None
+    if args.cli_reads:
+        cli = cli_end_to_end(first_arrays, prefix, tmpdir, args.cli_reads, k, max_occ, info.kmer_len, direct, smem)
     shutil.rmtree(tmpdir, ignore_errors=True)
 
     roofline = None
@@ -461,21 +583,34 @@ def main():
         # reads, counted by the oracle; W for the reference's procedure
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
-        act = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads for st in stats) / (len(stats) * args.batch)
-        traffic, traffic_src = profiled_traffic(args.config, args.kernel, info.kmer_len, direct, args.batch)
+        issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads for st in stats) / (len(stats) * args.batch)
+        prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct)
+        traffic, traffic_src = profiled_traffic(prof, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
                     "frac": achieved / 8e12, "traffic": traffic, "traffic_source": traffic_src,
                     "kernel": "seed_persistent", "kernel_ms": seed_s * 1e3,
-                    "bytes_per_read": wm["bytes_per_read"], "work_sample_reads": wm["reads"],
-                    "random_access": {"requests_per_read": req_per_read,
-                                      "issued_requests_per_read": act,
-                                      "achieved_requests_per_s": req_per_read * args.batch / seed_s,
-                                      "measured_ceiling_requests_per_s": r_max,
-                                      "frac": req_per_read * args.batch / seed_s / r_max,
-                                      "note": "requests include those served by L2 (about 5 %, rocprofv3 "
-                                              "TCC_HIT_sum / TCC_MISS_sum); the ceiling probe's dependent "
-                                              "16-byte gathers over 8 GiB all miss L2"}}
+                    "bytes_per_read": wm["bytes_per_read"], "work_sample_reads": wm["reads"]}
+        # every request that misses L2 moves one 128-byte line from HBM: the
+        # kernel's own request count over the timed launches times the L2
+        # miss share rocprofv3 measured on this workload, against the
+        # measured ceiling of dependent random 16-byte requests (all misses)
+        ra = {"requests_per_read_model": req_per_read, "requests_per_read_issued": issued,
+              "measured_ceiling_requests_per_s": r_max}
+        if sample_counts:
+            sc = sample_counts
+            ra["requests_per_read_issued_on_model_sample"] = (sc["table"] + sc["occ"] + sc["direct"]) / sc["reads"]
+            ra["model_sample_reads"] = sc["reads"]
+        if prof and "tcc_hit" in prof:
+            miss = prof["tcc_miss"] / (prof["tcc_hit"] + prof["tcc_miss"])
+            miss_rate = issued * args.batch * miss / seed_s
+            ra.update({"l2_miss_share": miss, "l2_miss_source": prof["path"],
+                       "achieved_l2_miss_requests_per_s": miss_rate, "frac": miss_rate / r_max})
+            roofline["line_bytes"] = {"bytes_per_read": issued * miss * 128.0,
+                                      "achieved": issued * miss * 128.0 * args.batch / seed_s / 1e9,
+                                      "frac": issued * miss * 128.0 * args.batch / seed_s / 8e12,
+                                      "unit": "GB/s", "note": "one 128-byte HBM line per request that misses L2"}
+        roofline["random_access"] = ra
         if cpu:
             c = cpu["counters"]
             roofline["reference_procedure_requests_per_read"] = (c["w_tasks"] + c["w_blocks96"]) / cpu["reads"]
@@ -500,18 +635,21 @@ def main():
                                   rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
                    "mode": "SMEM" if smem else "MEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
-                   "direct_extension": bool(direct),
+                   "direct_extension": bool(direct), "device_bytes": shape["device_bytes"],
                    "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {
             "value": cpu["reads_per_s"], "unit": "reads/s", "cores": cpu["threads"], "kind": "port",
+            "nproc": nproc, "cpus_available": avail, "cpu_model": cpu_model(),
             "sample": "first %d reads of the first timed batch, oracle with distance-1 pruning and the "
-                      "13-mer table (table build excluded), %s, compared bit for bit with the GPU output"
-                      % (cpu["reads"], cpu_model())},
+                      "13-mer table (table build excluded), %d threads, compared bit for bit with the GPU output"
+                      % (cpu["reads"], cpu["threads"])},
         "index_build_s": t_build,
         "plain": None if not plain else {"reads_per_s": plain["reads_per_s"], "seed_kernel_ms": plain["seed_ms"],
                                          "batched_over_plain": value / plain["reads_per_s"]},
         "host_to_host": e2e,
+        "reference_shaped_index": ref_shape,
+        "cli_end_to_end": cli,
         "kernel_ms": {"seed": seed_s * 1e3,
                       "count": sum(s.t_count for s in stats) / len(stats) * 1e3,
                       "write": sum(s.t_write for s in stats) / len(stats) * 1e3,

@@ -452,6 +452,7 @@ int main(int argc, char** argv) {
             info.device_bytes / 1e9);
   }
 
+  const double t_ready = now();  /* index and its structures on the GPU */
   fifo q_in, q_out;
   fifo_init(&q_in, 2);
   fifo_init(&q_out, 2);
@@ -509,12 +510,14 @@ int main(int argc, char** argv) {
   fclose(fp);
   gs_index_close(idx);
   if (seed_rc || W.failed) return EXIT_FAILURE;
-  const double t_total = now() - t_start;
+  const double t_end = now(), t_total = t_end - t_start, t_seeding = t_end - t_ready;
   fprintf(stderr,
           "%s seeds: %llu reads, %llu bases, %llu device batches, %llu positions\n"
-          "time: total %.3f s, input %.3f s, device %.3f s (kernels %.3f s), output %.3f s; %.1f reads/s\n",
+          "time: total %.3f s, input %.3f s, device %.3f s (kernels %.3f s), output %.3f s; %.1f reads/s\n"
+          "index ready after %.3f s; seeding %.3f s (%.1f reads/s)\n",
           o.smem ? "SMEM" : "MEM", (unsigned long long)R.n_reads, (unsigned long long)R.n_bases,
           (unsigned long long)n_batches, (unsigned long long)W.n_pos, t_total, R.t_read, t_gpu, t_kernels,
-          W.t_write, t_total > 0 ? R.n_reads / t_total : 0.0);
+          W.t_write, t_total > 0 ? R.n_reads / t_total : 0.0, t_ready - t_start, t_seeding,
+          t_seeding > 0 ? R.n_reads / t_seeding : 0.0);
   return 0;
 }
