@@ -101,6 +101,14 @@ SEED_DTYPE = np.dtype([("qbeg", "<u4"), ("qend", "<u4"), ("l", "<u4"), ("u", "<u
                        ("n_pos", "<u4"), ("pad", "<u4")])
 
 
+class ResultBuffers(C.Structure):
+    _fields_ = [("read_nseeds", C.POINTER(C.c_uint32)), ("read_npos", C.POINTER(C.c_uint32)),
+                ("read_seed_off", C.POINTER(C.c_uint64)), ("read_pos_off", C.POINTER(C.c_uint64)),
+                ("seeds", C.POINTER(SeedRec)), ("pos", C.POINTER(C.c_uint32)),
+                ("seeds_cap", C.c_uint64), ("pos_cap", C.c_uint64),
+                ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64)]
+
+
 class Result(C.Structure):
     _fields_ = [("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
                 ("read_nseeds", C.POINTER(C.c_uint32)), ("read_npos", C.POINTER(C.c_uint32)),
@@ -163,6 +171,8 @@ def lib():
         L.gs_bwa_ref_hole.argtypes = [C.c_void_p, C.c_int, u64p, u64p, C.c_char_p]
         L.gs_bwa_ref_locate.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), u64p]
         L.gs_set_launch_chunk.argtypes = [C.c_uint64]
+        L.gs_seed_noalloc.argtypes = [C.c_void_p, C.c_char_p, u64p, u32p, C.c_uint64, C.POINTER(Opts),
+                                      C.POINTER(ResultBuffers)]
         _lib = L
     return _lib
 
@@ -373,6 +383,17 @@ class Index:
 class SeedOutput:
     """Host copy of a gs_result as numpy arrays."""
 
+    @classmethod
+    def from_arrays(cls, nseeds, npos, seed_off, pos_off, seeds, pos):
+        self = cls.__new__(cls)
+        self.n_reads = len(nseeds)
+        self.read_nseeds, self.read_npos = nseeds, npos
+        self.read_seed_off, self.read_pos_off = seed_off, pos_off
+        self.seeds = seeds.view(np.uint32).reshape(-1).view(SEED_DTYPE)
+        self.pos = pos
+        self.times = {}
+        return self
+
     def __init__(self, res):
         n = res.n_reads
         self.n_reads = n
@@ -448,6 +469,35 @@ def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0
         return SeedOutput(res)
     finally:
         lib().gs_result_free(C.byref(res))
+
+
+def seed_noalloc(index, arrays, seeds_cap=None, pos_cap=None, opts=None, **kw):
+    """gs_seed_noalloc: the caller's output arrays (sized seeds_cap and
+    pos_cap; by default large enough for any result of the call that fits
+    the library's fixed device array).  Raises GsError with rc GS_ERANGE and
+    attributes n_seeds/n_pos when the result does not fit."""
+    buf, offs, lens = _arrays(arrays)
+    o = opts if opts is not None else make_opts(**kw)
+    n = len(lens)
+    if seeds_cap is None:
+        seeds_cap = (256 << 20) // 24
+    if pos_cap is None:
+        pos_cap = (256 << 20) // 4
+    nseeds = np.zeros(n, np.uint32)
+    npos = np.zeros(n, np.uint32)
+    so = np.zeros(n + 1, np.uint64)
+    po = np.zeros(n + 1, np.uint64)
+    seeds = np.zeros((max(seeds_cap, 1), 6), np.uint32)
+    pos = np.zeros(max(pos_cap, 1), np.uint32)
+    rb = ResultBuffers(_ptr(nseeds, C.c_uint32), _ptr(npos, C.c_uint32), _ptr(so, C.c_uint64), _ptr(po, C.c_uint64),
+                       C.cast(seeds.ctypes.data, C.POINTER(SeedRec)), _ptr(pos, C.c_uint32), seeds_cap, pos_cap, 0, 0)
+    rc = lib().gs_seed_noalloc(index.h, C.cast(buf.ctypes.data, C.c_char_p), _ptr(offs, C.c_uint64),
+                               _ptr(lens, C.c_uint32), n, C.byref(o), C.byref(rb))
+    if rc != GS_OK:
+        e = GsError("gs_seed_noalloc", rc)
+        e.n_seeds, e.n_pos = rb.n_seeds, rb.n_pos
+        raise e
+    return SeedOutput.from_arrays(nseeds, npos, so, po, seeds[:rb.n_seeds].copy(), pos[:rb.n_pos].copy())
 
 
 class BatchStats(C.Structure):

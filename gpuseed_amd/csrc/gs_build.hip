@@ -456,6 +456,7 @@ extern "C" GS_API int gs_index_build(const uint8_t* codes, uint64_t n, uint32_t 
   const double t4 = wall();
   if ((rc = gs_build_kmer_table(X))) return fail(rc);
   X->t_table = wall() - t4;
+  (void)gs::noalloc_setup(device);  // a failed lookup makes gs_seed_noalloc return GS_EHIP
   *out = X;
   return GS_OK;
 }

@@ -569,6 +569,7 @@ static int index_from_host(const gs_host_index* H, int device, gs_index** out) {
   X->t_upload = t1 - t0;
   X->t_repack = t2 - t1;
   X->t_table = t3 - t2;
+  (void)gs::noalloc_setup(device);  // a failed lookup makes gs_seed_noalloc return GS_EHIP
   *out = X;
   return GS_OK;
 }

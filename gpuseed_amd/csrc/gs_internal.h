@@ -59,6 +59,9 @@ void index_release(gs_index* X);  // frees the index with its last reference
 // count is a 32-bit quantity, so launches over more items than this are
 // split.  2^30 unless a test sets it smaller (gs_set_launch_chunk).
 uint64_t launch_chunk();
+// Looks up the device address of gs_seed_noalloc's scratch array on `device`
+// (which must be current); called once per device where an index is set up.
+int noalloc_setup(int device);
 }  // namespace gs
 
 namespace gs {

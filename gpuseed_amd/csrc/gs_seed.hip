@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -684,8 +685,15 @@ __global__ void zero_bytes(uint8_t* __restrict__ p, uint32_t n) {
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  bool own = true;  // false: a view into memory this buffer must not free or replace
+  void view(void* q, size_t bytes) {
+    p = q;
+    cap = bytes;
+    own = false;
+  }
   int ensure(size_t bytes) {
     if (bytes <= cap) return GS_OK;
+    if (!own) return GS_ENOMEM;
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -699,7 +707,7 @@ struct DevBuf {
     return reinterpret_cast<T*>(p);
   }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
   }
 };
 
@@ -1632,4 +1640,239 @@ extern "C" GS_API void gs_result_free(gs_result* r) {
   free(r->seeds);
   free(r->pos);
   memset(r, 0, sizeof(*r));
+}
+
+// ---- gs_seed_noalloc: the same pipeline without any allocation --------------
+// Every device buffer of the call is a slice of `scratch`, a file-scope array
+// that each GPU holds from the library's load to the end of the process.  The
+// host side keeps no vectors either: the task map, code-chunk offsets, work
+// items and reference-batch ids that gs_batch_create plans on the CPU are
+// computed on the device from the uploaded offsets and lengths.  Calls on
+// one GPU share its array, so calls that overlap on one GPU race.
+
+namespace {
+
+constexpr uint64_t kNoAllocBytes = 256ull << 20;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocDevices];  // device address of `scratch` on each device (null: not looked up)
+int g_scratch_rc[kNoAllocDevices];     // the lookup's result once it has run
+
+// Per read: the offset rebased onto the uploaded span, the task and code
+// chunk counts that the scans turn into offsets, and the two work items of a
+// whole-read stream (items may be null: plain kernel).
+__global__ void noalloc_prepare(uint64_t* __restrict__ offs, const uint32_t* __restrict__ lens, uint32_t n,
+                                uint64_t lo, uint32_t k, uint32_t* __restrict__ ntask,
+                                uint32_t* __restrict__ ncode, uint64_t* __restrict__ items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t L = lens[i];
+  offs[i] -= lo;
+  ntask[i] = L >= k ? L - k + 1 : 0u;
+  ncode[i] = 2u * ((L + 31u) / 32u);
+  if (items) {
+    items[2ull * i] = i;
+    items[2ull * i + 1] = i | (1ull << 32);
+  }
+}
+
+// Reference-batch id of every read: a batch closes after the read that
+// brings it to batch_bases (seed_gen.cu:2955-2991), a running count with a
+// reset.  One wave: the lanes load 64 lengths at a time and every lane
+// follows the whole count, keeping the id of its own read.
+__global__ void __launch_bounds__(64) noalloc_batch_ids(const uint32_t* __restrict__ lens, uint32_t n,
+                                                        uint32_t batch_bases, uint32_t* __restrict__ batch) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t id = 0;
+  uint64_t acc = 0;
+  for (uint32_t base = 0; base < n; base += 64) {
+    const uint32_t i = base + lane;
+    const uint32_t L = i < n ? lens[i] : 0u;
+    const uint32_t m = n - base < 64u ? n - base : 64u;
+    uint32_t mine = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+      const uint32_t Lj = (uint32_t)__shfl((int)L, (int)j, 64);
+      if (lane == j) mine = id;
+      acc += Lj;
+      if (acc >= batch_bases) {
+        ++id;
+        acc = 0;
+      }
+    }
+    if (i < n) batch[i] = mine;
+  }
+}
+
+inline uint64_t align256(uint64_t v) { return (v + 255) & ~255ull; }
+
+}  // namespace
+
+// Looks up this device's copy of `scratch` (the current device must be
+// `device`); called where an index is set up on a device.
+int gs::noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return GS_EINVAL;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return GS_OK;
+  if (g_scratch_rc[device]) return g_scratch_rc[device];
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    g_scratch_rc[device] = GS_EHIP;
+    return GS_EHIP;
+  }
+  g_scratch_ptr[device] = p;
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_seed_noalloc(gs_index* X, const char* bases, const uint64_t* offsets,
+                                      const uint32_t* lens, uint64_t n_reads, const gs_opts* opts,
+                                      gs_result_buffers* out) {
+  if (!X || !opts || !out || (n_reads && (!bases || !offsets || !lens))) return GS_EINVAL;
+  if (!out->read_seed_off || !out->read_pos_off || (n_reads && (!out->read_nseeds || !out->read_npos)))
+    return GS_EINVAL;
+  out->n_seeds = out->n_pos = 0;
+  int rc = gs::plan_check(opts);
+  if (rc) return rc;
+  if (n_reads > 0xFFFFFFFFull / 2) return GS_ERANGE;
+  const uint32_t n = (uint32_t)n_reads, k = (uint32_t)opts->k;
+  // sizes from the lengths alone (one pass, nothing allocated)
+  uint64_t lo = UINT64_MAX, hi = 0, tasks = 0, chunks = 0;
+  uint32_t maxlen = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t L = lens[i];
+    if (offsets[i] < lo) lo = offsets[i];
+    if (offsets[i] + L > hi) hi = offsets[i] + L;
+    tasks += L >= k ? L - k + 1 : 0;
+    chunks += 2 * ((L + 31) / 32);
+    if (L > maxlen) maxlen = L;
+  }
+  if (hi < lo) lo = hi = 0;
+  const uint64_t span = hi - lo, nr1 = (uint64_t)n + 1;
+  // the persistent kernel takes reads held whole in its LDS window; a call
+  // with longer reads runs the plain kernel (the same output)
+  const int kernel = opts->kernel != 0 && maxlen <= gs::kPersistLmax ? opts->kernel : 0;
+  const bool long_scratch = kernel == 0 && maxlen > 256;
+
+  // slices of the scratch array, in order; outputs (seed records, rows) come
+  // after the counts are known
+  uint64_t at = 0;
+  auto take = [&](uint64_t bytes) {
+    const uint64_t o = at;
+    at = align256(at + bytes);
+    return o;
+  };
+  const uint64_t o_bases = take(span + 32), o_offs = take(8 * nr1), o_lens = take(4 * nr1),
+                 o_toff = take(8 * nr1), o_batch = take(4 * nr1), o_code_off = take(8 * nr1),
+                 o_items = kernel ? take(8 * (2 * (uint64_t)n + 1)) : 0,
+                 o_codes = kernel ? take(16 * (chunks + 1)) : 0, o_cand = take(16 * (2 * tasks + 1)),
+                 o_nF = take(4 * nr1), o_nR = take(4 * nr1), o_ns = take(4 * nr1), o_np = take(4 * nr1),
+                 o_soff = take(8 * nr1), o_poff = take(8 * nr1), o_tiles = take(8 * (nr1 / kScanTile + 2)),
+                 o_tot = take(16), o_queue = take(16), o_counters = take(32),
+                 o_long = long_scratch ? take(4 * (span + 16)) : 0;
+  if (at > kNoAllocBytes) return GS_ERANGE;
+  const uint64_t fixed = at;
+
+  if (X->device < 0 || X->device >= kNoAllocDevices) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  uint8_t* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = static_cast<uint8_t*>(g_scratch_ptr[X->device]);
+  }
+  if (!base) return GS_EHIP;  // the lookup at set-up failed
+  hipStream_t st = X->stream;
+
+  // a batch whose buffers are views into the scratch array; it owns nothing
+  gs_batch B;
+  B.X = X;
+  B.stream = st;
+  B.opts = *opts;
+  B.opts.kernel = kernel;
+  B.n_reads = n;
+  B.span = span;
+  B.tasks = tasks;
+  B.maxlen = maxlen;
+  B.code_chunks = chunks;
+  B.n_items = kernel ? 2ull * n : 0;
+  B.n_long = 0;
+  B.seg_tasks = opts->seg_tasks ? opts->seg_tasks : 64u;
+  B.dyn_cap = 0;
+  uint64_t h_tot[2] = {0, 0};
+  B.h_tot = h_tot;
+  B.bases.view(base + o_bases, span + 32);
+  B.offs.view(base + o_offs, 8 * nr1);
+  B.lens.view(base + o_lens, 4 * nr1);
+  B.toff.view(base + o_toff, 8 * nr1);
+  B.batch.view(base + o_batch, 4 * nr1);
+  B.code_off.view(base + o_code_off, 8 * nr1);
+  if (kernel) {
+    B.items.view(base + o_items, 8 * (2 * (uint64_t)n + 1));
+    B.codes.view(base + o_codes, 16 * (chunks + 1));
+  }
+  B.cand.view(base + o_cand, 16 * (2 * tasks + 1));
+  B.nF.view(base + o_nF, 4 * nr1);
+  B.nR.view(base + o_nR, 4 * nr1);
+  B.ns.view(base + o_ns, 4 * nr1);
+  B.np.view(base + o_np, 4 * nr1);
+  B.soff.view(base + o_soff, 8 * nr1);
+  B.poff.view(base + o_poff, 8 * nr1);
+  B.tiles.view(base + o_tiles, 8 * (nr1 / kScanTile + 2));
+  B.tot.view(base + o_tot, 16);
+  B.queue.view(base + o_queue, 16);
+  B.counters.view(base + o_counters, 32);
+  if (long_scratch) B.scratch.view(base + o_long, 4 * (span + 16));
+
+  // inputs, then the plan on the device
+  if ((span && hipMemcpyAsync(B.bases.p, bases + lo, span, hipMemcpyHostToDevice, st) != hipSuccess) ||
+      (n && (hipMemcpyAsync(B.offs.p, offsets, 8ull * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipMemcpyAsync(B.lens.p, lens, 4ull * n, hipMemcpyHostToDevice, st) != hipSuccess)))
+    return GS_EHIP;
+  hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(32), 0, st, B.bases.as<uint8_t>() + span, 32u);
+  if (n) {
+    hipLaunchKernelGGL(noalloc_prepare, dim3(grid_for(n, 256)), dim3(256), 0, st, B.offs.as<uint64_t>(),
+                       B.lens.as<uint32_t>(), n, lo, k, B.ns.as<uint32_t>(), B.np.as<uint32_t>(),
+                       kernel ? B.items.as<uint64_t>() : (uint64_t*)nullptr);
+    scan_counts(B.ns.as<uint32_t>(), n, B.toff.as<uint64_t>(), B.tiles.as<uint64_t>(), B.tot.as<uint64_t>(), st);
+    scan_counts(B.np.as<uint32_t>(), n, B.code_off.as<uint64_t>(), B.tiles.as<uint64_t>(),
+                B.tot.as<uint64_t>() + 1, st);
+    hipLaunchKernelGGL(noalloc_batch_ids, dim3(1), dim3(64), 0, st, B.lens.as<uint32_t>(), n,
+                       opts->batch_bases ? opts->batch_bases : 1000000u, B.batch.as<uint32_t>());
+  }
+  if (hipGetLastError() != hipSuccess) return GS_EHIP;
+
+  // extension and counts, as gs_batch_run
+  unsigned pgrid = 0;
+  if ((rc = persistent_grid_for(&B, kernel, &pgrid))) return rc;
+  SeedParams S;
+  PostParams Q;
+  fill_params(&B, S, Q);
+  if ((rc = enqueue_extension(&B, S, kernel, pgrid, st))) return rc;
+  if ((rc = enqueue_counts(&B, Q, st))) return rc;
+  GS_HIP(hipStreamSynchronize(st));
+  const uint64_t n_seeds = n ? h_tot[0] : 0, n_pos = n ? h_tot[1] : 0;
+  out->n_seeds = n_seeds;
+  out->n_pos = n_pos;
+  const uint64_t o_seeds = fixed, o_rows = align256(o_seeds + sizeof(gs_seed_rec) * (n_seeds + 1));
+  if (o_rows + 4 * (n_pos + 1) > kNoAllocBytes) return GS_ERANGE;
+  if (n_seeds > out->seeds_cap || n_pos > out->pos_cap || (n_seeds && !out->seeds) || (n_pos && !out->pos))
+    return GS_ERANGE;  // n_seeds and n_pos say what the call needs
+  B.n_seeds = n_seeds;
+  B.n_pos = n_pos;
+  B.seeds.view(base + o_seeds, sizeof(gs_seed_rec) * (n_seeds + 1));
+  B.rows.view(base + o_rows, 4 * (n_pos + 1));
+  fill_params(&B, S, Q);
+  if ((rc = enqueue_write(&B, Q, st))) return rc;
+  if ((rc = enqueue_walk(&B, S, Q, st))) return rc;
+  if ((n && (hipMemcpyAsync(out->read_nseeds, B.ns.p, 4ull * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipMemcpyAsync(out->read_npos, B.np.p, 4ull * n, hipMemcpyDeviceToHost, st) != hipSuccess)) ||
+      hipMemcpyAsync(out->read_seed_off, B.soff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(out->read_pos_off, B.poff.p, 8 * nr1, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      (n_seeds && hipMemcpyAsync(out->seeds, B.seeds.p, sizeof(gs_seed_rec) * n_seeds, hipMemcpyDeviceToHost,
+                                 st) != hipSuccess) ||
+      (n_pos && hipMemcpyAsync(out->pos, B.rows.p, 4 * n_pos, hipMemcpyDeviceToHost, st) != hipSuccess))
+    return GS_EHIP;
+  GS_HIP(hipStreamSynchronize(st));
+  return GS_OK;
 }

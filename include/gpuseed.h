@@ -294,6 +294,34 @@ GS_API int gs_seed(gs_index* idx, const char* bases, const uint64_t* offsets, co
                    uint64_t n_reads, const gs_opts* opts, gs_result* out);
 GS_API void gs_result_free(gs_result* res);
 
+/* gs_seed for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it): no hipMalloc, hipFree or
+ * other allocation, on the device or the host.  The device buffers of the
+ * call (reads, task maps, candidates, counts, seed records, rows) are slices
+ * of a fixed 256 MiB array that each GPU the library runs on holds until the
+ * process ends; the results go into the caller's arrays below.  A call whose
+ * buffers do not fit the array returns GS_ERANGE (about 7 million bases of
+ * 150 bp reads per call; candidates take 32 bytes per base); so does a call
+ * whose seeds or positions exceed seeds_cap or pos_cap, with n_seeds and
+ * n_pos set to what it needs.  Output equals gs_seed's.  The persistent
+ * kernel runs when every read fits its 160-base window (opts->kernel 1..3),
+ * else the plain kernel.  Calls on one GPU share the array: calls that
+ * overlap on one GPU race.  Runs on the index's stream.  No counterpart in
+ * the reference. */
+typedef struct {
+  uint32_t* read_nseeds;   /* [n_reads]                                       */
+  uint32_t* read_npos;     /* [n_reads]                                       */
+  uint64_t* read_seed_off; /* [n_reads + 1]                                   */
+  uint64_t* read_pos_off;  /* [n_reads + 1]                                   */
+  gs_seed_rec* seeds;      /* [seeds_cap]                                     */
+  uint32_t* pos;           /* [pos_cap]                                       */
+  uint64_t seeds_cap, pos_cap;
+  uint64_t n_seeds, n_pos; /* set by the call                                 */
+} gs_result_buffers;
+
+GS_API int gs_seed_noalloc(gs_index* idx, const char* bases, const uint64_t* offsets, const uint32_t* lens,
+                           uint64_t n_reads, const gs_opts* opts, gs_result_buffers* out);
+
 /* Device-resident batches: gs_seed is gs_batch_create + gs_batch_run +
  * gs_batch_fetch.  gs_batch_create uploads the reads once; gs_batch_run runs
  * the whole device pipeline and leaves the results in HBM (it may be called
