@@ -58,9 +58,10 @@ def host_cpus():
     return os.cpu_count() or avail, avail
 
 
-def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0, prune_sat=0):
+def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0, prune_sat=0,
+               pair_probe=0):
     """Memory operations per read of the device algorithm, counted by the
-    oracle on the first `sample` reads: 13-mer table lookups (8 bytes),
+    oracle on `sample` reads spread over the batch: 13-mer table lookups (8 bytes),
     extended-table lookups (16 bytes: interval and presence bits), occ
     entries (16 bytes) and direct-extension reads (4-byte suffix-array and
     inverse words, 16-byte text chunks), with distance-1 pruning, the SMEM
@@ -68,22 +69,21 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, 
     records saturate at prune_sat like the kernel's (0: exact)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
-    buf, offs, lens = arrays
-    n = min(sample, len(lens))
-    end = int(offs[n - 1] + lens[n - 1])
-    sub = (buf[:end], offs[:n], lens[:n])
+    sub = strided_sample(arrays, sample)
+    n = len(sub[2])
     ox = ob.Index(prefix)
     c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
                        ext_len=ext_len if ext_len > 13 else 0, miss_skip=1, direct_min=direct_min,
-                       prune_sat=prune_sat)[4]
+                       prune_sat=prune_sat, pair_probe=pair_probe)[4]
     ox.close()
-    table = (c["w_tasks"] + c["w_ext"]) / n
+    table = (c["w_tasks"] + c["w_ext"] + c["w_pair"]) / n
     occ = c["w_blocks96"] / n
     direct, text = c["w_direct"] / n, c["w_text"] / n
     ext_bytes = 16.0 if ext_len > 13 else 8.0
     return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct,
                 requests_per_read=table + occ + direct,
-                bytes_per_read=8.0 * c["w_tasks"] / n + ext_bytes * c["w_ext"] / n + 16.0 * occ
+                pair_per_read=c["w_pair"] / n,
+                bytes_per_read=8.0 * c["w_tasks"] / n + ext_bytes * c["w_ext"] / n + 1.0 * c["w_pair"] / n + 16.0 * occ
                 + 4.0 * (direct - text) + 16.0 * text)
 
 
@@ -180,19 +180,35 @@ def run_load(g, X, requests, rate, opts, exclusive, seed, label, growth=0, worke
     if errors:
         raise errors[0]
     lat = np.array([i.t_done - i.t_submit for i in infos])
+    sizes = np.array([int(i.n_reads) for i in infos])
+    # latency by request size: tenths of the requests ordered by read count
+    deciles = []
+    order = np.argsort(sizes, kind="stable")
+    for d in range(10):
+        sel = order[d * len(order) // 10:(d + 1) * len(order) // 10]
+        if len(sel):
+            deciles.append({"reads_min": int(sizes[sel].min()), "reads_max": int(sizes[sel].max()),
+                            "p50_ms": float(np.percentile(lat[sel], 50) * 1e3),
+                            "p99_ms": float(np.percentile(lat[sel], 99) * 1e3)})
     reads = sum(int(i.n_reads) for i in infos)
     span = max(i.t_done for i in infos) - min(i.t_submit for i in infos)
     out = dict(requests=len(infos), reads=reads, seconds=span, reads_per_s=reads / span,
                requests_per_s=len(infos) / span, p50_ms=float(np.percentile(lat, 50) * 1e3),
                p99_ms=float(np.percentile(lat, 99) * 1e3), max_ms=float(lat.max() * 1e3),
+               by_size_decile=deciles,
                rounds=int(st.n_rounds), device_busy=st.busy_seconds / max(span, 1e-9),
-               round_ms={"load": st.t_load / max(st.n_rounds, 1) * 1e3, "run": st.t_run / max(st.n_rounds, 1) * 1e3,
+               round_ms={"plan": st.t_plan / max(st.n_rounds, 1) * 1e3,
+                         "load": st.t_load / max(st.n_rounds, 1) * 1e3, "run": st.t_run / max(st.n_rounds, 1) * 1e3,
                          "pipeline": st.device_seconds / max(st.n_rounds, 1) * 1e3,
+                         "seed": st.t_seed / max(st.n_rounds, 1) * 1e3,
+                         "post": st.t_post / max(st.n_rounds, 1) * 1e3,
+                         "locate": st.t_locate / max(st.n_rounds, 1) * 1e3,
                          "fetch": st.t_fetch / max(st.n_rounds, 1) * 1e3,
-                         "split": st.t_split / max(st.n_rounds, 1) * 1e3})
-    log("%s: %d requests, %.0f reads/s, p50 %.1f ms, p99 %.1f ms, %d rounds; per round ms %s" %
+                         "split": st.t_split / max(st.n_rounds, 1) * 1e3},
+               worker_idle=st.t_idle / max(span, 1e-9), bases=int(st.n_bases))
+    log("%s: %d requests, %.0f reads/s, p50 %.1f ms, p99 %.1f ms, %d rounds; per round ms %s; workers idle %.2f" %
         (label, out["requests"], out["reads_per_s"], out["p50_ms"], out["p99_ms"], out["rounds"],
-         " ".join("%s %.1f" % kv for kv in out["round_ms"].items())))
+         " ".join("%s %.1f" % kv for kv in out["round_ms"].items()), out["worker_idle"]))
     return out
 
 
@@ -258,13 +274,30 @@ def profiled_traffic(kv, reads):
 
 
 def sample_arrays(arrays, n):
+    """The first n reads."""
     buf, offs, lens = arrays
     n = min(n, len(lens))
     end = int(offs[n - 1] + lens[n - 1])
     return buf[:end], offs[:n], lens[:n]
 
 
-def cli_end_to_end(arrays, prefix, tmpdir, n_reads, k, max_occ, kmer_len, direct, smem=True):
+def strided_sample(arrays, n):
+    """n reads spread evenly over the whole batch (every len/n-th read),
+    packed into arrays of their own."""
+    import numpy as np
+    buf, offs, lens = arrays
+    if n >= len(lens):
+        return arrays
+    idx = (np.arange(n, dtype=np.int64) * len(lens)) // n
+    L = lens[idx].astype(np.int64)
+    starts = offs[idx].astype(np.int64)
+    new_offs = np.zeros(n, dtype=np.uint64)
+    new_offs[1:] = np.cumsum(L[:-1])
+    src = np.repeat(starts - new_offs.astype(np.int64), L) + np.arange(int(L.sum()), dtype=np.int64)
+    return buf[src], new_offs, lens[idx].copy()
+
+
+def cli_end_to_end(arrays, prefix, tmpdir, n_reads, k, max_occ, kmer_len, direct, smem=True, pairs=0):
     """The drop-in command line, seed_gen, on a FASTA file of the first
     n_reads reads: index files loaded and structures built by the process
     itself, reads parsed from the file, seeds formatted to /dev/null with -o
@@ -283,24 +316,35 @@ def cli_end_to_end(arrays, prefix, tmpdir, n_reads, k, max_occ, kmer_len, direct
             f.write(b"\n")
     exe = os.path.join(ROOT, "gpuseed_amd", "bin", "seed_gen")
     out = {"reads": int(len(lens)), "bases": int(lens.sum()),
-           "command": "seed_gen %s-k %d --max-occ %d --kmer-len %d%s [-o] <prefix> reads.fa"
-                      % ("-s " if smem else "", k, max_occ, kmer_len, " --direct" if direct else "")}
-    for label, extra in (("with_o", ["-o"]), ("without_o", [])):
-        cmd = [exe] + (["-s"] if smem else []) + ["-k", str(k), "--max-occ", str(max_occ), "--kmer-len",
-                                                  str(kmer_len)] + (["--direct"] if direct else []) + extra
-        cmd += [prefix, fa]
-        t = time.perf_counter()
-        p = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
-        wall = time.perf_counter() - t
-        if p.returncode != 0:
-            raise RuntimeError("seed_gen failed (%d): %s" % (p.returncode, p.stderr[-2000:]))
-        err = p.stderr
-        m = re.search(r"index ready after ([0-9.]+) s; seeding ([0-9.]+) s", err)
-        t_index, t_seed = (float(m.group(1)), float(m.group(2))) if m else (None, None)
-        r = {"wall_s": wall, "reads_per_s_wall": len(lens) / wall, "index_ready_s": t_index,
-             "seeding_s": t_seed, "reads_per_s_after_index": len(lens) / t_seed if t_seed else None}
-        out[label] = r
-        log("seed_gen %s: %.2f s wall (index ready after %s s, seeding %s s)" % (label, wall, t_index, t_seed))
+           "command": "seed_gen %s-k %d --max-occ %d --kmer-len %d%s%s [-o] <prefix> reads.fa"
+                      % ("-s " if smem else "", k, max_occ, kmer_len, " --direct" if direct else "",
+                         " --pairs" if pairs else "")}
+    # each form twice, alternating; the faster run of each is kept (a run
+    # right after another process released its device memory can be slower)
+    for rep in range(2):
+        for label, extra in (("with_o", ["-o"]), ("without_o", [])):
+            cmd = [exe] + (["-s"] if smem else []) + ["-k", str(k), "--max-occ", str(max_occ), "--kmer-len",
+                                                      str(kmer_len)] + (["--direct"] if direct else []) + \
+                (["--pairs"] if pairs else []) + extra
+            cmd += [prefix, fa]
+            t = time.perf_counter()
+            p = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+            wall = time.perf_counter() - t
+            if p.returncode != 0:
+                raise RuntimeError("seed_gen failed (%d): %s" % (p.returncode, p.stderr[-2000:]))
+            err = p.stderr
+            m = re.search(r"index ready after ([0-9.]+) s; seeding ([0-9.]+) s", err)
+            t_index, t_seed = (float(m.group(1)), float(m.group(2))) if m else (None, None)
+            m = re.search(r"time: total [0-9.]+ s, input ([0-9.]+) s, device ([0-9.]+) s \(kernels ([0-9.]+) s\), "
+                          r"output ([0-9.]+) s", err)
+            parts = dict(zip(("input_s", "device_s", "kernels_s", "output_s"), map(float, m.groups()))) if m else {}
+            r = {"wall_s": wall, "reads_per_s_wall": len(lens) / wall, "index_ready_s": t_index,
+                 "seeding_s": t_seed, "reads_per_s_after_index": len(lens) / t_seed if t_seed else None}
+            r.update(parts)
+            log("seed_gen %s: %.2f s wall (index ready after %s s, seeding %s s: %s)" %
+                (label, wall, t_index, t_seed, " ".join("%s %.3f" % kv for kv in parts.items())))
+            if label not in out or wall < out[label]["wall_s"]:
+                out[label] = r
     os.unlink(fa)
     return out
 
@@ -343,6 +387,8 @@ def main():
                          "(default 16; configuration 1, a 1 Mbp reference: 13)")
     ap.add_argument("--no-direct", action="store_true",
                     help="no direct-extension structures (full suffix array, inverse, text)")
+    ap.add_argument("--no-pairs", action="store_true",
+                    help="no pair table (one byte per 18-base pattern: the bases next to it in the text)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=3,
@@ -396,7 +442,7 @@ def main():
     # whole pipeline on its batch
     w = [qp.classes[i].weight for i in range(qp.n_classes)]
     mean_len = sum(wi * (qp.classes[i].len_min + qp.classes[i].len_max) / 2 for i, wi in enumerate(w)) / sum(w)
-    resident = max(1, min(n_batches, int(120e9 // ((36.0 * mean_len + 200) * args.batch))))
+    resident = max(1, min(n_batches, int((120e9 if args.no_pairs else 60e9) // ((36.0 * mean_len + 200) * args.batch))))
     qp.n_reads = resident * args.batch
 
     log("generating %.2f Gbp reference (config %d)" % (rp.length / 1e9, args.config))
@@ -424,6 +470,11 @@ def main():
         X.set_direct(True)
         info = X.info()
         log("direct-extension structures in %.2fs, %.2f GB on device" % (info.t_direct, info.device_bytes / 1e9))
+    pairs = 0 if args.no_pairs or args.config == 1 or info.kmer_len != 16 or k < 19 else 1
+    if pairs:
+        X.set_pairs(True)
+        info = X.info()
+        log("pair table in %.2fs, %.2f GB on device" % (info.t_pairs, info.device_bytes / 1e9))
     if args.serve_only:
         serving = serve_bench(g, X, ref, k, max_occ, args)
         X.close()
@@ -444,7 +495,8 @@ def main():
     log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (resident, args.batch, len(buf) / 1e9))
     batches = [batches[i % resident] for i in range(args.warmup + args.steps)]
 
-    shape = {"kmer_table": info.kmer_len, "direct_extension": bool(direct), "device_bytes": int(info.device_bytes)}
+    shape = {"kmer_table": info.kmer_len, "direct_extension": bool(direct), "pair_table": bool(pairs),
+             "device_bytes": int(info.device_bytes)}
     elapsed, stats = timed_steps(batches, args.warmup, args.kernel)
     reads = args.steps * args.batch
     value = reads / elapsed
@@ -489,6 +541,7 @@ def main():
     # region; the output must be the same bit for bit
     ref_shape = None
     if not args.no_reference_shape and (info.kmer_len > 13 or direct):
+        X.set_pairs(False)
         X.set_kmer_len(13)
         X.set_direct(False)
         rinfo = X.info()
@@ -515,12 +568,14 @@ def main():
             X.set_kmer_len(info.kmer_len)
         if direct:
             X.set_direct(True)
+        if pairs:
+            X.set_pairs(True)
 
     # the extension kernel's own counters on exactly the reads the work
     # model counts (no work splitting, which the model does not describe)
     sample_counts = None
     if not args.no_cpu:
-        sa = sample_arrays(first_arrays, args.work_sample)
+        sa = strided_sample(first_arrays, args.work_sample)
         sb = g.Batch(X, sa, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks, split_tasks=g.NO_SPLIT)
         st_s = sb.run(args.kernel)
         sb.close()
@@ -559,7 +614,8 @@ def main():
     wm = None
     if not args.no_cpu:
         wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem,
-                        direct_min=4 if direct else 0, prune_sat={1: 255, 2: 65535, 3: 15}.get(args.kernel, 0))
+                        direct_min=4 if direct else 0, prune_sat={1: 255, 2: 65535, 3: 15}.get(args.kernel, 0),
+                        pair_probe=pairs)
         log("work model: %.1f table + %.1f occ + %.1f direct requests per read (%.0f bytes)" %
             (wm["table_per_read"], wm["occ_per_read"], wm["direct_per_read"], wm["bytes_per_read"]))
     serving = None
@@ -569,7 +625,8 @@ def main():
     X.close()
     cli = None
     if args.cli_reads:
-        cli = cli_end_to_end(first_arrays, prefix, tmpdir, args.cli_reads, k, max_occ, info.kmer_len, direct, smem)
+        cli = cli_end_to_end(first_arrays, prefix, tmpdir, args.cli_reads, k, max_occ, info.kmer_len, direct, smem,
+                             pairs)
     shutil.rmtree(tmpdir, ignore_errors=True)
 
     roofline = None
@@ -635,7 +692,7 @@ def main():
                                   rp.length / 1e9, max_occ),
                    "reference_bp": int(rp.length), "reads_per_step": args.batch, "k": k,
                    "mode": "SMEM" if smem else "MEM", "max_occ": max_occ, "sa_intv": 7, "kmer_table": info.kmer_len,
-                   "direct_extension": bool(direct), "device_bytes": shape["device_bytes"],
+                   "direct_extension": bool(direct), "pair_table": bool(pairs), "device_bytes": shape["device_bytes"],
                    "kernel": "batched-%d" % args.kernel},
         "roofline": roofline,
         "cpu_baseline": None if not cpu else {

@@ -58,7 +58,7 @@ class IndexInfo(C.Structure):
                 ("t_read", C.c_double), ("t_upload", C.c_double), ("t_repack", C.c_double),
                 ("t_table", C.c_double), ("t_sa", C.c_double), ("t_write", C.c_double),
                 ("sa_rounds", C.c_int), ("kmer_len", C.c_int), ("t_ext", C.c_double),
-                ("direct", C.c_int), ("t_direct", C.c_double)]
+                ("direct", C.c_int), ("t_direct", C.c_double), ("pairs", C.c_int), ("t_pairs", C.c_double)]
 
 
 class Opts(C.Structure):
@@ -84,7 +84,8 @@ class ServerStats(C.Structure):
     _fields_ = [("n_requests", C.c_uint64), ("n_rounds", C.c_uint64), ("n_reads", C.c_uint64),
                 ("n_bases", C.c_uint64), ("busy_seconds", C.c_double), ("device_seconds", C.c_double),
                 ("t_load", C.c_double), ("t_run", C.c_double), ("t_fetch", C.c_double), ("t_split", C.c_double),
-                ("peak_round_bytes", C.c_uint64), ("elapsed", C.c_double)]
+                ("peak_round_bytes", C.c_uint64), ("elapsed", C.c_double), ("t_plan", C.c_double),
+                ("t_idle", C.c_double), ("t_seed", C.c_double), ("t_post", C.c_double), ("t_locate", C.c_double)]
 
 
 class TuneResult(C.Structure):
@@ -149,6 +150,8 @@ def lib():
         L.gs_index_read_kext.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_set_kmer_len.argtypes = [C.c_void_p, C.c_int]
         L.gs_index_set_direct.argtypes = [C.c_void_p, C.c_int]
+        L.gs_index_set_pairs.argtypes = [C.c_void_p, C.c_int]
+        L.gs_index_read_pairs.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint8)]
         L.gs_index_locate.argtypes = [C.c_void_p, u32p, C.c_uint64, u32p]
         L.gs_index_read_direct.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p, u32p,
                                            C.POINTER(C.c_uint8)]
@@ -348,6 +351,16 @@ class Index:
         """Adds or drops the direct-extension structures (full suffix array,
         its inverse, 2-bit text)."""
         _check("gs_index_set_direct", lib().gs_index_set_direct(self.h, 1 if on else 0))
+
+    def set_pairs(self, on=True):
+        """Adds or drops the pair table (one byte per 18-base pattern: which
+        bases follow and which precede it in the text)."""
+        _check("gs_index_set_pairs", lib().gs_index_set_pairs(self.h, 1 if on else 0))
+
+    def read_pairs(self, first, count):
+        out = np.empty(count, dtype=np.uint8)
+        _check("gs_index_read_pairs", lib().gs_index_read_pairs(self.h, first, count, _ptr(out, C.c_uint8)))
+        return out
 
     def read_direct(self, first, count):
         """(suffix array rows, inverse positions, text codes) over [first, first+count)."""

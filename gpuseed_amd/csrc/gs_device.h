@@ -19,6 +19,7 @@ namespace gs {
 constexpr uint32_t kBlockBases = 96;
 constexpr int kPrecalc = 13;  // pre_calc_seed_len (seed_gen.cu:2822)
 constexpr uint32_t kMaxQuirk = 64;  // rows of the primary+1 walk kept for locating from the full suffix array
+constexpr int kPairCore = 18;       // bases of a pair-table core: its bytes answer 19-base windows
 
 // floor(i / 96) for any 32-bit i: (i >> 5) / 3 via the exact multiply-high
 // reciprocal of 3.
@@ -53,6 +54,11 @@ struct DevIndex {
   // with the position that walk reports (at most kMaxQuirk; none if the
   // list did not fit, and then positions come from the walk)
   const uint2* quirk;
+  // optional pair table (4^18 bytes), else null: for every 18-base core, bit
+  // c (0..3) set iff the core followed by base c occurs in the text, bit
+  // 4 + c iff base c followed by the core occurs; core index as the interval
+  // tables' (sum of code_j * 4^j, the core's last base as j = 0)
+  const uint8_t* pair;
   uint32_t n_quirk, locate_direct;
   uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)

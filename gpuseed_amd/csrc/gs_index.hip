@@ -37,6 +37,7 @@ DevIndex gs_index::dev() const {
   d.isa = isa;
   d.text = reinterpret_cast<const uint4*>(text);
   d.quirk = quirk;
+  d.pair = pair;
   d.n_quirk = n_quirk;
   d.locate_direct = sa_full && n_quirk <= gs::kMaxQuirk ? 1u : 0u;
   d.primary = primary;
@@ -188,8 +189,8 @@ __global__ void direct_chains(const DevIndex X, uint64_t first, uint64_t end, ui
       bad |= 2u;
       break;
     }
-    sa_full[row] = pos;
-    isa[pos] = row;
+    if (sa_full) sa_full[row] = pos;  // both null: only the text is wanted
+    if (isa) isa[pos] = row;
     ++visited;
     if (pos == 0) {  // the suffix at position 0, preceded by $: row primary
       if (row != X.primary) bad |= 1u;
@@ -234,6 +235,23 @@ __global__ void pack_text(const uint8_t* __restrict__ text8, uint64_t n, uint32_
     if (pos < n) v |= (uint32_t)(text8[pos] & 3u) << (2 * q);
   }
   text[w] = v;
+}
+
+// Pair table: the 18-base core at text position p (its index: the sum of
+// text[p + 17 - j] * 4^j) gets bit text[p + 18] (a base that follows it) and
+// bit 4 + text[p - 1] (a base that precedes it).  One thread per core; the
+// text at 2 bits per base, 16 per word.
+__global__ void pair_bits(const uint32_t* __restrict__ text, uint64_t n, uint64_t first, uint64_t end,
+                          uint32_t* __restrict__ table) {
+  const uint64_t p = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (p >= end || p + gs::kPairCore > n) return;
+  auto base = [&](uint64_t i) -> uint32_t { return (text[i >> 4] >> (2 * (i & 15))) & 3u; };
+  uint64_t idx = 0;
+  for (int j = 0; j < gs::kPairCore; ++j) idx |= (uint64_t)base(p + gs::kPairCore - 1 - j) << (2 * j);
+  uint32_t bits = 0;
+  if (p + gs::kPairCore < n) bits |= 1u << base(p + gs::kPairCore);
+  if (p > 0) bits |= 16u << base(p - 1);
+  if (bits) atomicOr(table + (idx >> 2), bits << (8 * (idx & 3)));
 }
 
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -465,6 +483,88 @@ extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
   return rc;
 }
 
+// Pair table: one byte per 18-base core saying which bases follow and which
+// precede it in the text, so that one lookup answers whether two adjacent
+// 19-base windows of a read occur.  Built from the text of the
+// direct-extension structures when the index has them, else from a text
+// derived here by the same walk of the index.
+extern "C" GS_API int gs_index_set_pairs(gs_index* X, int on) {
+  if (!X) return GS_EINVAL;
+  if (X->servers.load() > 0) return GS_EINVAL;  // their workers may be reading the table
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  const uint64_t bytes = 1ull << (2 * gs::kPairCore);
+  if (X->pair) {
+    GS_HIP(hipDeviceSynchronize());  // batches run on streams of their own
+    (void)hipFree(X->pair);
+    X->pair = nullptr;
+    X->device_bytes -= bytes;
+  }
+  if (!on) return GS_OK;
+  const double t0 = wall();
+  const uint64_t n = X->seq_len, rows = n + 1;
+  const uint64_t words = 4 * ((n + 63) / 64 + 1);
+  const uint32_t* text = X->text;
+  uint32_t* own_text = nullptr;
+  uint8_t* text8 = nullptr;
+  uint32_t* d_err = nullptr;
+  int rc = GS_OK;
+  hipStream_t st = X->stream;
+  const uint64_t step = gs::launch_chunk();
+  if (!text) {
+    if (hipMalloc(&own_text, words * 4) != hipSuccess || hipMalloc(&text8, n + 1) != hipSuccess ||
+        hipMalloc(&d_err, 16) != hipSuccess)
+      rc = GS_ENOMEM;
+    uint32_t h_err[4] = {0, 0, 0, 0};
+    if (!rc && hipMemsetAsync(d_err, 0, 16, st) != hipSuccess) rc = GS_EHIP;
+    const DevIndex D = X->dev();
+    for (uint64_t first = 0; !rc && first < X->n_sa; first += step) {
+      const uint64_t m = X->n_sa - first < step ? X->n_sa - first : step;
+      hipLaunchKernelGGL(direct_chains, dim3(grid_for(m, 256)), dim3(256), 0, st, D, first, first + m,
+                         (uint32_t*)nullptr, (uint32_t*)nullptr, text8, d_err);
+    }
+    if (!rc)
+      hipLaunchKernelGGL(pack_text, dim3(grid_for(words, 256)), dim3(256), 0, st, text8, n, own_text, words);
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(h_err, d_err, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess))
+      rc = GS_EHIP;
+    const uint64_t visited = (uint64_t)h_err[2] | ((uint64_t)h_err[3] << 32);
+    if (!rc && (h_err[0] != 0 || visited != rows)) rc = GS_EFORMAT;
+    (void)hipFree(text8);
+    (void)hipFree(d_err);
+    text = own_text;
+  }
+  if (!rc && hipMalloc(&X->pair, bytes) != hipSuccess) {
+    X->pair = nullptr;
+    rc = GS_ENOMEM;
+  }
+  if (!rc && hipMemsetAsync(X->pair, 0, bytes, st) != hipSuccess) rc = GS_EHIP;
+  const uint64_t cores = n >= (uint64_t)gs::kPairCore ? n - gs::kPairCore + 1 : 0;
+  for (uint64_t first = 0; !rc && first < cores; first += step) {
+    const uint64_t m = cores - first < step ? cores - first : step;
+    hipLaunchKernelGGL(pair_bits, dim3(grid_for(m, 256)), dim3(256), 0, st, text, n, first, first + m,
+                       reinterpret_cast<uint32_t*>(X->pair));
+  }
+  if (!rc && (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)) rc = GS_EHIP;
+  (void)hipFree(own_text);
+  if (rc) {
+    (void)hipFree(X->pair);
+    X->pair = nullptr;
+  } else {
+    X->device_bytes += bytes;
+  }
+  X->t_pairs = wall() - t0;
+  return rc;
+}
+
+extern "C" GS_API int gs_index_read_pairs(const gs_index* X, uint64_t first, uint64_t count, uint8_t* out) {
+  const uint64_t entries = 1ull << (2 * gs::kPairCore);
+  if (!X || !out || !X->pair || first > entries || count > entries - first) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  GS_HIP(hipMemcpy(out, X->pair + first, count, hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
 extern "C" GS_API int gs_index_read_direct(const gs_index* X, uint32_t first, uint32_t count, uint32_t* sa_out,
                                            uint32_t* isa_out, uint8_t* text_out) {
   if (!X || !X->sa_full || (uint64_t)first + count > (uint64_t)X->seq_len + 1) return GS_EINVAL;
@@ -491,6 +591,7 @@ static void free_index(gs_index* X) {
   (void)hipFree(X->sa);
   (void)hipFree(X->kmer);
   (void)hipFree(X->kext);
+  (void)hipFree(X->pair);
   free_direct(X);
   if (X->stream) (void)hipStreamDestroy(X->stream);
   delete X;
@@ -634,6 +735,8 @@ extern "C" GS_API int gs_index_get_info(const gs_index* X, gs_index_info* info) 
   info->t_ext = X->t_ext;
   info->direct = X->sa_full ? 1 : 0;
   info->t_direct = X->t_direct;
+  info->pairs = X->pair ? 1 : 0;
+  info->t_pairs = X->t_pairs;
   return GS_OK;
 }
 

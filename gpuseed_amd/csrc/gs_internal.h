@@ -38,6 +38,8 @@ struct gs_index {
   // positions (locate from the full suffix array); n_quirk > kMaxQuirk: walk
   uint2* quirk = nullptr;
   uint32_t n_quirk = 0;
+  uint8_t* pair = nullptr;  // pair table (gs_index_set_pairs), 4^18 bytes
+  double t_pairs = 0;
   uint64_t device_bytes = 0;
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0, t_direct = 0;
   int sa_rounds = 0;
@@ -59,6 +61,10 @@ void index_release(gs_index* X);  // frees the index with its last reference
 // count is a 32-bit quantity, so launches over more items than this are
 // split.  2^30 unless a test sets it smaller (gs_set_launch_chunk).
 uint64_t launch_chunk();
+// malloc for result arrays handed to callers (freed with free()); large ones
+// are advised to use transparent huge pages, so that filling them faults in
+// 2 MB pages instead of 4 KB ones
+void* result_alloc(size_t bytes);
 // Looks up the device address of gs_seed_noalloc's scratch array on `device`
 // (which must be current); called once per device where an index is set up.
 int noalloc_setup(int device);
@@ -154,14 +160,16 @@ int launch_join(const JoinParams& J, hipStream_t st);
 namespace gs {
 
 // The CPU side of a batch (see gs_batch_create): built on any thread, then
-// uploaded by batch_load.
+// uploaded by batch_load.  Per read only; the work items of the persistent
+// kernel (n_items of them, n_long reads cut into segments) are written on the
+// device when the batch is loaded.
 struct BatchPlan {
   gs_opts opts{};
   uint32_t seg_tasks = 64, maxlen = 0, next_batch_id = 0;
-  uint64_t n_reads = 0, span = 0, tasks = 0, code_chunks = 0;
+  uint64_t n_reads = 0, span = 0, tasks = 0, code_chunks = 0, n_items = 0, n_long = 0;
   std::vector<std::pair<const char*, uint64_t>> pieces;  // bytes uploaded back to back
-  std::vector<uint64_t> offs, task_off, code_off, items, item0;
-  std::vector<uint32_t> lens, rbatch, long_reads;
+  std::vector<uint64_t> offs, task_off, code_off;
+  std::vector<uint32_t> lens, rbatch;
 };
 
 int plan_check(const gs_opts* o);

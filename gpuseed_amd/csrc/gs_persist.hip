@@ -109,12 +109,19 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
   uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0;
-  // the 19 codes left of the task end e: code(e-1-j) at bits 2j of w19 for
-  // j = 0..18, and bit j of bad19 set if that code is above 3 or its position
+  // the 20 codes left of the task end e: code(e-1-j) at bits 2j of w20 for
+  // j = 0..19, and bit j of bad20 set if that code is above 3 or its position
   // is left of the read start; the 13-mer pattern is the low 26 bits, an
-  // extended window and its 3 presence bases the bits above
-  uint64_t w19 = 0;
-  uint32_t bad19 = 0;
+  // extended window and its 3 presence bases the bits above, and code(e-20)
+  // (the base left of the 19-base window) the top two
+  uint64_t w20 = 0;
+  uint32_t bad20 = 0;
+  // pair table: probe is set while the last clean 19-base window looked up
+  // was absent; known_e is the task end whose window the last pair byte
+  // also answered, known_p whether that window occurs
+  const bool pair_on = P.X.pair && P.X.ext_len == 16 && k >= 19;
+  bool probe = false, known_p = false;
+  int32_t known_e = -1;
   uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
   int32_t e = 0, x = 0, e_min = 0, w0 = 0;
   uint32_t t0 = 0, first_b = 0, flags = 0, work = 0;
@@ -164,10 +171,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   };
   auto advance = [&]() __attribute__((always_inline)) -> bool {
     if (e - 1 < e_min) return false;
-    const int32_t in = e - 20;  // enters the window as position e' - 19 of the next end e' = e - 1
+    const int32_t in = e - 21;  // enters the window as position e' - 20 of the next end e' = e - 1
     const uint32_t cn = in >= 0 ? code(in) : 15u;
-    w19 = (w19 >> 2) | ((uint64_t)(cn & 3u) << 36);
-    bad19 = (bad19 >> 1) | ((cn > 3 ? 1u : 0u) << 18);
+    w20 = (w20 >> 2) | ((uint64_t)(cn & 3u) << 38);
+    bad20 = (bad20 >> 1) | ((cn > 3 ? 1u : 0u) << 19);
     --e;
     return true;
   };
@@ -203,12 +210,18 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
         __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
       split();
-    do {
+    while (true) {
       if (!advance()) {
         end_stream();
         return;
       }
-    } while (bad19 & kBad13);
+      if (bad20 & kBad13) continue;
+      if (known_e == e && !known_p) {  // the last pair byte said this window is absent: no lookup
+        probe = true;
+        continue;
+      }
+      break;
+    }
   };
   // size record at pos (positions left of the window keep none: pruning is
   // only an early exit, so a missing record never changes the output)
@@ -306,20 +319,22 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     link = link0;  // set before the item can end (a piece whose windows all hold an N ends here)
     e = (int32_t)(L - t0);
     e_min = e - (int32_t)tcount + 1;
-    w19 = 0;
-    bad19 = 0;
-    for (int j = 0; j < 19; ++j) {
+    w20 = 0;
+    bad20 = 0;
+    for (int j = 0; j < 20; ++j) {
       const int32_t pos = e - 1 - j;
       const uint32_t c = pos >= 0 ? code(pos) : 15u;
-      bad19 |= (c > 3 ? 1u : 0u) << j;
-      w19 |= (uint64_t)(c & 3u) << (2 * j);
+      bad20 |= (c > 3 ? 1u : 0u) << j;
+      w20 |= (uint64_t)(c & 3u) << (2 * j);
     }
     have = true;
     ++started;
     phase = 0;
     ext_miss = false;
     ones = 0;
-    if (bad19 & kBad13) next_task();  // no work done yet, so this never splits
+    probe = false;
+    known_e = -1;
+    if (bad20 & kBad13) next_task();  // no work done yet, so this never splits
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
     item = it;
@@ -455,18 +470,25 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const uint4* a2 = nullptr;
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
-    uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)w19 & 0x3FFFFFFu;  // table lookup: pattern length and entry
+    uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)w20 & 0x3FFFFFFu;  // table lookup: pattern length and entry
     uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
+    const uint8_t* w1 = nullptr;  // a 1-byte read: the pair table (op 1)
     if (op == 1) {
       a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (tidx >> 1);  // two 8-byte entries per load
       const uint32_t el = P.X.ext_len;
       // the extended window: codes 13 .. el-1 clean and inside the read
-      if (el && !ext_miss && !(bad19 & (((1u << el) - 1u) & ~kBad13))) {
+      if (el && !ext_miss && !(bad20 & (((1u << el) - 1u) & ~kBad13))) {
         tlen = el;
-        tidx = (uint32_t)(w19 & ((1ull << (2 * el)) - 1ull));
+        tidx = (uint32_t)(w20 & ((1ull << (2 * el)) - 1ull));
         a1 = P.X.kext + tidx;
         // presence bit c1 + 4*c2 + 16*c3 of the next three bases, if clean
-        if (k >= tlen + 3 && !(bad19 & (7u << el))) pbit = (uint32_t)(w19 >> (2 * el)) & 63u;
+        if (k >= tlen + 3 && !(bad20 & (7u << el))) pbit = (uint32_t)(w20 >> (2 * el)) & 63u;
+      }
+      // after an absent window, a task whose 20 codes are clean reads the
+      // pair byte of its 18-base core read[e-19, e-1) instead
+      if (pair_on && probe && known_e != e && !(bad20 & 0xFFFFFu)) {
+        a1 = nullptr;
+        w1 = P.X.pair + ((w20 >> 2) & ((1ull << 36) - 1ull));
       }
       ++n_table;
       ++work;
@@ -500,6 +522,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     if (a2) d2 = *a2;
     uint32_t v4 = 0;
     if (w4) v4 = *w4;
+    if (w1) v4 = *w1;
 
     // Text comparison of the lanes in phase 4, one such lane at a time with
     // the whole wave: lane i checks position x - i of that lane's read against
@@ -592,7 +615,18 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       nu = base + o2;
       nb = (uint32_t)x;
     }
-    if (op >= 3) {
+    if (w1) {
+      // the pair byte: bit code(e-1) says whether this task's window
+      // read[e-19, e) occurs, bit 4 + code(e-20) whether the next task's
+      // window read[e-20, e-1) does; an absent window ends the task as an
+      // absent presence bit would, a present one is looked up next
+      known_e = e - 1;
+      known_p = (v4 >> (4u + ((uint32_t)(w20 >> 38) & 3u))) & 1u;
+      if (!((v4 >> ((uint32_t)w20 & 3u)) & 1u))
+        next_task();
+      else
+        probe = false;
+    } else if (op >= 3) {
       int res = 0;
       if (op == 3) {  // the text position of read[b]
         tq = v4;
@@ -657,9 +691,12 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
           ext_miss = true;
         else
           res = op == 2 ? 2 : 1;
+        if (pair_on && op == 1 && pbit < 64) probe = true;  // an absent clean 19-base window
       } else if (absent) {
         res = 1;  // the same holds for an absent clean (tlen + 3)-base window with k >= tlen + 3
+        if (pair_on) probe = true;
       } else {
+        if (pair_on && op == 1 && pbit < 64) probe = false;  // a present clean 19-base window
         if (op == 2) ones = nl == nu ? ones + 1u : 0u;
         l = nl;
         u = nu;

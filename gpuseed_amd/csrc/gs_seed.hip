@@ -30,6 +30,7 @@
 //    the same reason an SMEM stream ends when a task reaches the read start:
 //    every later task would end there too and be filtered out.
 #include <stdlib.h>
+#include <sys/mman.h>
 #include <string.h>
 
 #include <algorithm>
@@ -681,6 +682,57 @@ __global__ void zero_bytes(uint8_t* __restrict__ p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// ---- work items of the persistent kernel, written at load time ---------------
+// Whole-read streams (reads held whole in the LDS window) come first, two
+// items per read (forward, reverse) in read order; then every read cut into
+// segments, segment-major: items (read | seg << 33) and (read | 1 << 32 |
+// seg << 33) for seg = 0..nseg-1, with the read in long_reads and its first
+// item in item0.  A whole-read stream holds up to ~150 task ends and a
+// segment at most seg_tasks, so the last items a persistent kernel hands
+// out are the small ones.
+
+__device__ __forceinline__ bool read_is_long(uint32_t L, uint32_t k, uint32_t nseg) {
+  return nseg > 1 || (L > gs::kPersistLmax && L >= k);
+}
+
+// per read: items as a whole-read stream, items as segments, 1 if segmented
+__global__ void item_counts(const uint32_t* __restrict__ lens, uint32_t n, uint32_t k, uint32_t seg_tasks,
+                            uint32_t* __restrict__ c_whole, uint32_t* __restrict__ c_seg,
+                            uint32_t* __restrict__ c_long) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t L = lens[i];
+  const uint32_t nseg = gs::n_segments(L, k, seg_tasks);
+  const bool lng = read_is_long(L, k, nseg);
+  c_whole[i] = lng ? 0u : 2u;
+  c_seg[i] = lng ? 2u * nseg : 0u;
+  c_long[i] = lng ? 1u : 0u;
+}
+
+// o_whole, o_seg, o_long: exclusive scans of the counts (n + 1 entries)
+__global__ void item_fill(const uint32_t* __restrict__ lens, uint32_t n, uint32_t k, uint32_t seg_tasks,
+                          const uint64_t* __restrict__ o_whole, const uint64_t* __restrict__ o_seg,
+                          const uint64_t* __restrict__ o_long, uint64_t* __restrict__ items,
+                          uint32_t* __restrict__ long_reads, uint64_t* __restrict__ item0) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t L = lens[i];
+  const uint32_t nseg = gs::n_segments(L, k, seg_tasks);
+  if (!read_is_long(L, k, nseg)) {
+    const uint64_t p = o_whole[i];
+    items[p] = i;
+    items[p + 1] = i | (1ull << 32);
+    return;
+  }
+  const uint64_t j = o_long[i], first = o_whole[n] + o_seg[i];
+  long_reads[j] = i;
+  item0[j] = first;
+  for (uint64_t g = 0; g < nseg; ++g) {
+    items[first + 2 * g] = i | (g << 33);
+    items[first + 2 * g + 1] = i | (1ull << 32) | (g << 33);
+  }
+}
+
 // ---- device buffers of one gs_seed call -------------------------------------
 struct DevBuf {
   void* p = nullptr;
@@ -741,7 +793,7 @@ struct gs_batch {
   uint32_t maxlen = 0;
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
   DevBuf codes, code_off, queue, counters;  // persistent kernel
-  DevBuf items, seg_meta, long_reads, item0;
+  DevBuf items, seg_meta, long_reads, item0, item_tmp;  // item_tmp: counts and scans of the item plan
   DevBuf seg_link, dyn_items, dyn_meta, dyn_link, dyn_ready, dyn_ctl, wmeta, wlink;  // work splitting
   uint64_t code_chunks = 0, n_items = 0;
   uint32_t n_long = 0, seg_tasks = 64, dyn_cap = 0;
@@ -792,6 +844,19 @@ struct gs_batch {
 
 static void free_graph(gs_batch* B);
 static void free_host_results(gs_batch* B);
+
+void* gs::result_alloc(size_t bytes) {
+  void* p = malloc(bytes);
+  constexpr size_t kPage = 4096, kHuge = 2u << 20;
+  if (p && bytes >= 2 * kHuge) {
+    const uintptr_t a = ((uintptr_t)p + kPage - 1) & ~(uintptr_t)(kPage - 1);
+    const uintptr_t z = ((uintptr_t)p + bytes) & ~(uintptr_t)(kPage - 1);
+    if (z > a) (void)madvise((void*)a, z - a, MADV_HUGEPAGE);  // advice only: failure changes nothing
+  }
+  return p;
+}
+
+constexpr size_t kStageChunk = 64u << 20;  // pinned chunks of staged uploads and downloads
 
 static void destroy_batch(gs_batch* B) {
   if (!B) return;
@@ -853,7 +918,6 @@ int gs::plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const
   grow(P.task_off, nr + 1);
   grow(P.rbatch, nr);
   grow(P.code_off, nr + 1);
-  grow(P.items, 2 * nr);  // one item per strand of every read that fits the LDS window
   const uint32_t k = (uint32_t)P.opts.k;
   const uint32_t batch_bases = P.opts.batch_bases ? P.opts.batch_bases : 1000000u;
   uint64_t lo = UINT64_MAX, hi = 0;
@@ -881,50 +945,31 @@ int gs::plan_add(BatchPlan& P, const char* bases, const uint64_t* offsets, const
       acc = 0;
     }
     if (L > P.maxlen) P.maxlen = L;
-    const uint64_t ri = P.n_reads++;
+    ++P.n_reads;
     P.code_off.push_back(P.code_chunks);
     P.code_chunks += 2 * ((L + 31) / 32);
     const uint32_t nseg = gs::n_segments(L, k, P.seg_tasks);
-    if (nseg > 1 || (L > gs::kPersistLmax && L >= k)) {
-      P.long_reads.push_back((uint32_t)ri);
-      P.item0.push_back(P.items.size());
-    }
-    for (uint64_t g = 0; g < nseg; ++g) {
-      P.items.push_back(ri | (g << 33));
-      P.items.push_back(ri | (1ull << 32) | (g << 33));
-    }
+    if (nseg > 1 || (L > gs::kPersistLmax && L >= k)) ++P.n_long;
+    P.n_items += 2ull * nseg;
   }
   P.next_batch_id = bid + 1;  // the next group starts a batch of its own
-  if (P.items.size() > 0xF0000000ull) return GS_ERANGE;
+  if (P.n_items > 0xF0000000ull) return GS_ERANGE;
   return GS_OK;
 }
 
-// Work items in dispatch order: whole-read streams (up to ~150 task ends each)
-// before the segments of long reads (at most seg_tasks each), so that the
-// last items a persistent kernel hands out are the small ones.
+// Closes the per-read offset arrays; the work items (whole-read streams
+// first, then the segments of long reads, so that the last items a
+// persistent kernel hands out are small) are written on the device by
+// batch_load.
 void gs::plan_finish(BatchPlan& P) {
   P.task_off.push_back(P.tasks);
   P.code_off.push_back(P.code_chunks);
-  if (P.long_reads.empty() || P.long_reads.size() == P.n_reads) return;
-  std::vector<uint8_t> is_long(P.n_reads, 0);
-  for (uint32_t r : P.long_reads) is_long[r] = 1;
-  std::vector<uint64_t> items;
-  items.reserve(P.items.size());
-  for (uint64_t v : P.items)
-    if (!is_long[(uint32_t)v]) items.push_back(v);
-  for (size_t i = 0; i < P.long_reads.size(); ++i) {
-    const uint64_t a = P.item0[i];
-    const uint64_t n = 2ull * gs::n_segments(P.lens[P.long_reads[i]], (uint32_t)P.opts.k, P.seg_tasks);
-    P.item0[i] = items.size();
-    items.insert(items.end(), P.items.begin() + a, P.items.begin() + a + n);
-  }
-  P.items.swap(items);
 }
 
 uint64_t gs::plan_device_bytes(const BatchPlan& P) {
   const uint64_t nr1 = P.n_reads + 1;
-  return (P.span + 32) + 16 * (P.code_chunks + 1) + 16 * (2 * P.tasks + 1) + 8 * P.items.size() +
-         (P.long_reads.empty() ? 0 : 16 * P.items.size()) + 12 * P.long_reads.size() + nr1 * (8 * 5 + 4 * 7) +
+  return (P.span + 32) + 16 * (P.code_chunks + 1) + 16 * (2 * P.tasks + 1) + 8 * P.n_items +
+         (P.n_long ? 16 * P.n_items : 0) + 12 * P.n_long + nr1 * (8 * 5 + 4 * 7 + 36) +
          (P.maxlen > 256 ? 4 * (P.span + 16) : 0);
 }
 
@@ -945,6 +990,7 @@ static int ensure_buffers(gs_batch* B, uint64_t span, uint64_t n, uint64_t tasks
   const uint64_t nr1 = n + 1;
   if (B->codes.ensure(16 * (code_chunks + 1)) || B->code_off.ensure(8 * nr1) ||
       B->items.ensure(8 * (n_items + 1)) || B->long_reads.ensure(4 * (n_long + 1)) ||
+      B->item_tmp.ensure(36 * nr1 + 8) ||
       B->item0.ensure(8 * (n_long + 1)) || B->seg_meta.ensure(n_long ? 16 * n_items : 16) ||
       B->bases.ensure(span + 32) || B->offs.ensure(8 * nr1) || B->lens.ensure(4 * nr1) ||
       B->toff.ensure(8 * nr1) || B->batch.ensure(4 * nr1) || B->cand.ensure(16 * (2 * tasks + 1)) ||
@@ -969,7 +1015,81 @@ int gs::batch_reserve(gs_batch* B, uint64_t bases, uint64_t reads, uint32_t seg_
   // at most as many tasks as bases and code chunks as 32-base spans (+1)
   const uint64_t items = 2 * (reads + bases / seg_tasks + 1);
   const uint32_t dyn = (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, items));
-  return ensure_buffers(B, bases, reads, bases, 2 * (bases / 32 + reads), items, reads, 0xFFFFFFFFu, dyn);
+  int rc = ensure_buffers(B, bases, reads, bases, 2 * (bases / 32 + reads), items, reads, 0xFFFFFFFFu, dyn);
+  if (!rc && bases > (4u << 20)) {  // the pinned chunks of staged uploads
+    const size_t ch = std::min<uint64_t>(bases, kStageChunk);
+    if (B->st_a.ensure(ch) || B->st_b.ensure(ch)) rc = GS_ENOMEM;
+  }
+  return rc;
+}
+
+// Copies len bytes to dst by calling part(a, z) for disjoint ranges
+// [a, z) of [0, len) on up to 8 threads (the calling thread takes the first
+// part, and every part no thread could be started for).
+template <typename F>
+static void parallel_parts(size_t len, F&& part) {
+  constexpr size_t kThreads = 8, kMinPart = 1u << 20;
+  const size_t nt = std::max<size_t>(1, std::min(kThreads, len / kMinPart));
+  const size_t each = (len + nt - 1) / nt;
+  std::vector<std::thread> th;
+  size_t done = std::min(len, each);  // bytes covered by this thread or started helpers
+  try {
+    for (size_t t = 1; t < nt; ++t) {
+      const size_t a = std::min(len, t * each), z = std::min(len, a + each);
+      th.emplace_back([=, &part] { part(a, z); });
+      done = z;
+    }
+  } catch (...) {  // no thread to spare: this one copies the rest
+  }
+  part(0, std::min(len, each));
+  if (done < len) part(done, len);
+  for (auto& t : th) t.join();
+}
+
+// The batch's read bytes (the plan's pieces, back to back) to dst through two
+// pinned chunks: the bytes of chunk c+1 are gathered on several threads while
+// chunk c's DMA runs.  Pageable sources otherwise go through the runtime's
+// own staging, one thread at a fraction of the link's rate.
+static int staged_upload(gs_batch* B, char* dst, const std::vector<std::pair<const char*, uint64_t>>& pieces,
+                         uint64_t total) {
+  constexpr size_t kChunk = kStageChunk;
+  hipStream_t st = B->stream;
+  if (total <= (4u << 20)) {  // small: the pieces directly
+    uint64_t at = 0;
+    for (const auto& pc : pieces) {
+      if (hipMemcpyAsync(dst + at, pc.first, pc.second, hipMemcpyHostToDevice, st) != hipSuccess) return GS_EHIP;
+      at += pc.second;
+    }
+    return GS_OK;
+  }
+  const size_t ch = std::min<uint64_t>(total, kChunk);
+  if (B->st_a.ensure(ch) || B->st_b.ensure(ch)) return GS_ENOMEM;
+  for (auto& e : B->cev)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return GS_EHIP;
+  std::vector<uint64_t> start(pieces.size() + 1, 0);  // offset of each piece in the batch
+  for (size_t i = 0; i < pieces.size(); ++i) start[i + 1] = start[i] + pieces[i].second;
+  char* stage[2] = {static_cast<char*>(B->st_a.p), static_cast<char*>(B->st_b.p)};
+  const size_t nch = (total + ch - 1) / ch;
+  for (size_t c = 0; c < nch; ++c) {
+    // the buffer of chunk c held chunk c-2, whose DMA must be done
+    if (c >= 2 && hipEventSynchronize(B->cev[c & 1]) != hipSuccess) return GS_EHIP;
+    const uint64_t off = c * ch, len = std::min<uint64_t>(ch, total - off);
+    char* sp = stage[c & 1];
+    parallel_parts(len, [&](size_t a, size_t z) {
+      uint64_t g = off + a;  // batch offset of the next byte to copy
+      size_t i = std::upper_bound(start.begin(), start.end(), g) - start.begin() - 1;
+      while (g < off + z) {
+        const uint64_t in = g - start[i], n = std::min(pieces[i].second - in, off + z - g);
+        memcpy(sp + (g - off), pieces[i].first + in, n);
+        g += n;
+        ++i;
+      }
+    });
+    if (hipMemcpyAsync(dst + off, sp, len, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventRecord(B->cev[c & 1], st) != hipSuccess)
+      return GS_EHIP;
+  }
+  return GS_OK;
 }
 
 // Uploads a finished plan into B, reusing B's device buffers where they are
@@ -986,8 +1106,8 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   B->tasks = P.tasks;
   B->maxlen = P.maxlen;
   B->code_chunks = P.code_chunks;
-  B->n_items = P.items.size();
-  B->n_long = (uint32_t)P.long_reads.size();
+  B->n_items = P.n_items;
+  B->n_long = (uint32_t)P.n_long;
   B->seg_tasks = P.seg_tasks;
   B->n_seeds = B->n_pos = 0;
   B->ran = false;
@@ -1010,25 +1130,31 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
     return GS_ENOMEM;
   hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
-  uint64_t at = 0;
-  for (const auto& pc : P.pieces) {
-    if (hipMemcpyAsync(B->bases.as<char>() + at, pc.first, pc.second, hipMemcpyHostToDevice, st) != hipSuccess)
-      return GS_EHIP;
-    at += pc.second;
-  }
+  if ((rc = staged_upload(B, B->bases.as<char>(), P.pieces, P.span))) return rc;
   hipLaunchKernelGGL(zero_bytes, dim3(1), dim3(32), 0, st, B->bases.as<uint8_t>() + B->span, 32u);
   if (n &&
       (hipMemcpyAsync(B->offs.p, P.offs.data(), 8 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
        hipMemcpyAsync(B->lens.p, P.lens.data(), 4 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
        hipMemcpyAsync(B->toff.p, P.task_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
        hipMemcpyAsync(B->batch.p, P.rbatch.data(), 4 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
-       hipMemcpyAsync(B->code_off.p, P.code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess ||
-       hipMemcpyAsync(B->items.p, P.items.data(), 8 * B->n_items, hipMemcpyHostToDevice, st) != hipSuccess ||
-       (B->n_long &&
-        (hipMemcpyAsync(B->long_reads.p, P.long_reads.data(), 4 * B->n_long, hipMemcpyHostToDevice, st) !=
-             hipSuccess ||
-         hipMemcpyAsync(B->item0.p, P.item0.data(), 8 * B->n_long, hipMemcpyHostToDevice, st) != hipSuccess))))
+       hipMemcpyAsync(B->code_off.p, P.code_off.data(), 8 * nr1, hipMemcpyHostToDevice, st) != hipSuccess))
     return GS_EHIP;
+  if (n) {  // the work items, from the lengths just uploaded
+    uint32_t* cw = B->item_tmp.as<uint32_t>();
+    uint32_t *cs = cw + nr1, *cl = cw + 2 * nr1;
+    uint64_t* ow = reinterpret_cast<uint64_t*>(B->item_tmp.as<uint8_t>() + ((12 * nr1 + 7) & ~7ull));
+    uint64_t *os = ow + nr1, *ol = ow + 2 * nr1;
+    const uint32_t k = (uint32_t)P.opts.k;
+    hipLaunchKernelGGL(item_counts, dim3(grid_for(n, 256)), dim3(256), 0, st, B->lens.as<uint32_t>(), (uint32_t)n,
+                       k, B->seg_tasks, cw, cs, cl);
+    scan_counts(cw, n, ow, B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
+    scan_counts(cs, n, os, B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
+    scan_counts(cl, n, ol, B->tiles.as<uint64_t>(), B->tot.as<uint64_t>(), st);
+    hipLaunchKernelGGL(item_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, B->lens.as<uint32_t>(), (uint32_t)n, k,
+                       B->seg_tasks, ow, os, ol, B->items.as<uint64_t>(), B->long_reads.as<uint32_t>(),
+                       B->item0.as<uint64_t>());
+    if (hipGetLastError() != hipSuccess) return GS_EHIP;
+  }
   if (hipEventRecord(B->ev[1], st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return GS_EHIP;
   float ms = 0;
   (void)hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
@@ -1425,7 +1551,7 @@ extern "C" GS_API int gs_batch_replay(gs_batch* B, gs_batch_stats* stats) {
 // threads when it is large.  Pageable destinations otherwise go through the
 // runtime's own staging at a fraction of the link's rate.
 static int staged_d2h(gs_batch* B, void* dst, const void* src, size_t bytes) {
-  constexpr size_t kChunk = 64u << 20;
+  constexpr size_t kChunk = kStageChunk;
   hipStream_t st = B->stream;
   if (bytes <= (4u << 20))  // small: one plain copy
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -1480,12 +1606,12 @@ extern "C" GS_API int gs_batch_fetch(gs_batch* B, gs_result* out) {
   out->n_reads = n;
   out->n_seeds = B->n_seeds;
   out->n_pos = B->n_pos;
-  out->read_nseeds = (uint32_t*)malloc(4 * nr1);
-  out->read_npos = (uint32_t*)malloc(4 * nr1);
-  out->read_seed_off = (uint64_t*)malloc(8 * nr1);
-  out->read_pos_off = (uint64_t*)malloc(8 * nr1);
-  out->seeds = (gs_seed_rec*)malloc(sizeof(gs_seed_rec) * (B->n_seeds + 1));
-  out->pos = (uint32_t*)malloc(4 * (B->n_pos + 1));
+  out->read_nseeds = (uint32_t*)gs::result_alloc(4 * nr1);
+  out->read_npos = (uint32_t*)gs::result_alloc(4 * nr1);
+  out->read_seed_off = (uint64_t*)gs::result_alloc(8 * nr1);
+  out->read_pos_off = (uint64_t*)gs::result_alloc(8 * nr1);
+  out->seeds = (gs_seed_rec*)gs::result_alloc(sizeof(gs_seed_rec) * (B->n_seeds + 1));
+  out->pos = (uint32_t*)gs::result_alloc(4 * (B->n_pos + 1));
   if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
       !out->pos) {
     gs_result_free(out);

@@ -27,6 +27,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -53,10 +54,12 @@ struct Request {
   uint64_t next_read = 0;      // first read not yet scheduled
   uint64_t sched_bases = 0;    // bases of the reads before next_read
   uint64_t done_reads = 0;
-  struct Piece {  // results of one chunk, written by the worker that runs it
-    std::vector<uint32_t> nseeds, npos;
-    std::vector<gs_seed_rec> seeds;
-    std::vector<uint32_t> pos;
+  // results of one chunk, written by the worker that runs it, in the layout
+  // of a gs_result (offsets relative to the chunk): a request of one chunk
+  // hands these arrays to its caller as they are
+  struct Piece {
+    gs_result r{};
+    ~Piece() { gs_result_free(&r); }
   };
   // by chunk ordinal, i.e. in read order; each piece is its own allocation so
   // that a worker filling one never races with the planner adding another
@@ -233,6 +236,7 @@ void planner_main(gs_server* S) {
     if (S->active.empty() && S->stop) break;
     if (S->ready || S->active.empty()) continue;
     Request* front = S->active.front();
+    const double tp0 = S->now();
     std::unique_ptr<Round> R = plan_round(S);
     if (R->chunks.empty()) {  // nothing could be planned for the front request
       if (!front->done) finish_request(S, front, GS_ERANGE);
@@ -242,6 +246,7 @@ void planner_main(gs_server* S) {
     lk.unlock();
     build_round(S, *R);
     lk.lock();
+    S->st.t_plan += S->now() - tp0;
     S->building = false;
     S->ready = std::move(R);
     S->cv_dev.notify_all();
@@ -252,8 +257,10 @@ void planner_main(gs_server* S) {
 void device_main(gs_server* S, gs_batch* B) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
+    const double tw = S->now();
     S->cv_dev.wait(lk, [&] { return S->ready || (S->stop && S->active.empty() && !S->building); });
     if (!S->ready) break;
+    S->st.t_idle += S->now() - tw;
     std::unique_ptr<Round> R = std::move(S->ready);
     ++S->running;
     const int injected = S->rounds_taken++ == S->fail_round ? S->fail_rc : GS_OK;
@@ -281,16 +288,37 @@ void device_main(gs_server* S, gs_batch* B) {
     if (!rc) {
       // each chunk's results into its piece; large rounds copy on several
       // threads (the pieces are disjoint)
+      std::atomic<int> alloc_failed{0};
       auto hand_off = [&](size_t first, size_t step) {
         for (size_t i = first; i < R->chunks.size(); i += step) {
           const Chunk& c = R->chunks[i];
           if (skip[i]) continue;
-          Request::Piece& pc = *c.piece;
-          const uint64_t a = c.at, z = c.at + (c.r1 - c.r0);
-          pc.nseeds.assign(res.nseeds + a, res.nseeds + z);
-          pc.npos.assign(res.npos + a, res.npos + z);
-          pc.seeds.assign(res.seeds + res.soff[a], res.seeds + res.soff[z]);
-          pc.pos.assign(res.pos + res.poff[a], res.pos + res.poff[z]);
+          gs_result& o = c.piece->r;
+          gs_result_free(&o);  // a chunk copied twice (see below) keeps the last copy
+          const uint64_t a = c.at, m = c.r1 - c.r0, z = a + m;
+          const uint64_t s0 = res.soff[a], p0 = res.poff[a];
+          o.n_reads = m;
+          o.n_seeds = res.soff[z] - s0;
+          o.n_pos = res.poff[z] - p0;
+          o.read_nseeds = (uint32_t*)gs::result_alloc(4 * (m + 1));
+          o.read_npos = (uint32_t*)gs::result_alloc(4 * (m + 1));
+          o.read_seed_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
+          o.read_pos_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
+          o.seeds = (gs_seed_rec*)gs::result_alloc(sizeof(gs_seed_rec) * (o.n_seeds + 1));
+          o.pos = (uint32_t*)gs::result_alloc(4 * (o.n_pos + 1));
+          if (!o.read_nseeds || !o.read_npos || !o.read_seed_off || !o.read_pos_off || !o.seeds || !o.pos) {
+            gs_result_free(&o);
+            alloc_failed = 1;
+            continue;
+          }
+          memcpy(o.read_nseeds, res.nseeds + a, 4 * m);
+          memcpy(o.read_npos, res.npos + a, 4 * m);
+          for (uint64_t j = 0; j <= m; ++j) {
+            o.read_seed_off[j] = res.soff[a + j] - s0;
+            o.read_pos_off[j] = res.poff[a + j] - p0;
+          }
+          memcpy(o.seeds, res.seeds + s0, sizeof(gs_seed_rec) * o.n_seeds);
+          memcpy(o.pos, res.pos + p0, 4 * o.n_pos);
         }
       };
       const uint64_t bytes = res.n_seeds * sizeof(gs_seed_rec) + 4 * res.n_pos;
@@ -310,6 +338,7 @@ void device_main(gs_server* S, gs_batch* B) {
       }
       hand_off(0, step);
       for (auto& h : helpers) h.join();
+      if (alloc_failed) rc = GS_ENOMEM;
     }
     const double t_end = S->now();
     lk.lock();
@@ -317,6 +346,9 @@ void device_main(gs_server* S, gs_batch* B) {
     S->st.n_rounds++;
     S->st.busy_seconds += t_end - t_round;
     S->st.device_seconds += bs.t_total;
+    S->st.t_seed += bs.t_seed;
+    S->st.t_post += bs.t_count + bs.t_write;
+    S->st.t_locate += bs.t_locate;
     S->st.t_load += t1 - t_round;
     S->st.t_run += t2 - t1;
     S->st.t_fetch += t3 - t2;
@@ -339,8 +371,8 @@ void device_main(gs_server* S, gs_batch* B) {
           finish_request(S, q, rc);
         } else {
           q->done_reads += c.r1 - c.r0;
-          q->n_seeds += c.piece->seeds.size();
-          q->n_pos += c.piece->pos.size();
+          q->n_seeds += c.piece->r.n_seeds;
+          q->n_pos += c.piece->r.n_pos;
           if (q->done_reads == q->n) finish_request(S, q, GS_OK);
         }
       }
@@ -468,43 +500,49 @@ extern "C" GS_API int gs_server_wait(gs_server* S, uint64_t ticket, gs_result* o
     release_request(q);
     return rc;
   }
-  // a successful request has no chunk left in any round: its vectors are final
+  // a successful request has no chunk left in any round: its pieces are final
   lk.unlock();
   const uint64_t n = q->n, nr1 = n + 1;
   int rc = GS_OK;
-  out->n_reads = n;
-  out->n_seeds = q->n_seeds;
-  out->n_pos = q->n_pos;
-  out->read_nseeds = (uint32_t*)malloc(4 * nr1);
-  out->read_npos = (uint32_t*)malloc(4 * nr1);
-  out->read_seed_off = (uint64_t*)malloc(8 * nr1);
-  out->read_pos_off = (uint64_t*)malloc(8 * nr1);
-  out->seeds = (gs_seed_rec*)malloc(sizeof(gs_seed_rec) * (out->n_seeds + 1));
-  out->pos = (uint32_t*)malloc(4 * (out->n_pos + 1));
-  if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
-      !out->pos) {
-    gs_result_free(out);
-    rc = GS_ENOMEM;
+  if (q->pieces.size() == 1) {  // one chunk: its arrays are the result
+    *out = q->pieces[0]->r;
+    memset(&q->pieces[0]->r, 0, sizeof(gs_result));
   } else {
-    uint64_t so = 0, po = 0, i = 0;
-    for (const auto& pp : q->pieces) {  // chunks in read order
-      const Request::Piece& pc = *pp;
-      for (size_t j = 0; j < pc.nseeds.size(); ++j, ++i) {
-        out->read_nseeds[i] = pc.nseeds[j];
-        out->read_npos[i] = pc.npos[j];
-        out->read_seed_off[i] = so;
-        out->read_pos_off[i] = po;
-        so += pc.nseeds[j];
-        po += pc.npos[j];
-      }
-      memcpy(out->seeds + (so - (pc.seeds.size())), pc.seeds.data(), sizeof(gs_seed_rec) * pc.seeds.size());
-      memcpy(out->pos + (po - pc.pos.size()), pc.pos.data(), 4 * pc.pos.size());
-    }
-    out->read_seed_off[n] = so;
-    out->read_pos_off[n] = po;
-    if (i != n || so != out->n_seeds || po != out->n_pos) {
+    out->n_reads = n;
+    out->n_seeds = q->n_seeds;
+    out->n_pos = q->n_pos;
+    out->read_nseeds = (uint32_t*)gs::result_alloc(4 * nr1);
+    out->read_npos = (uint32_t*)gs::result_alloc(4 * nr1);
+    out->read_seed_off = (uint64_t*)gs::result_alloc(8 * nr1);
+    out->read_pos_off = (uint64_t*)gs::result_alloc(8 * nr1);
+    out->seeds = (gs_seed_rec*)gs::result_alloc(sizeof(gs_seed_rec) * (out->n_seeds + 1));
+    out->pos = (uint32_t*)gs::result_alloc(4 * (out->n_pos + 1));
+    if (!out->read_nseeds || !out->read_npos || !out->read_seed_off || !out->read_pos_off || !out->seeds ||
+        !out->pos) {
       gs_result_free(out);
-      rc = GS_EHIP;  // pieces do not add up: must not happen
+      rc = GS_ENOMEM;
+    } else {
+      uint64_t so = 0, po = 0, i = 0;
+      for (const auto& pp : q->pieces) {  // chunks in read order
+        const gs_result& pc = pp->r;
+        memcpy(out->read_nseeds + i, pc.read_nseeds, 4 * pc.n_reads);
+        memcpy(out->read_npos + i, pc.read_npos, 4 * pc.n_reads);
+        for (uint64_t j = 0; j < pc.n_reads; ++j) {
+          out->read_seed_off[i + j] = so + pc.read_seed_off[j];
+          out->read_pos_off[i + j] = po + pc.read_pos_off[j];
+        }
+        memcpy(out->seeds + so, pc.seeds, sizeof(gs_seed_rec) * pc.n_seeds);
+        memcpy(out->pos + po, pc.pos, 4 * pc.n_pos);
+        i += pc.n_reads;
+        so += pc.n_seeds;
+        po += pc.n_pos;
+      }
+      out->read_seed_off[n] = so;
+      out->read_pos_off[n] = po;
+      if (i != n || so != out->n_seeds || po != out->n_pos) {
+        gs_result_free(out);
+        rc = GS_EHIP;  // pieces do not add up: must not happen
+      }
     }
   }
   lk.lock();

@@ -33,6 +33,11 @@
  *                      (8 bytes per base plus 2 bits): the batched kernels
  *                      then compare single-row matches with the text 64
  *                      bases per load; the output is the same
+ *   --pairs            also the pair table (4^18 bytes: which bases follow and
+ *                      precede every 18-base pattern): with --kmer-len 16
+ *                      and -k >= 19, one lookup answers two adjacent 19-base
+ *                      windows of a read strand that does not match; the
+ *                      output is the same
  *   --autotune         pick the batched kernel's launch parameters by timing
  *                      them on the first device batch (same output)
  *
@@ -363,7 +368,7 @@ int main(int argc, char** argv) {
   memset(&o, 0, sizeof o);
   o.k = 20;
   o.kernel = 3;
-  int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, direct = 0, c, li;
+  int print = 0, device = 0, reference_capacity = 0, kmer_len = 13, autotune = 0, direct = 0, pairs = 0, c, li;
   uint64_t device_bases = 256000000ull;
   static const struct option longopts[] = {{"kernel", required_argument, 0, 1},
                                            {"max-occ", required_argument, 0, 2},
@@ -373,6 +378,7 @@ int main(int argc, char** argv) {
                                            {"kmer-len", required_argument, 0, 6},
                                            {"autotune", no_argument, 0, 7},
                                            {"direct", no_argument, 0, 8},
+                                           {"pairs", no_argument, 0, 9},
                                            {0, 0, 0, 0}};
   while ((c = getopt_long(argc, argv, "k:so", longopts, &li)) >= 0) {
     switch (c) {
@@ -387,12 +393,13 @@ int main(int argc, char** argv) {
       case 6: kmer_len = atoi(optarg); break;
       case 7: autotune = 1; break;
       case 8: direct = 1; break;
+      case 9: pairs = 1; break;
       default: break; /* like the reference: getopt's message, then go on */
     }
   }
   if (optind + 2 > argc) {
     fprintf(stderr, "Usage: seed_gen [-k INT] [-s] [-o] [--kernel N] [--max-occ N] [--device-bases N] "
-                    "[--device N] [--reference-capacity] [--kmer-len 13..16] [--direct] [--autotune] "
+                    "[--device N] [--reference-capacity] [--kmer-len 13..16] [--direct] [--pairs] [--autotune] "
                     "<index-prefix> <reads.fa>\n");
     return 1;
   }
@@ -451,6 +458,15 @@ int main(int argc, char** argv) {
     fprintf(stderr, "direct-extension structures in %.3f seconds, %.2f GB on the GPU\n", info.t_direct,
             info.device_bytes / 1e9);
   }
+  if (pairs) {
+    rc = gs_index_set_pairs(idx, 1);
+    if (rc) {
+      fprintf(stderr, "seed_gen: building the pair table failed: %s (%d)\n", errname(rc), rc);
+      return 1;
+    }
+    gs_index_get_info(idx, &info);
+    fprintf(stderr, "pair table in %.3f seconds, %.2f GB on the GPU\n", info.t_pairs, info.device_bytes / 1e9);
+  }
 
   const double t_ready = now();  /* index and its structures on the GPU */
   fifo q_in, q_out;
@@ -508,9 +524,9 @@ int main(int argc, char** argv) {
   pthread_join(tr, NULL);
   pthread_join(tw, NULL);
   fclose(fp);
+  const double t_end = now(), t_total = t_end - t_start, t_seeding = t_end - t_ready;
   gs_index_close(idx);
   if (seed_rc || W.failed) return EXIT_FAILURE;
-  const double t_end = now(), t_total = t_end - t_start, t_seeding = t_end - t_ready;
   fprintf(stderr,
           "%s seeds: %llu reads, %llu bases, %llu device batches, %llu positions\n"
           "time: total %.3f s, input %.3f s, device %.3f s (kernels %.3f s), output %.3f s; %.1f reads/s\n"

@@ -168,6 +168,8 @@ typedef struct {
   double t_ext;         /* gs_index_set_kmer_len: table build, seconds       */
   int direct;           /* 1: direct-extension structures on the device      */
   double t_direct;      /* gs_index_set_direct: build, seconds               */
+  int pairs;            /* 1: pair table on the device                       */
+  double t_pairs;       /* gs_index_set_pairs: build, seconds                */
 } gs_index_info;
 
 GS_API int gs_index_open(const char* prefix, int device, gs_index** out);
@@ -191,11 +193,11 @@ GS_API int gs_index_read_kmer(const gs_index* idx, uint64_t first, uint64_t coun
  * back to those steps when the pattern does not occur or its window holds a
  * code above 3.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
-/* gs_index_set_kmer_len and gs_index_set_direct change structures that
- * every batch and server of the index reads: both return GS_EINVAL while a
- * server made on the index is alive, and wait for all device work of the
- * process before freeing anything; the caller must not run a batch of the
- * index on another thread during the call. */
+/* gs_index_set_kmer_len, gs_index_set_direct and gs_index_set_pairs change
+ * structures that every batch and server of the index reads: they return
+ * GS_EINVAL while a server made on the index is alive, and wait for all
+ * device work of the process before freeing anything; the caller must not
+ * run a batch of the index on another thread during the call. */
 /* Adds (on != 0) or drops the direct-extension structures: the full suffix
  * array, its inverse and the text at 2 bits per base, derived on the device
  * from the index itself (8 * (seq_len + 1) + seq_len / 4 bytes of HBM: 25.6 GB
@@ -205,6 +207,19 @@ GS_API int gs_index_set_kmer_len(gs_index* idx, int len);
  * and reading the row where the match ends, in place of one backward step
  * per base.  The output is unchanged.  No counterpart in the reference. */
 GS_API int gs_index_set_direct(gs_index* idx, int on);
+/* Adds (on != 0) or drops the pair table: one byte per 18-base pattern
+ * ("core"), bits 0-3 saying which bases follow it in the text, bits 4-7
+ * which precede it (4^18 bytes = 68.7 GB of HBM, at any text length).  With
+ * the 16-mer extended table and k >= 19, once a task of a read strand has
+ * found its clean 19-base window absent, the next task reads the byte of its
+ * window's core instead: one lookup answers whether its own window and the
+ * next task's occur, so on a strand that does not match the reference two
+ * tasks end per lookup instead of one.  The output is unchanged.  Built from
+ * the text of the direct-extension structures, or from a text derived from
+ * the index by the same walk.  No counterpart in the reference. */
+GS_API int gs_index_set_pairs(gs_index* idx, int on);
+/* Bytes [first, first+count) of the pair table (tests). */
+GS_API int gs_index_read_pairs(const gs_index* idx, uint64_t first, uint64_t count, uint8_t* out);
 /* Text positions of n suffix-array rows as the reference's walk reports
  * them, row primary+1 behaviour included (locate_seeds_gpu with
  * bwt_inv_psi_gpu, seed_gen.cu:455-484, 668-684); rows past seq_len and row
@@ -428,6 +443,9 @@ typedef struct {
   double t_load, t_run, t_fetch, t_split; /* wall time of the round stages, summed */
   uint64_t peak_round_bytes; /* largest round's device buffers, outputs excluded */
   double elapsed;            /* server clock now                                */
+  double t_plan;             /* planner: choosing and planning rounds, summed  */
+  double t_idle;             /* device threads waiting for a planned round, summed */
+  double t_seed, t_post, t_locate; /* device time of the pipeline stages, summed */
 } gs_server_stats;
 
 GS_API int gs_server_create(gs_index* idx, const gs_server_opts* opts, gs_server** out);

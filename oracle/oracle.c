@@ -327,7 +327,7 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair;
 } work_t;
 
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
@@ -351,10 +351,16 @@ static void count_blocks(const or_index* X, uint32_t l, uint32_t u, work_t* w) {
  * change which steps run and are counted (see or_opts). */
 static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t L, uint32_t k,
                                   int prune, int smem_stop, uint32_t ext_len, uint32_t sat,
-                                  uint32_t direct_min, int miss_skip, int smem, cand_t* out, uint32_t* A,
-                                  work_t* w) {
+                                  uint32_t direct_min, int miss_skip, int pair_probe, int smem, cand_t* out,
+                                  uint32_t* A, work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
+  /* pair-table model (or_opts.pair_probe), with the 16-base extended table:
+   * probe is set while the last looked-up clean 19-base window was absent;
+   * known_e is the task end whose window the last pair byte also answered */
+  const int pair = pair_probe && miss_skip && ext_len == 16 && k >= 19;
+  int probe = 0;
+  int64_t known_e = -1;
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
   uint32_t last_b = 0xFFFFFFFFu; /* left end of the last unpruned task (the device's in-lane SMEM filter) */
   for (uint32_t e = L; e >= k; --e) {
@@ -375,22 +381,37 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       int clean = 1;
       for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
       if (clean) { /* one lookup answers the first ext_len bases when they occur */
-        ++w->ext;
         uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
         l = (uint32_t)iv;
         u = (uint32_t)(iv >> 32);
         for (uint32_t j = PRECALC; j < ext_len && l <= u; ++j) step(X, &l, &u, v[e - 1 - j]);
-        if (l <= u && miss_skip && k >= ext_len + 3 && e >= ext_len + 3) {
-          /* the table's presence bits: the clean (ext_len + 3)-base window */
-          const uint32_t c1 = v[e - ext_len - 1], c2 = v[e - ext_len - 2], c3 = v[e - ext_len - 3];
-          if (c1 <= 3 && c2 <= 3 && c3 <= 3) {
-            uint32_t l3 = l, u3 = u;
-            step(X, &l3, &u3, c1);
-            if (l3 <= u3) step(X, &l3, &u3, c2);
-            if (l3 <= u3) step(X, &l3, &u3, c3);
-            if (l3 > u3) continue; /* absent: as below, with k >= ext_len + 3 */
-          }
+        int absent3 = 0; /* the clean (ext_len + 3)-base window does not occur */
+        const int win3 = miss_skip && k >= ext_len + 3 && e >= ext_len + 3 && v[e - ext_len - 1] <= 3 &&
+                         v[e - ext_len - 2] <= 3 && v[e - ext_len - 3] <= 3;
+        if (l <= u && win3) {
+          /* the table's presence bits */
+          uint32_t l3 = l, u3 = u;
+          step(X, &l3, &u3, v[e - ext_len - 1]);
+          if (l3 <= u3) step(X, &l3, &u3, v[e - ext_len - 2]);
+          if (l3 <= u3) step(X, &l3, &u3, v[e - ext_len - 3]);
+          absent3 = l3 > u3;
         }
+        if (pair && win3) { /* the 19-base window read[e-19, e) is clean */
+          const int present = l <= u && !absent3;
+          if (known_e == (int64_t)e) { /* the last pair byte answered this window */
+            if (present) ++w->ext;
+          } else if (probe && e >= 20 && v[e - 20] <= 3) { /* this task's pair byte */
+            ++w->pair;
+            known_e = (int64_t)e - 1;
+            if (present) ++w->ext;
+          } else {
+            ++w->ext;
+          }
+          probe = !present;
+        } else {
+          ++w->ext;
+        }
+        if (absent3) continue; /* absent: as below, with k >= ext_len + 3 */
         if (l <= u) {
           have = 1;
           b = e - ext_len;
@@ -526,14 +547,14 @@ static void phase_candidates(void* vctx, uint64_t i, int tid) {
   const uint32_t ext = (uint32_t)C->o->ext_len;
   const uint32_t sat = C->o->prune_sat;
   const uint32_t dmin = C->o->direct_min;
-  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip, C->o->smem, tmp + ntask, A,
-                                  &C->work[tid]);
+  uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip,
+                                  C->o->pair_probe, C->o->smem, tmp + ntask, A, &C->work[tid]);
   /* forward: decreasing end -> slot order (increasing end) */
   for (uint32_t q = 0; q < nf; ++q) Ls->F[q] = tmp[ntask + nf - 1 - q];
   Ls->nF = nf;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_rev(s[L - 1 - q]);
-  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip, C->o->smem, Ls->R, A,
-                                  &C->work[tid]);
+  uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip,
+                                  C->o->pair_probe, C->o->smem, Ls->R, A, &C->work[tid]);
   for (uint32_t q = 0; q < nr; ++q) { /* virtual [b,e) -> reverse (L-e, L-b) with strand bits */
     cand_t c = Ls->R[q];
     Ls->R[q] = (cand_t){(L - c.e) | 0x80000000u, (L - c.b) | 0x80000000u, c.l, c.u};
@@ -824,6 +845,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_sa += work[t].sa;
     res->w_direct += work[t].direct;
     res->w_text += work[t].text;
+    res->w_pair += work[t].pair;
     free(scratch[t]);
   }
   if (!o->prune) res->w_steps_unpruned = res->w_steps;

@@ -101,6 +101,18 @@ typedef struct {
    * whose clean (ext_len + 3)-base window is absent ends after the lookup
    * (the device table's presence bits).  Output unchanged; counters only. */
   int miss_skip;
+  /* pair_probe: work model of the device's pair table (one byte per 18-base
+   * core: which single bases follow it and which precede it in the text).
+   * With k >= 19 and miss_skip, once a task's clean 19-base window was
+   * found absent, the next task whose 20 bases read[e-20, e) are clean reads
+   * the byte of its core read[e-19, e-1) first: one lookup (w_pair) says
+   * whether its window read[e-19, e) occurs (a following base) and whether
+   * the next task's window read[e-20, e-1) does (a preceding base).  A task
+   * whose window is absent ends without its extended-table lookup; the next
+   * task, answered by the same byte, needs no lookup at all when its window
+   * is absent.  A present window is then looked up as usual (w_ext).  Output
+   * unchanged; counters only.  0: off. */
+  int pair_probe;
 } or_opts;
 
 typedef struct {
@@ -133,6 +145,7 @@ typedef struct {
   uint64_t w_direct;      /* direct-extension reads (opts.direct_min): 4-byte
                              suffix-array and inverse reads and, of them,     */
   uint64_t w_text;        /* 16-byte text reads                               */
+  uint64_t w_pair;        /* pair-table bytes read (opts.pair_probe)          */
 } or_result;
 
 /* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line

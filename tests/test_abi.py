@@ -111,6 +111,8 @@ def test_null_handles_are_rejected(built):
         "gs_index_read_kext": ([vp, u64, u64, vp], [None, 0, 1, p]),
         "gs_index_set_kmer_len": ([vp, i], [None, 14]),
         "gs_index_set_direct": ([vp, i], [None, 1]),
+        "gs_index_set_pairs": ([vp, i], [None, 1]),
+        "gs_index_read_pairs": ([vp, u64, u64, vp], [None, 0, 1, p]),
         "gs_index_locate": ([vp, vp, u64, vp], [None, p, 1, p]),
         "gs_index_read_direct": ([vp, u32, u32, vp, vp, vp], [None, 0, 1, p, p, p]),
         "gs_batch_create": ([vp, vp, vp, vp, u64, vp, vp], [None, p, p, p, 0, p, p]),

@@ -201,6 +201,26 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
                 assert cg["w_tasks"] < noskip[(stop, ext)]["w_tasks"], (smem, k, stop, ext)
             else:  # k below the window length: the option must not apply
                 assert cg == noskip[(stop, ext)], (smem, k, stop, ext)
+    # the pair-table model: with the 16-mer table and k >= 19, a lookup of an
+    # 18-base core answers two adjacent windows; same seeds, fewer table
+    # reads; with k < 19 (or no 16-mer table) it must not apply
+    for smem, k in ((True, 19), (False, 20), (True, 16)):
+        off = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=16, miss_skip=1)
+        for ext in (16, 14):
+            on = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=ext, miss_skip=1,
+                                pair_probe=1)
+            for a, b in zip(off[:4], on[:4]):
+                assert (a == b).all(), (smem, k, ext)
+            if ext != 16 or k < 19:
+                ref = off if ext == 16 else ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1,
+                                                         ext_len=ext, miss_skip=1)
+                assert on[4] == ref[4], (smem, k, ext)
+                continue
+            co, cn = off[4], on[4]
+            assert cn["w_pair"] > 0 and cn["w_ext"] + cn["w_pair"] < co["w_ext"], (smem, k)
+            assert cn["w_ext"] < co["w_ext"]
+            # everything after the table lookups is the same work
+            assert all(cn[f] == co[f] for f in ("w_tasks", "w_steps", "w_blocks96", "w_direct")), (smem, k)
     # saturating size records only lose pruning: same seeds, at least as
     # much work, and more work the lower the saturation (repeats of the
     # 1 Mbp reference reach interval sizes of 10, so 4 and 8 lose prunes)
