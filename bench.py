@@ -221,7 +221,8 @@ def serve_bench(g, X, ref, k, max_occ, args):
     qp.n_reads = args.serve_pool
     log("serving: generating a pool of %d configuration-4 reads" % qp.n_reads)
     pool = g.synth_reads(ref, qp)[:3]
-    opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel, seg_tasks=args.seg_tasks)
+    opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel, seg_tasks=args.seg_tasks,
+                       split_tasks=args.serve_split_tasks)
     plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
     # peak: a burst the server drains at full speed
     burst = make_requests(pool, args.serve_burst, 200000, seed=50)
@@ -243,7 +244,7 @@ def serve_bench(g, X, ref, k, max_occ, args):
                          "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
 
 
-def profiled_counters(config, kernel, kmer_len, direct):
+def profiled_counters(config, kernel, kmer_len, direct, pairs):
     """The committed rocprofv3 counter summary of the extension kernel
     (profiles/seed_persistent_config<N>.txt) if it was measured on this
     workload and index shape, else None."""
@@ -255,7 +256,8 @@ def profiled_counters(config, kernel, kmer_len, direct):
         parts = line.split()
         if len(parts) == 2 and not line.startswith("#"):
             kv[parts[0]] = float(parts[1])
-    if kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len or kv.get("direct", 0) != direct:
+    if (kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len or kv.get("direct", 0) != direct
+            or kv.get("pairs", 0) != pairs):
         return None
     kv["path"] = os.path.relpath(path, ROOT)
     return kv
@@ -411,6 +413,8 @@ def main():
     ap.add_argument("--serve-plain-requests", type=int, default=25,
                     help="requests per load level for the plain per-request server (0: none)")
     ap.add_argument("--serve-workers", type=int, default=0, help="device workers of the server (0: its default, 3)")
+    ap.add_argument("--serve-split-tasks", type=int, default=0,
+                    help="work splitting in served rounds: split_tasks (0: the library's default)")
     ap.add_argument("--serve-growth", type=int, default=0,
                     help="rounds grow up to this many times 16 Mbases under backlog (0: the server's "
                          "default, 4; 1: fixed rounds)")
@@ -641,7 +645,7 @@ def main():
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
         issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads for st in stats) / (len(stats) * args.batch)
-        prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct)
+        prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct, pairs)
         traffic, traffic_src = profiled_traffic(prof, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",

@@ -61,9 +61,7 @@ void index_release(gs_index* X);  // frees the index with its last reference
 // count is a 32-bit quantity, so launches over more items than this are
 // split.  2^30 unless a test sets it smaller (gs_set_launch_chunk).
 uint64_t launch_chunk();
-// malloc for result arrays handed to callers (freed with free()); large ones
-// are advised to use transparent huge pages, so that filling them faults in
-// 2 MB pages instead of 4 KB ones
+// allocation of result arrays handed to callers (freed with free())
 void* result_alloc(size_t bytes);
 // Looks up the device address of gs_seed_noalloc's scratch array on `device`
 // (which must be current); called once per device where an index is set up.

@@ -30,7 +30,6 @@
 //    the same reason an SMEM stream ends when a task reaches the read start:
 //    every later task would end there too and be filtered out.
 #include <stdlib.h>
-#include <sys/mman.h>
 #include <string.h>
 
 #include <algorithm>
@@ -845,16 +844,7 @@ struct gs_batch {
 static void free_graph(gs_batch* B);
 static void free_host_results(gs_batch* B);
 
-void* gs::result_alloc(size_t bytes) {
-  void* p = malloc(bytes);
-  constexpr size_t kPage = 4096, kHuge = 2u << 20;
-  if (p && bytes >= 2 * kHuge) {
-    const uintptr_t a = ((uintptr_t)p + kPage - 1) & ~(uintptr_t)(kPage - 1);
-    const uintptr_t z = ((uintptr_t)p + bytes) & ~(uintptr_t)(kPage - 1);
-    if (z > a) (void)madvise((void*)a, z - a, MADV_HUGEPAGE);  // advice only: failure changes nothing
-  }
-  return p;
-}
+void* gs::result_alloc(size_t bytes) { return malloc(bytes); }
 
 constexpr size_t kStageChunk = 64u << 20;  // pinned chunks of staged uploads and downloads
 
