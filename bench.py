@@ -59,14 +59,15 @@ def host_cpus():
 
 
 def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, direct_min=0, prune_sat=0,
-               pair_probe=0):
+               pair_probe=0, tandem=0):
     """Memory operations per read of the device algorithm, counted by the
     oracle on `sample` reads spread over the batch: 13-mer table lookups (8 bytes),
     extended-table lookups (16 bytes: interval and presence bits), occ
     entries (16 bytes) and direct-extension reads (4-byte suffix-array and
-    inverse words, 16-byte text chunks), with distance-1 pruning, the SMEM
-    read-start stop and tasks that cannot reach k bases ending at once; size
-    records saturate at prune_sat like the kernel's (0: exact)."""
+    inverse words, 16-byte text chunks), pair-table bytes and recorded
+    tandem intervals (8 bytes), with distance-1 pruning, the SMEM read-start
+    stop and tasks that cannot reach k bases ending at once; size records
+    saturate at prune_sat like the kernel's (0: exact)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
     sub = strided_sample(arrays, sample)
@@ -74,17 +75,18 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, 
     ox = ob.Index(prefix)
     c = ob.seed_arrays(ox, *sub, k, smem, max_occ=max_occ, threads=threads, smem_stop=1,
                        ext_len=ext_len if ext_len > 13 else 0, miss_skip=1, direct_min=direct_min,
-                       prune_sat=prune_sat, pair_probe=pair_probe)[4]
+                       prune_sat=prune_sat, pair_probe=pair_probe, tandem=tandem)[4]
     ox.close()
     table = (c["w_tasks"] + c["w_ext"] + c["w_pair"]) / n
     occ = c["w_blocks96"] / n
     direct, text = c["w_direct"] / n, c["w_text"] / n
+    hist = c["w_hist"] / n
     ext_bytes = 16.0 if ext_len > 13 else 8.0
-    return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct,
-                requests_per_read=table + occ + direct,
+    return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct, hist_per_read=hist,
+                requests_per_read=table + occ + direct + hist,
                 pair_per_read=c["w_pair"] / n,
                 bytes_per_read=8.0 * c["w_tasks"] / n + ext_bytes * c["w_ext"] / n + 1.0 * c["w_pair"] / n + 16.0 * occ
-                + 4.0 * (direct - text) + 16.0 * text)
+                + 4.0 * (direct - text) + 16.0 * text + 8.0 * hist)
 
 
 def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ, smem=True):
@@ -222,7 +224,7 @@ def serve_bench(g, X, ref, k, max_occ, args):
     log("serving: generating a pool of %d configuration-4 reads" % qp.n_reads)
     pool = g.synth_reads(ref, qp)[:3]
     opts = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=args.kernel, seg_tasks=args.seg_tasks,
-                       split_tasks=args.serve_split_tasks)
+                       split_tasks=args.serve_split_tasks, tandem=g.NO_TANDEM if args.no_tandem else 0)
     plain = g.make_opts(k=k, smem=True, max_occ=max_occ, kernel=0)
     # peak: a burst the server drains at full speed
     burst = make_requests(pool, args.serve_burst, 200000, seed=50)
@@ -364,6 +366,11 @@ def timed_steps(batches, warmup, kernel):
         log("kernel %d step %d: %.1f ms total, seed %.1f, count %.1f, write %.1f, locate %.1f; "
             "%d seeds, %d positions" % (kernel, i, s.t_total * 1e3, s.t_seed * 1e3, s.t_count * 1e3,
                                         s.t_write * 1e3, s.t_locate * 1e3, s.n_seeds, s.n_pos))
+    if kernel and stats and stats[0].n_reads:
+        s = stats[0]
+        log("kernel %d requests per read: %.1f table, %.1f occ, %.1f direct, %.1f tandem history" %
+            (kernel, s.n_table_reads / s.n_reads, s.n_occ_reads / s.n_reads, s.n_direct_reads / s.n_reads,
+             s.n_hist_reads / s.n_reads))
     return t1 - t0, stats
 
 
@@ -391,6 +398,8 @@ def main():
                     help="no direct-extension structures (full suffix array, inverse, text)")
     ap.add_argument("--no-pairs", action="store_true",
                     help="no pair table (one byte per 18-base pattern: the bases next to it in the text)")
+    ap.add_argument("--no-tandem", action="store_true",
+                    help="no tandem jumps in the batched kernel (gs_opts.tandem 0xFFFF)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=3,
@@ -475,6 +484,7 @@ def main():
         info = X.info()
         log("direct-extension structures in %.2fs, %.2f GB on device" % (info.t_direct, info.device_bytes / 1e9))
     pairs = 0 if args.no_pairs or args.config == 1 or info.kmer_len != 16 or k < 19 else 1
+    tandem = g.NO_TANDEM if args.no_tandem else 0
     if pairs:
         X.set_pairs(True)
         info = X.info()
@@ -495,7 +505,8 @@ def main():
         base = int(offs[lo])
         end = int(offs[hi - 1] + lens[hi - 1])
         arrays = (buf[base:end], offs[lo:hi] - np.uint64(base), lens[lo:hi])
-        batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks)))
+        batches.append((arrays, g.Batch(X, arrays, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks,
+                                        kernel=args.kernel, tandem=tandem)))
     log("%d batches of %d reads (%.2f Gbases) resident in HBM" % (resident, args.batch, len(buf) / 1e9))
     batches = [batches[i % resident] for i in range(args.warmup + args.steps)]
 
@@ -560,7 +571,7 @@ def main():
                                    "count": sum(x.t_count for x in r_stats) / len(r_stats) * 1e3,
                                    "write": sum(x.t_write for x in r_stats) / len(r_stats) * 1e3,
                                    "locate": sum(x.t_locate for x in r_stats) / len(r_stats) * 1e3},
-                     "requests_per_read_issued": sum(x.n_table_reads + x.n_occ_reads + x.n_direct_reads
+                     "requests_per_read_issued": sum(x.n_table_reads + x.n_occ_reads + x.n_direct_reads + x.n_hist_reads
                                                      for x in r_stats) / (len(r_stats) * args.batch),
                      "same_output_as_headline": bool(same)}
         log("reference-shaped index (13-mer table, suffix-array walk): %.0f reads/s, seed %.1f ms, locate %.1f ms, "
@@ -580,11 +591,12 @@ def main():
     sample_counts = None
     if not args.no_cpu:
         sa = strided_sample(first_arrays, args.work_sample)
-        sb = g.Batch(X, sa, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks, split_tasks=g.NO_SPLIT)
+        sb = g.Batch(X, sa, k=k, smem=smem, max_occ=max_occ, seg_tasks=args.seg_tasks, split_tasks=g.NO_SPLIT,
+                     kernel=args.kernel, tandem=tandem)
         st_s = sb.run(args.kernel)
         sb.close()
         sample_counts = dict(reads=len(sa[2]), table=st_s.n_table_reads, occ=st_s.n_occ_reads,
-                             direct=st_s.n_direct_reads)
+                             direct=st_s.n_direct_reads, hist=st_s.n_hist_reads)
     for _, bt in batches:
         bt.close()
     del batches
@@ -619,7 +631,7 @@ def main():
     if not args.no_cpu:
         wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem,
                         direct_min=4 if direct else 0, prune_sat={1: 255, 2: 65535, 3: 15}.get(args.kernel, 0),
-                        pair_probe=pairs)
+                        pair_probe=pairs, tandem=0 if args.no_tandem else 6)
         log("work model: %.1f table + %.1f occ + %.1f direct requests per read (%.0f bytes)" %
             (wm["table_per_read"], wm["occ_per_read"], wm["direct_per_read"], wm["bytes_per_read"]))
     serving = None
@@ -644,7 +656,8 @@ def main():
         # reads, counted by the oracle; W for the reference's procedure
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
-        issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads for st in stats) / (len(stats) * args.batch)
+        issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads + st.n_hist_reads
+                     for st in stats) / (len(stats) * args.batch)
         prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct, pairs)
         traffic, traffic_src = profiled_traffic(prof, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
@@ -660,7 +673,7 @@ def main():
               "measured_ceiling_requests_per_s": r_max}
         if sample_counts:
             sc = sample_counts
-            ra["requests_per_read_issued_on_model_sample"] = (sc["table"] + sc["occ"] + sc["direct"]) / sc["reads"]
+            ra["requests_per_read_issued_on_model_sample"] = (sc["table"] + sc["occ"] + sc["direct"] + sc["hist"]) / sc["reads"]
             ra["model_sample_reads"] = sc["reads"]
         if prof and "tcc_hit" in prof:
             miss = prof["tcc_miss"] / (prof["tcc_hit"] + prof["tcc_miss"])

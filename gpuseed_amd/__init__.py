@@ -65,7 +65,7 @@ class Opts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("max_occ", C.c_uint32),
                 ("batch_bases", C.c_uint32), ("kernel", C.c_int), ("seg_tasks", C.c_uint32),
                 ("wg", C.c_uint16), ("walk_rows", C.c_uint16), ("split_tasks", C.c_uint16),
-                ("split_work", C.c_uint16), ("direct_min", C.c_uint16), ("pad", C.c_uint16)]
+                ("split_work", C.c_uint16), ("direct_min", C.c_uint16), ("tandem", C.c_uint16)]
 
 
 class ServerOpts(C.Structure):
@@ -453,10 +453,13 @@ NO_SPLIT = 0xFFFF
 NO_DIRECT = 0xFFFF
 
 
+NO_TANDEM = 0xFFFF
+
+
 def make_opts(k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0, wg=0, walk_rows=0,
-              split_tasks=0, split_work=0, direct_min=0):
+              split_tasks=0, split_work=0, direct_min=0, tandem=0):
     return Opts(k, 1 if smem else 0, max_occ, batch_bases, kernel, seg_tasks, wg, walk_rows, split_tasks,
-                split_work, direct_min)
+                split_work, direct_min, tandem)
 
 
 def _arrays(arrays):
@@ -466,11 +469,12 @@ def _arrays(arrays):
 
 
 def seed(index, reads=None, k=20, smem=False, max_occ=0, batch_bases=0, kernel=0, arrays=None,
-         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0):
+         text_fd=None, seg_tasks=0, wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0,
+         tandem=0):
     """Seeds a list of reads (bytes) or arrays=(buf u8, offsets u64, lens u32)."""
     buf, offs, lens = _arrays(arrays if arrays is not None else pack_reads(reads))
     o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                walk_rows, split_tasks, split_work, direct_min)
+                                                walk_rows, split_tasks, split_work, direct_min, tandem)
     smem = bool(o.smem)
     res = Result()
     _check("gs_seed", lib().gs_seed(index.h, C.cast(buf.ctypes.data, C.c_char_p),
@@ -517,7 +521,8 @@ class BatchStats(C.Structure):
     _fields_ = [("t_h2d", C.c_double), ("t_seed", C.c_double), ("t_count", C.c_double),
                 ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
                 ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
-                ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64), ("n_direct_reads", C.c_uint64)]
+                ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64), ("n_direct_reads", C.c_uint64),
+                ("n_hist_reads", C.c_uint64)]
 
 
 def _bind_batch(L):
@@ -552,12 +557,12 @@ class Batch:
     """Reads uploaded once (gs_batch_create); run() leaves results in HBM."""
 
     def __init__(self, index, arrays, k=19, smem=True, max_occ=0, batch_bases=0, kernel=0, seg_tasks=0,
-                 wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0):
+                 wg=0, walk_rows=0, split_tasks=0, split_work=0, opts=None, direct_min=0, tandem=0):
         L = lib()
         _bind_batch(L)
         buf, offs, lens = _arrays(arrays)
         o = opts if opts is not None else make_opts(k, smem, max_occ, batch_bases, kernel, seg_tasks, wg,
-                                                    walk_rows, split_tasks, split_work, direct_min)
+                                                    walk_rows, split_tasks, split_work, direct_min, tandem)
         self.smem = bool(o.smem)
         self.h = C.c_void_p()
         _check("gs_batch_create", L.gs_batch_create(index.h, C.cast(buf.ctypes.data, C.c_char_p),

@@ -376,8 +376,8 @@ extern "C" GS_API int gs_index_set_kmer_len(gs_index* X, int len) {
 }
 
 static void free_direct(gs_index* X) {
-  const uint64_t rows = (uint64_t)X->seq_len + 1;
-  if (X->sa_full) X->device_bytes -= 2 * rows * sizeof(uint32_t) + X->text_words * sizeof(uint32_t);
+  const uint64_t padded = ((uint64_t)X->seq_len + 1 + 3) & ~3ull;  // as allocated
+  if (X->sa_full) X->device_bytes -= 2 * padded * sizeof(uint32_t) + X->text_words * sizeof(uint32_t);
   (void)hipFree(X->sa_full);
   (void)hipFree(X->isa);
   (void)hipFree(X->text);
@@ -440,13 +440,15 @@ extern "C" GS_API int gs_index_set_direct(gs_index* X, int on) {
   uint8_t* text8 = nullptr;
   uint32_t* d_err = nullptr;  // [0] error bits, [2..3] rows visited
   int rc = GS_OK;
-  if (hipMalloc(&X->sa_full, rows * 4) != hipSuccess || hipMalloc(&X->isa, rows * 4) != hipSuccess ||
+  // whole 16-byte chunks: the persistent kernel reads the 16 bytes that hold a word
+  const uint64_t padded = (rows + 3) & ~3ull;
+  if (hipMalloc(&X->sa_full, padded * 4) != hipSuccess || hipMalloc(&X->isa, padded * 4) != hipSuccess ||
       hipMalloc(&X->text, words * 4) != hipSuccess || hipMalloc(&text8, n + 1) != hipSuccess ||
       hipMalloc(&d_err, 16) != hipSuccess)
     rc = GS_ENOMEM;
   if (!rc) {
     X->text_words = words;
-    X->device_bytes += 2 * rows * 4 + words * 4;
+    X->device_bytes += 2 * padded * 4 + words * 4;
     const DevIndex D = X->dev();
     uint32_t h_err[4] = {0, 0, 0, 0};
     if (hipMemsetAsync(d_err, 0, 16, X->stream) != hipSuccess) rc = GS_EHIP;

@@ -76,6 +76,35 @@ namespace gs {
 // reads longer than that are cut into segments of seg_tasks consecutive task
 // ends, one item per (read, segment, strand).  items[i] = read | (segment << 33)
 // | (strand << 32); the items of a read are adjacent, segment-major.
+// The fields of a persistent launch that the kernel's loop uses only at
+// item starts and ends, in work splitting and for its counters.  The launch
+// writes them to device memory (the 256 bytes after the counters) and the
+// kernel reads them with scalar loads where it uses them, so that its
+// arguments fit the scalar registers instead of spilling into vector ones.
+struct PersistCold {
+  const uint64_t* code_off;
+  const uint32_t* lens;
+  const uint64_t* task_off;
+  const uint64_t* items;
+  uint4* cand;
+  uint32_t* nF;
+  uint32_t* nR;
+  uint4* seg_meta;
+  uint32_t* seg_link;
+  uint64_t* counters;
+  uint4* dyn_items;
+  uint4* dyn_meta;
+  uint32_t* dyn_link;
+  uint32_t* dyn_ready;
+  uint32_t* dyn_ctl;
+  uint4* wmeta;
+  uint32_t* wlink;
+  uint32_t seg_tasks, epoch;
+};
+constexpr uint32_t kCounterBytes = 32;   // four 64-bit counters
+constexpr uint32_t kColdBytes = 256;     // PersistCold, after the counters
+static_assert(sizeof(PersistCold) <= kColdBytes, "PersistCold must fit its slot");
+
 struct PersistParams {
   DevIndex X;
   const uint4* codes;        // 4-bit virtual codes, per read: forward chunks then reverse chunks
@@ -91,7 +120,8 @@ struct PersistParams {
   uint4* seg_meta;           // per item of a segmented read: {emitted, first b, last b, flags}
   uint32_t* seg_link;        // per item of a segmented read: first piece split off it, or ~0
   uint32_t* queue;           // work-queue counter
-  uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads, [2] direct-extension reads
+  uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads, [2] direct-extension reads,
+                             // [3] tandem-history reads
   uint32_t direct_min;       // size-1 steps before a task reads the text (0: no direct extension)
   // work splitting (dyn_cap 0: off)
   uint4* dyn_items;          // pieces {read, strand, first task, tasks}
@@ -104,7 +134,8 @@ struct PersistParams {
   uint32_t dyn_cap, split_work, split_tasks, epoch;
   uint32_t split_below;           // split only while fewer items than this run (set at launch)
   uint32_t refill_group;          // idle lanes of a wave that trigger a refill (set at launch)
-  uint32_t pad2;
+  uint32_t tandem;                // tandem jumps: largest period (0: off; needs hist)
+  uint2* hist;                    // tandem history: kTandemMax * kTandemCap intervals per lane of the grid
 };
 
 // Joins pieces (segments of long reads, split-off pieces) into the per-read
@@ -136,6 +167,17 @@ constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that tri
                                                 // 2, 3, 4 and 6 within 1 %)
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
+// Tandem jumps of the persistent kernel: periods 1..kTandemMax; a stretch
+// opens at a task whose previous task's left end lay kTandemGate or more
+// below its end and spans at least kTandemSpan positions; a walker records
+// the positions R + p .. R + kTandemCap - 1 of the stretch [R, E).  With the
+// 20 codes of the rolling window a period is at most 19; the configurations'
+// tandem repeats have periods 1-6.
+constexpr uint32_t kTandemMax = 6;
+constexpr uint32_t kTandemCap = 128;
+constexpr uint32_t kTandemGate = 24;
+constexpr uint32_t kTandemSpan = 32;
+constexpr uint64_t kTandemLaneBytes = 8ull * kTandemMax * kTandemCap;
 constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start
 
 // segments of a read of length L (1 for reads held whole in the LDS window)

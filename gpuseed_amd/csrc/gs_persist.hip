@@ -62,8 +62,30 @@ __device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The persistent kernel's arguments: what its loop reads on every
+// iteration, and the device copy of the rest (gs::PersistCold, read where it
+// is used).
+struct PersistArgs {
+  gs::DevIndex X;
+  const uint4* codes;
+  const gs::PersistCold* cold;
+  uint32_t* queue;
+  uint2* hist;
+  uint64_t n_items;
+  uint32_t k, smem, direct_min, dyn_cap, split_work, split_tasks, split_below, refill_group, tandem;
+};
+
+// Waves per SIMD the register allocation must allow: 4 for the 4-bit
+// records (their LDS slot lets 16 waves share a CU, and this memory-bound
+// loop wants them all; the few values that do not fit, such as the
+// candidate pointer, are reloaded where a task emits), no bound for the
+// wider records, whose LDS slots allow fewer waves anyway.
+template <typename AT>
+constexpr int kMinWaves = std::is_same<AT, Nib>::value ? 4 : 1;
+
 template <typename AT, int LMAX, int BS>
-__global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kMinWaves<AT>)))
+seed_persistent(PersistArgs P) {
   static_assert(LMAX % 32 == 0, "LMAX must be a multiple of 32");
   // size records of RB bits per position: 4 (Nib, two per byte, low nibble
   // first), 8 or 16; a record saturates at 2^RB - 1 and then never prunes
@@ -101,14 +123,17 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   const uint32_t L2_0 = P.X.L2[0], L2_1 = P.X.L2[1], L2_2 = P.X.L2[2], L2_3 = P.X.L2[3];
   const uint32_t primary = P.X.primary;
   const uint64_t n_items = P.n_items;
-  // three control words, 256 bytes apart (separate cache lines and channels)
-  uint32_t* const reserved = P.dyn_ctl;          // pieces reserved by splitting lanes
-  uint32_t* const next_piece = P.dyn_ctl + 64;
-  uint32_t* const running = P.dyn_ctl + 128;     // items being processed
+  // three control words, 256 bytes apart (separate cache lines and channels):
+  // pieces reserved by splitting lanes, the next piece to claim, items being
+  // processed
+  const gs::PersistCold* const C = P.cold;
+  auto reserved = [&]() __attribute__((always_inline)) { return C->dyn_ctl; };
+  auto next_piece = [&]() __attribute__((always_inline)) { return C->dyn_ctl + 64; };
+  auto running = [&]() __attribute__((always_inline)) { return C->dyn_ctl + 128; };
 
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
-  uint32_t r = 0, strand = 0, L = 0, ntask = 0, nout = 0, last_b = 0;
+  uint32_t r = 0, strand = 0, L = 0, nout = 0, last_b = 0;
   // the 20 codes left of the task end e: code(e-1-j) at bits 2j of w20 for
   // j = 0..19, and bit j of bad20 set if that code is above 3 or its position
   // is left of the read start; the 13-mer pattern is the low 26 bits, an
@@ -122,16 +147,17 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   const bool pair_on = P.X.pair && P.X.ext_len == 16 && k >= 19;
   bool probe = false, known_p = false;
   int32_t known_e = -1;
-  uint32_t l = 0, u = 0, b = 0, s = 0, cx = 0;
+  uint32_t l = 0, u = 0, b = 0, cx = 0;
   int32_t e = 0, x = 0, e_min = 0, w0 = 0;
   uint32_t t0 = 0, first_b = 0, flags = 0, work = 0;
   uint32_t pending = kNone;                                // claimed piece not visible yet
-  uint32_t started = 0, ended = 0;                          // items this iteration (for `running`)
+  int32_t drun = 0;                                         // items started minus ended this iteration
   uint32_t poll_wait = 0;                                   // wave-uniform: iterations to the next poll
   uint32_t backoff = 1;                                     // wave-uniform: sleeps while all lanes wait
   uint32_t link = kNone;                                   // first piece split off this item
-  uint64_t item = 0;
+  uint32_t item = 0;  // index of the item (or piece) in its list
   int phase = 0;
+  bool need_next = false;  // the task ended: next_task at the end of the iteration
   bool ext_miss = false;  // the task's extended-table pattern is absent: use the 13-mer path
   // direct extension: single-row steps of the task so far, the text position
   // of read[b] in a run, and why a run reads the inverse suffix array (1: the
@@ -139,35 +165,52 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   uint32_t ones = 0, tq = 0;
   int rout = 0;
   uint4* out = nullptr;
-  const uint4* gcodes = nullptr;  // the stream's codes in device memory
-  uint64_t n_table = 0, n_occ = 0, n_direct = 0;
+  uint32_t gcodes = 0;  // the stream's first 16-byte code chunk (a batch has fewer than 2^32)
+  uint32_t n_table = 0, n_occ = 0, n_direct = 0, n_hist = 0;  // this lane's memory operations
+  // Tandem jumps.  In a stretch [tR, tE) of period tp (code(z) == code(z + tp)
+  // <= 3 for z in [tR, tE - tp)), read[y, e) equals read[y + d, e + d) for d
+  // a multiple of tp, y >= tR and e + d <= tE, so its interval is the one
+  // task end e + d had at position y + d.  The task ends tE .. tE - tp + 1
+  // (walker j = tE - e) extend as usual and record their intervals at the
+  // positions tR + tp .. tR + kTandemCap - 1 they compute them for, while
+  // those stay contiguous (a direct run or an N leaves a gap); any later
+  // task end e of the stretch, walker j = (tE - e) mod tp, d = tE - j - e,
+  // reads the interval at max(tR, lowest record of j - d) + d (phase 6) in
+  // place of its table lookup and of the steps above that position, and
+  // extends from there as after a lookup.  Skipped size records only prune.
+  // Byte j of t_lo / t_hi: offset from tR of walker j's lowest / highest
+  // record (t_hi < t_lo: none).  tp 0: no stretch.
+  uint32_t tp = 0;
+  int32_t tR = 0, tE = 0;
+  uint64_t t_lo = 0, t_hi = 0;
+  const bool tandem_on = P.tandem && P.hist;
 
   // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
   // an extension may run left of it and then reads device memory
   auto code = [&](int32_t i) __attribute__((always_inline)) -> uint32_t {
     const int32_t j = i - w0;
     if (j >= 0) return (lds[sbase + (j >> 1)] >> ((j & 1) << 2)) & 15u;
-    const uint32_t w = reinterpret_cast<const uint32_t*>(gcodes + (i >> 5))[(i >> 3) & 3];
+    const uint32_t w = reinterpret_cast<const uint32_t*>(P.codes + gcodes + (uint32_t)(i >> 5))[(i >> 3) & 3];
     return (w >> ((i & 7) << 2)) & 15u;
   };
 
   auto end_stream = [&]() __attribute__((always_inline)) {
     const uint4 meta = make_uint4(nout, first_b, last_b, flags);
     if (kind == 2) {
-      P.dyn_meta[item] = meta;
-      P.dyn_link[item] = link;
+      C->dyn_meta[item] = meta;
+      C->dyn_link[item] = link;
     } else if (kind == 1) {
-      P.seg_meta[item] = meta;
-      if (P.dyn_cap) P.seg_link[item] = link;  // allocated only with splitting on
+      C->seg_meta[item] = meta;
+      if (P.dyn_cap) C->seg_link[item] = link;  // allocated only with splitting on
     } else if (link != kNone) {  // a whole read's stream that was split: joined later
-      P.wmeta[2 * r + strand] = meta;
-      P.wlink[2 * r + strand] = link;
-      (strand ? P.nR : P.nF)[r] = gs::kSplitStream;
+      C->wmeta[2 * r + strand] = meta;
+      C->wlink[2 * r + strand] = link;
+      (strand ? C->nR : C->nF)[r] = gs::kSplitStream;
     } else {
-      (strand ? P.nR : P.nF)[r] = nout;
+      (strand ? C->nR : C->nF)[r] = nout;
     }
     have = false;
-    ++ended;  // folded into `running` once per wave and iteration
+    --drun;  // folded into `running` once per wave and iteration
   };
   auto advance = [&]() __attribute__((always_inline)) -> bool {
     if (e - 1 < e_min) return false;
@@ -186,21 +229,85 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   auto split = [&]() __attribute__((always_inline)) -> bool {
     const uint32_t n_rem = (uint32_t)(e - e_min);
     const uint32_t n_push = n_rem / 2;
-    if (__hip_atomic_load(reserved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= P.dyn_cap) return false;
-    const uint32_t id = atomicAdd(reserved, 1u);
+    if (__hip_atomic_load(reserved(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= P.dyn_cap) return false;
+    const uint32_t id = atomicAdd(reserved(), 1u);
     if (id >= P.dyn_cap) return false;  // reserved past the end: nothing to write, nobody claims it
     const uint32_t t_first = L - (uint32_t)(e_min + (int32_t)n_push - 1);  // task index of end e_min + n_push - 1
-    P.dyn_items[id] = make_uint4(r, strand, t_first, n_push);
-    P.dyn_link[id] = link;  // initial value: the piece's own successor
+    C->dyn_items[id] = make_uint4(r, strand, t_first, n_push);
+    C->dyn_link[id] = link;  // initial value: the piece's own successor
     __threadfence();
-    store_release(P.dyn_ready + id, P.epoch);
+    store_release(C->dyn_ready + id, C->epoch);
     link = id;
     e_min += (int32_t)n_push;
     work = 0;
     return true;
   };
+  // the lane's tandem history: kTandemMax walkers of kTandemCap intervals
+  auto hist_base = [&]() __attribute__((always_inline)) -> uint2* {
+    return P.hist + (size_t)(blockIdx.x * BS + threadIdx.x) * (gs::kTandemMax * gs::kTandemCap);
+  };
+  // a walker records the interval it just reached at b
+  auto t_record = [&]() __attribute__((always_inline)) {
+    const uint32_t j = (uint32_t)(tE - e);
+    if (j >= tp) return;  // not a walker (or no stretch: tp 0)
+    const int32_t off = (int32_t)b - tR;
+    if (off < (int32_t)tp || off >= (int32_t)gs::kTandemCap) return;
+    const uint32_t sh = 8u * j;
+    const uint32_t lo = (uint32_t)(t_lo >> sh) & 255u, hi = (uint32_t)(t_hi >> sh) & 255u;
+    if (hi >= lo && (uint32_t)off + 1u != lo) return;  // not next to the records so far
+    hist_base()[j * gs::kTandemCap + (uint32_t)off] = make_uint2(l, u);
+    if (hi < lo) t_hi = (t_hi & ~(255ull << sh)) | ((uint64_t)off << sh);
+    t_lo = (t_lo & ~(255ull << sh)) | ((uint64_t)off << sh);
+  };
+  // At a task's start: a jump (phase 6, the position in x, the record in
+  // tq), a walker, or none; with no stretch open, opens one when `gate`
+  // (the previous task's left end lay kTandemGate or more below its end)
+  // and the 20 codes left of e have a period.
+  auto t_start = [&](bool gate) __attribute__((always_inline)) {
+    if (tp) {
+      const uint32_t dd = (uint32_t)(tE - e);
+      if (dd < tp) return;  // a walker
+      const uint32_t j = dd % tp, d = dd - j;
+      const uint32_t lo = (uint32_t)(t_lo >> (8u * j)) & 255u, hi = (uint32_t)(t_hi >> (8u * j)) & 255u;
+      if (hi >= lo) {
+        const int32_t yo = (int32_t)lo - (int32_t)d > 0 ? (int32_t)lo - (int32_t)d : 0;
+        if (yo + (int32_t)d <= (int32_t)hi) {
+          x = tR + yo;
+          tq = j * gs::kTandemCap + (uint32_t)yo + d;
+          phase = 6;
+          return;
+        }
+      }
+      tp = 0;  // the stretch ends
+    }
+    if (gate && e >= 20 && bad20 == 0) {
+      uint32_t q = 0;
+      for (uint32_t p = 1; p <= P.tandem; ++p) {
+        if (!((w20 ^ (w20 >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) {
+          q = p;
+          break;
+        }
+      }
+      if (q) {
+        int32_t z = e - 21;
+        while (z >= 0) {
+          const uint32_t c = code(z);
+          if (c > 3 || c != code(z + (int32_t)q)) break;
+          --z;
+        }
+        if (e - (z + 1) >= (int32_t)gs::kTandemSpan) {
+          tp = q;
+          tR = z + 1;
+          tE = e;
+          t_lo = 0xFFFFFFFFFFFFull;  // every walker without records
+          t_hi = 0;
+        }
+      }
+    }
+  };
   // next task with a clean 13-base window, or end of the item
   auto next_task = [&]() __attribute__((always_inline)) {
+    const bool gate = tandem_on && e - (int32_t)b >= (int32_t)gs::kTandemGate;
     phase = 0;
     ext_miss = false;
     ones = 0;
@@ -208,7 +315,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     // than half the lanes, so pieces find idle lanes instead of adding work
     if (P.dyn_cap && work > P.split_work && e - e_min >= 2 * (int32_t)P.split_tasks &&
         __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
-        __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
+        __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
       split();
     while (true) {
       if (!advance()) {
@@ -222,6 +329,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       }
       break;
     }
+    b = (uint32_t)e;  // the task's left end until its lookup
+    if (tandem_on) t_start(gate);
   };
   // size record at pos (positions left of the window keep none: pruning is
   // only an early exit, so a missing record never changes the output)
@@ -237,9 +346,9 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const bool emit = (uint32_t)e - b >= k && (!P.smem || b < last_b);
     if (emit) {
       if (strand == 0)
-        out[ntask - 1 - t0 - nout] = make_uint4(b, (uint32_t)e, l, u);
+        out[L - k - t0 - nout] = make_uint4(b, (uint32_t)e, l, u);  // task slots: L - k + 1 per strand
       else
-        out[ntask + t0 + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
+        out[L - k + 1 + t0 + nout] = make_uint4((L - (uint32_t)e) | 0x80000000u, (L - b) | 0x80000000u, l, u);
       ++nout;
     }
     if (!(flags & 2u)) {  // the item's first finished task
@@ -259,7 +368,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         return 0;
       }
       b = (uint32_t)x;  // code > 3 after the window: interval kept, position consumed
-      if (prune_at(x, s)) return 1;
+      if (prune_at(x, u - l + 1u)) return 1;
       --x;
     }
   };
@@ -274,7 +383,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   // suffix array (phase 5) when it is needed: for a candidate that is
   // emitted, or to go on with backward steps past an N.
   auto try_direct = [&]() __attribute__((always_inline)) {
-    if (P.direct_min && s == 1u && ones >= P.direct_min) {
+    if (P.direct_min && l == u && ones >= P.direct_min) {
       const int32_t j = x - w0;  // runs stay inside the LDS window
       if (j >= 0 && rec_get(sbase, j) != 1u) phase = 3;
     }
@@ -292,12 +401,11 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   // Loads task ends e_hi .. e_hi - tcount + 1 of stream (r, strand) into the
   // lane: the LDS code window [w0, top) covers every task's 13-base start.
   auto start_item = [&](uint32_t tstart, uint32_t tcount, uint32_t link0) __attribute__((always_inline)) {
-    ntask = L - k + 1;
     const uint32_t nch = (L + 31) >> 5;
-    const uint4* src = P.codes + P.code_off[r] + strand * nch;
+    gcodes = (uint32_t)C->code_off[r] + strand * nch;
+    const uint4* src = P.codes + gcodes;
     uint32_t top;
     t0 = tstart;
-    gcodes = src;
     if (L <= (uint32_t)LMAX) {  // the whole stream fits the window
       top = nch << 5;
       w0 = 0;
@@ -310,7 +418,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const uint32_t za = ((top - (uint32_t)w0) * (uint32_t)RB / 8 + 15) >> 4;
     for (uint32_t c = 0; c < za; ++c)
       *reinterpret_cast<uint4*>(lds + sbase + CODE_BYTES + 16 * c) = make_uint4(0, 0, 0, 0);
-    out = P.cand + 2 * P.task_off[r];
+    out = C->cand + 2 * C->task_off[r];
     nout = 0;
     last_b = 0xFFFFFFFFu;
     first_b = 0xFFFFFFFFu;
@@ -328,23 +436,25 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       w20 |= (uint64_t)(c & 3u) << (2 * j);
     }
     have = true;
-    ++started;
+    ++drun;
     phase = 0;
     ext_miss = false;
     ones = 0;
     probe = false;
     known_e = -1;
-    if (bad20 & kBad13) next_task();  // no work done yet, so this never splits
+    b = (uint32_t)e;
+    tp = 0;
+    need_next = (bad20 & kBad13) != 0;  // no work done yet, so this never splits
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
-    item = it;
-    const uint64_t v = P.items[it];
+    item = (uint32_t)it;
+    const uint64_t v = C->items[it];
     r = (uint32_t)v;
     strand = (uint32_t)(v >> 32) & 1u;
     const uint32_t seg = (uint32_t)(v >> 33);
-    L = P.lens[r];
+    L = C->lens[r];
     if (L < k) {
-      (strand ? P.nR : P.nF)[r] = 0;
+      (strand ? C->nR : C->nF)[r] = 0;
       return;
     }
     if (L <= (uint32_t)LMAX) {
@@ -352,31 +462,28 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       start_item(0, L - k + 1, kNone);
     } else {
       kind = 1;
-      const uint32_t ts = seg * P.seg_tasks;
-      start_item(ts, min(P.seg_tasks, L - k + 1 - ts), kNone);
+      const uint32_t ts = seg * C->seg_tasks;
+      start_item(ts, min(C->seg_tasks, L - k + 1 - ts), kNone);
     }
   };
   auto start_piece = [&](uint32_t id) __attribute__((always_inline)) {
-    const uint4 d = P.dyn_items[id];
-    const uint32_t lk = P.dyn_link[id];
+    const uint4 d = C->dyn_items[id];
+    const uint32_t lk = C->dyn_link[id];
     item = id;
     kind = 2;
     r = d.x;
     strand = d.y;
-    L = P.lens[r];
+    L = C->lens[r];
     start_item(d.z, d.w, lk);  // pieces split off this one go in front of its successor
   };
   // items started minus items ended in this iteration, one atomic per wave;
   // a split pushes its pieces (and fences) before its item ends
   auto fold_running = [&]() __attribute__((always_inline)) {
-    if (__ballot(started != ended) == 0) {
-      started = ended = 0;
-      return;
-    }
-    int d = (int)started - (int)ended;
-    started = ended = 0;
+    if (__ballot(drun != 0) == 0) return;
+    int d = drun;
+    drun = 0;
     for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
-    if (lane == 0 && d) atomicAdd(running, (uint32_t)d);
+    if (lane == 0 && d) atomicAdd(running(), (uint32_t)d);
   };
   // overflow queue, polled by one lane per wave: claims pieces for the
   // wave's idle lanes with one CAS, or finds that none can come any more.
@@ -393,16 +500,16 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     const int leader = __ffsll((unsigned long long)idlemask) - 1;
     uint32_t got = 0, first = 0, none = 0;
     if (lane == leader) {
-      const uint32_t nx = __hip_atomic_load(next_piece, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t rs = __hip_atomic_load(reserved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t nx = __hip_atomic_load(next_piece(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t rs = __hip_atomic_load(reserved(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t lim = rs < P.dyn_cap ? rs : P.dyn_cap;
       if (nx < lim) {
         const uint32_t want = min((uint32_t)__popcll(idlemask), lim - nx);
-        if (atomicCAS(next_piece, nx, nx + want) == nx) {
+        if (atomicCAS(next_piece(), nx, nx + want) == nx) {
           got = want;
           first = nx;
         }
-      } else if (__hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      } else if (__hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
         none = 1;  // nothing to claim and no item anywhere that could still split
       }
     }
@@ -421,7 +528,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
   // a claimed piece starts once its entry is visible
   auto take_pending = [&]() __attribute__((always_inline)) {
     if (pending == kNone || have) return;
-    if (__hip_atomic_load(P.dyn_ready + pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != P.epoch) return;
+    if (__hip_atomic_load(C->dyn_ready + pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != C->epoch) return;
     __threadfence();
     const uint32_t id = pending;
     pending = kNone;
@@ -454,8 +561,8 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       poll_pieces();
       take_pending();
     }
-    const int op = have ? (phase == 0 ? 1 : (phase == 1 ? 2 : phase)) : 0;
-    if (__ballot(op != 0) == 0) {
+    const int op = have && !need_next ? (phase == 0 ? 1 : (phase == 1 ? 2 : phase)) : 0;
+    if (__ballot(op != 0 || need_next) == 0) {
       if (P.dyn_cap) fold_running();
       if (__ballot(!finished) == 0) break;
       // the whole wave waits for pieces: back off (about 3 us per sleep, up to 16)
@@ -465,14 +572,19 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       continue;
     }
     backoff = 1;
-    // one memory operation per lane
+    // One memory operation per lane: a 16-byte load (and for op 2 perhaps a
+    // second one).  The 1-, 4- and 8-byte items (pair byte, suffix-array and
+    // inverse words, recorded intervals) come from the aligned 16 bytes that
+    // hold them, `sub` saying which part: one load instruction and address
+    // for every kind, the same single random access each.
     const uint4* a1 = nullptr;
     const uint4* a2 = nullptr;
+    uint32_t sub = 0;
+    bool pair_read = false;  // op 1 reads the pair byte instead of a table entry
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
     uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)w20 & 0x3FFFFFFu;  // table lookup: pattern length and entry
     uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
-    const uint8_t* w1 = nullptr;  // a 1-byte read: the pair table (op 1)
     if (op == 1) {
       a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (tidx >> 1);  // two 8-byte entries per load
       const uint32_t el = P.X.ext_len;
@@ -487,8 +599,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       // after an absent window, a task whose 20 codes are clean reads the
       // pair byte of its 18-base core read[e-19, e-1) instead
       if (pair_on && probe && known_e != e && !(bad20 & 0xFFFFFu)) {
-        a1 = nullptr;
-        w1 = P.X.pair + ((w20 >> 2) & ((1ull << 36) - 1ull));
+        const uint64_t pi = (w20 >> 2) & ((1ull << 36) - 1ull);
+        a1 = reinterpret_cast<const uint4*>(P.X.pair) + (pi >> 4);
+        sub = (uint32_t)pi & 15u;
+        pair_read = true;
       }
       ++n_table;
       ++work;
@@ -503,16 +617,21 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       if (have1 && b1 != b2) a2 = P.X.occ + (size_t)b1 * 4 + cx;
       n_occ += a2 ? 2 : 1;
       work += a2 ? 2 : 1;
-    }
-    const uint32_t* w4 = nullptr;  // a 4-byte read: suffix array (op 3) or its inverse (op 5)
-    if (op == 3) {
-      w4 = P.X.sa_full + l;
+    } else if (op == 3) {  // the suffix-array word of row l (arrays padded to 16 bytes)
+      a1 = reinterpret_cast<const uint4*>(P.X.sa_full) + (l >> 2);
+      sub = l & 3u;
     } else if (op == 4) {
       a1 = P.X.text + ((tq - 1u) >> 6);  // tq >= 1 whenever phase 4 is entered
-    } else if (op == 5) {
-      w4 = P.X.isa + tq;
+    } else if (op == 5) {  // the inverse word of text position tq
+      a1 = reinterpret_cast<const uint4*>(P.X.isa) + (tq >> 2);
+      sub = tq & 3u;
+    } else if (op == 6) {  // the recorded interval tq of this lane (an even number of them per lane)
+      a1 = reinterpret_cast<const uint4*>(hist_base()) + (tq >> 1);
+      sub = tq & 1u;
+      ++n_hist;
+      ++work;
     }
-    if (op >= 3) {
+    if (op >= 3 && op <= 5) {
       ++n_direct;
       ++work;
     }
@@ -520,9 +639,9 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
     if (a1) d1 = *a1;
     uint4 d2 = d1;
     if (a2) d2 = *a2;
-    uint32_t v4 = 0;
-    if (w4) v4 = *w4;
-    if (w1) v4 = *w1;
+    // ops 3 and 5: word `sub`; the pair byte: byte `sub`
+    const uint32_t word = pick4(pair_read ? sub >> 2 : sub, d1.x, d1.y, d1.z, d1.w);
+    const uint32_t v4 = pair_read ? (word >> ((sub & 3u) << 3)) & 255u : word;
 
     // Text comparison of the lanes in phase 4, one such lane at a time with
     // the whole wave: lane i checks position x - i of that lane's read against
@@ -614,8 +733,18 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       nl = base + o1 + 1u;
       nu = base + o2;
       nb = (uint32_t)x;
+    } else if (op == 6) {  // the interval at x, recorded by the stretch's walker
+      nl = sub ? d1.z : d1.x;
+      nu = sub ? d1.w : d1.y;
+      nb = (uint32_t)x;
     }
-    if (w1) {
+    // How the task goes on: 0 continues, 1 pruned or a table miss (no
+    // candidate, 2145), 2 its interval emptied, 3 it reached the read start;
+    // `prep`: settle x on the next base (phase 1) and see whether a direct
+    // run starts.  Each tail below has one copy for every lane.
+    int res = 0;
+    bool prep = false;
+    if (pair_read) {
       // the pair byte: bit code(e-1) says whether this task's window
       // read[e-19, e) occurs, bit 4 + code(e-20) whether the next task's
       // window read[e-20, e-1) does; an absent window ends the task as an
@@ -623,11 +752,10 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
       known_e = e - 1;
       known_p = (v4 >> (4u + ((uint32_t)(w20 >> 38) & 3u))) & 1u;
       if (!((v4 >> ((uint32_t)w20 & 3u)) & 1u))
-        next_task();
+        res = 1;
       else
         probe = false;
-    } else if (op >= 3) {
-      int res = 0;
+    } else if (op >= 3 && op <= 5) {
       if (op == 3) {  // the text position of read[b]
         tq = v4;
         if (tq == 0)
@@ -638,6 +766,7 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         int oc = 0;  // 0: chunk used up, load the next one
         if (run_kind == 4) {
           oc = 4;
+          b = (uint32_t)(x - (int32_t)run_g);  // pruned at the position after the matched ones
         } else {
           if (run_g) {
             b = (uint32_t)(x - (int32_t)run_g + 1);  // one row, run_g positions further left
@@ -661,24 +790,12 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
           res = run_end(oc);
       } else {  // op 5: the row of read[b .. e)
         l = u = v4;
-        if (rout == 3) {
-          phase = 1;
-          res = prepare_x();  // consumes the N as a backward step would
-          if (res == 0) try_direct();
-        } else {
+        if (rout == 3)
+          prep = true;  // consumes the N as a backward step would
+        else
           res = rout == 1 ? 2 : 3;
-        }
       }
-      if (res >= 2) finish_task();
-      if (res == 3 && P.smem)
-        end_stream();
-      else if (res != 0)
-        next_task();
     } else if (op != 0) {
-      // how the task goes on: 0 continues, 1 pruned or a table miss (no
-      // candidate, 2145), 2 its interval emptied, 3 it reached the read
-      // start; the task logic below is inlined once for every lane
-      int res = 0;
       if (nl > nu) {
         // An absent clean tlen-base window: the task's candidate could only be
         // shorter than tlen.  With k >= tlen it can emit nothing, and what
@@ -696,40 +813,51 @@ __global__ void __launch_bounds__(BS) seed_persistent(gs::PersistParams P) {
         res = 1;  // the same holds for an absent clean (tlen + 3)-base window with k >= tlen + 3
         if (pair_on) probe = true;
       } else {
-        if (pair_on && op == 1 && pbit < 64) probe = false;  // a present clean 19-base window
+        // a present clean 19-base window (a jump's window occurs too)
+        if (pair_on && ((op == 1 && pbit < 64) || op == 6)) probe = false;
         if (op == 2) ones = nl == nu ? ones + 1u : 0u;
         l = nl;
         u = nu;
-        s = u - l + 1u;
         b = nb;
-        if (prune_at((int32_t)b, s)) {
+        if (tandem_on) t_record();
+        if (prune_at((int32_t)b, u - l + 1u)) {
           res = 1;
         } else {
           x = (int32_t)b - 1;
-          phase = 1;
-          res = prepare_x();
-          if (res == 0) try_direct();
+          prep = true;
         }
       }
-      if (res >= 2) finish_task();
-      if (res == 3 && P.smem)
-        end_stream();  // later tasks of the item end at the read start too and are filtered out
-      else if (res != 0)
-        next_task();
+    }
+    if (prep) {
+      phase = 1;
+      res = prepare_x();
+      if (res == 0) try_direct();
+    }
+    if (res >= 2) finish_task();
+    if (res == 3 && P.smem)
+      end_stream();  // later tasks of the item end at the read start too and are filtered out
+    else if (res != 0)
+      need_next = true;
+    if (need_next) {  // the one place a task ends into the next one
+      need_next = false;
+      next_task();
     }
     if (P.dyn_cap) fold_running();
   }
-  if (P.counters) {
+  if (C->counters) {
     // wave totals, one atomic per wave
+    uint64_t c0 = n_table, c1 = n_occ, c2 = n_direct, c3 = n_hist;
     for (int d = 32; d > 0; d >>= 1) {
-      n_table += __shfl_down(n_table, d, 64);
-      n_occ += __shfl_down(n_occ, d, 64);
-      n_direct += __shfl_down(n_direct, d, 64);
+      c0 += __shfl_down(c0, d, 64);
+      c1 += __shfl_down(c1, d, 64);
+      c2 += __shfl_down(c2, d, 64);
+      c3 += __shfl_down(c3, d, 64);
     }
     if (lane == 0) {
-      atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)n_table);
-      atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)n_occ);
-      atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)n_direct);
+      atomicAdd((unsigned long long*)&C->counters[0], (unsigned long long)c0);
+      atomicAdd((unsigned long long*)&C->counters[1], (unsigned long long)c1);
+      atomicAdd((unsigned long long*)&C->counters[2], (unsigned long long)c2);
+      atomicAdd((unsigned long long*)&C->counters[3], (unsigned long long)c3);
     }
   }
 }
@@ -880,11 +1008,50 @@ int grid_t(unsigned* grid) {
   return GS_OK;
 }
 
+// one thread: the launch's cold fields into their device slot
+__global__ void write_cold(gs::PersistCold c, gs::PersistCold* dst) { *dst = c; }
+
 template <typename AT, int LMAX, int BS>
-int launch_t(gs::PersistParams P, unsigned grid, hipStream_t st) {
-  P.split_below = grid * (uint32_t)BS / 2;  // half the lanes of the launch
-  P.refill_group = gs::kRefillGroup;
-  hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, P);
+int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
+  gs::PersistCold c;
+  c.code_off = P.code_off;
+  c.lens = P.lens;
+  c.task_off = P.task_off;
+  c.items = P.items;
+  c.cand = P.cand;
+  c.nF = P.nF;
+  c.nR = P.nR;
+  c.seg_meta = P.seg_meta;
+  c.seg_link = P.seg_link;
+  c.counters = P.counters;
+  c.dyn_items = P.dyn_items;
+  c.dyn_meta = P.dyn_meta;
+  c.dyn_link = P.dyn_link;
+  c.dyn_ready = P.dyn_ready;
+  c.dyn_ctl = P.dyn_ctl;
+  c.wmeta = P.wmeta;
+  c.wlink = P.wlink;
+  c.seg_tasks = P.seg_tasks;
+  c.epoch = P.epoch;
+  gs::PersistCold* dst = reinterpret_cast<gs::PersistCold*>(reinterpret_cast<uint8_t*>(P.counters) + gs::kCounterBytes);
+  hipLaunchKernelGGL(write_cold, dim3(1), dim3(1), 0, st, c, dst);
+  PersistArgs A;
+  A.X = P.X;
+  A.codes = P.codes;
+  A.cold = dst;
+  A.queue = P.queue;
+  A.hist = P.hist;
+  A.n_items = P.n_items;
+  A.k = P.k;
+  A.smem = P.smem;
+  A.direct_min = P.direct_min;
+  A.dyn_cap = P.dyn_cap;
+  A.split_work = P.split_work;
+  A.split_tasks = P.split_tasks;
+  A.split_below = grid * (uint32_t)BS / 2;  // half the lanes of the launch
+  A.refill_group = gs::kRefillGroup;
+  A.tandem = P.tandem;
+  hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, A);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
@@ -940,7 +1107,8 @@ int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid
     hipLaunchKernelGGL(zero_words, dim3(3), dim3(64), 0, st, P.dyn_ctl, 192u);
     hipLaunchKernelGGL(zero_words, dim3((P.dyn_cap + 63) / 64), dim3(64), 0, st, P.dyn_ready, P.dyn_cap);
   }
-  if (P.counters) hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 6u);
+  if (!P.counters) return GS_EINVAL;  // its buffer also holds the launch's PersistCold
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 8u);
   constexpr int L = kPersistLmax;
   switch (variant * 1000 + (wg ? wg : 128)) {
     case 64: return launch_t<uint8_t, L, 64>(P, grid, st);

@@ -791,7 +791,7 @@ struct gs_batch {
   uint64_t n_reads = 0, span = 0, tasks = 0;
   uint32_t maxlen = 0;
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
-  DevBuf codes, code_off, queue, counters;  // persistent kernel
+  DevBuf codes, code_off, queue, counters, hist;  // persistent kernel (hist: tandem history of its grid)
   DevBuf items, seg_meta, long_reads, item0, item_tmp;  // item_tmp: counts and scans of the item plan
   DevBuf seg_link, dyn_items, dyn_meta, dyn_link, dyn_ready, dyn_ctl, wmeta, wlink;  // work splitting
   uint64_t code_chunks = 0, n_items = 0;
@@ -882,6 +882,7 @@ int gs::plan_check(const gs_opts* o) {
   if (o->split_tasks > 64 && o->split_tasks != 0xFFFF) return GS_EINVAL;
   if (o->split_work == 0xFFFF) return GS_EINVAL;
   if (o->direct_min > 64 && o->direct_min != 0xFFFF) return GS_EINVAL;
+  if (o->tandem > gs::kTandemMax && o->tandem != 0xFFFF) return GS_EINVAL;
   return GS_OK;
 }
 
@@ -987,7 +988,7 @@ static int ensure_buffers(gs_batch* B, uint64_t span, uint64_t n, uint64_t tasks
       B->nF.ensure(4 * nr1) || B->nR.ensure(4 * nr1) || B->ns.ensure(4 * nr1) || B->np.ensure(4 * nr1) ||
       B->soff.ensure(8 * nr1) || B->poff.ensure(8 * nr1) || B->tiles.ensure(8 * (nr1 / kScanTile + 2)) ||
       B->tot.ensure(16) || (maxlen > 256 && B->scratch.ensure(4 * (span + 16))) ||
-      B->queue.ensure(16) || B->counters.ensure(32) ||
+      B->queue.ensure(16) || B->counters.ensure(gs::kCounterBytes + gs::kColdBytes) ||
       B->seg_link.ensure(n_long ? 4 * n_items : 16) ||
       (dyn_cap &&
        (B->dyn_items.ensure(16ull * dyn_cap) || B->dyn_meta.ensure(16ull * dyn_cap) ||
@@ -1118,6 +1119,17 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   B->dyn_cap = split_on ? (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, B->n_items)) : 0u;
   if (ensure_buffers(B, B->span, n, B->tasks, B->code_chunks, B->n_items, B->n_long, B->maxlen, B->dyn_cap))
     return GS_ENOMEM;
+  // the tandem history of the largest persistent grid (any batched kernel
+  // may run the batch); a plain-kernel batch has none
+  if (P.opts.kernel != 0 && P.opts.tandem != 0xFFFF && n) {
+    uint64_t need = 0;
+    for (int v = 0; v < 3; ++v) {
+      unsigned grid = 0;
+      if (gs::persistent_grid(v, P.opts.wg, &grid) == GS_OK)
+        need = std::max<uint64_t>(need, (uint64_t)grid * (P.opts.wg ? P.opts.wg : 128) * gs::kTandemLaneBytes);
+    }
+    if (need && B->hist.ensure(need)) return GS_ENOMEM;
+  }
   hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
   if ((rc = staged_upload(B, B->bases.as<char>(), P.pieces, P.span))) return rc;
@@ -1267,6 +1279,11 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   PP.split_tasks = B->opts.split_tasks ? B->opts.split_tasks : 8u;
   PP.split_work = 64u * (B->opts.split_work ? B->opts.split_work : 32u);
   PP.epoch = 1;  // dyn_ready is cleared by launch_persistent before every launch
+  // tandem jumps where the history holds the launch's grid (batch_load sizes it)
+  const uint64_t hist_need = (uint64_t)pgrid * (B->opts.wg ? B->opts.wg : 128) * gs::kTandemLaneBytes;
+  const bool tandem = B->opts.tandem != 0xFFFF && B->hist.p && B->hist.cap >= hist_need;
+  PP.tandem = tandem ? (B->opts.tandem ? std::min<uint32_t>(B->opts.tandem, gs::kTandemMax) : gs::kTandemMax) : 0u;
+  PP.hist = tandem ? B->hist.as<uint2>() : nullptr;
   gs::JoinParams J;
   J.long_reads = B->long_reads.as<uint32_t>();
   J.item0 = B->item0.as<uint64_t>();
@@ -1401,11 +1418,12 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
     stats->n_seeds = B->n_seeds;
     stats->n_pos = B->n_pos;
     if (kernel >= 1) {
-      uint64_t c[3] = {0, 0, 0};
-      if (hipMemcpy(c, B->counters.p, 24, hipMemcpyDeviceToHost) == hipSuccess) {
+      uint64_t c[4] = {0, 0, 0, 0};
+      if (hipMemcpy(c, B->counters.p, 32, hipMemcpyDeviceToHost) == hipSuccess) {
         stats->n_table_reads = c[0];
         stats->n_occ_reads = c[1];
         stats->n_direct_reads = c[2];
+        stats->n_hist_reads = c[3];
       }
     }
   }
@@ -1885,7 +1903,7 @@ extern "C" GS_API int gs_seed_noalloc(gs_index* X, const char* bases, const uint
                  o_codes = kernel ? take(16 * (chunks + 1)) : 0, o_cand = take(16 * (2 * tasks + 1)),
                  o_nF = take(4 * nr1), o_nR = take(4 * nr1), o_ns = take(4 * nr1), o_np = take(4 * nr1),
                  o_soff = take(8 * nr1), o_poff = take(8 * nr1), o_tiles = take(8 * (nr1 / kScanTile + 2)),
-                 o_tot = take(16), o_queue = take(16), o_counters = take(32),
+                 o_tot = take(16), o_queue = take(16), o_counters = take(gs::kCounterBytes + gs::kColdBytes),
                  o_long = long_scratch ? take(4 * (span + 16)) : 0;
   if (at > kNoAllocBytes) return GS_ERANGE;
   const uint64_t fixed = at;
@@ -1937,7 +1955,7 @@ extern "C" GS_API int gs_seed_noalloc(gs_index* X, const char* bases, const uint
   B.tiles.view(base + o_tiles, 8 * (nr1 / kScanTile + 2));
   B.tot.view(base + o_tot, 16);
   B.queue.view(base + o_queue, 16);
-  B.counters.view(base + o_counters, 32);
+  B.counters.view(base + o_counters, gs::kCounterBytes + gs::kColdBytes);
   if (long_scratch) B.scratch.view(base + o_long, 4 * (span + 16));
 
   // inputs, then the plan on the device

@@ -278,10 +278,18 @@ typedef struct {
   uint16_t direct_min;  /* batched kernels on an index with gs_index_set_direct:
                        a task reads the text once its interval has kept one
                        row for this many backward steps, 1..64 (0 = 4);
-                       0xFFFF: never.  The output depends on none of the
-                       tuning fields (kernel, seg_tasks, wg, walk_rows,
-                       split_*, direct_min)                                  */
-  uint16_t pad;
+                       0xFFFF: never                                         */
+  uint16_t tandem;     /* batched kernels: tandem jumps for repeat periods up
+                       to this, 1..6 (0 = 6); 0xFFFF: off.  Inside a stretch
+                       of the read with period p, the interval of read[y, e)
+                       is the one read[y + p, e + p) had, so after p task
+                       ends have extended through the stretch and recorded
+                       their intervals in device memory (about 1.6 GB per
+                       batch), each later task end of the stretch reads one
+                       recorded interval instead of its lookup and steps.
+                       The output depends on none of the tuning fields
+                       (kernel, seg_tasks, wg, walk_rows, split_*,
+                       direct_min, tandem)                                   */
 } gs_opts;
 
 typedef struct {
@@ -359,6 +367,7 @@ typedef struct {
   uint64_t n_table_reads, n_occ_reads; /* memory operations of the batched
                       extension kernel (0 for the plain kernel)                */
   uint64_t n_direct_reads; /* its suffix-array, text and inverse reads       */
+  uint64_t n_hist_reads;   /* its recorded tandem intervals read (gs_opts.tandem) */
 } gs_batch_stats;
 
 GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,

@@ -327,8 +327,41 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair, hist, tandem_bad, stretches;
 } work_t;
+
+#define TANDEM_CAP_MAX 256u
+
+/* the interval of the clean pattern v[b, e), e - b >= 13: its 13-mer table
+ * entry, then one backward step per further base */
+static void interval_of(const or_index* X, const uint8_t* v, uint32_t b, uint32_t e, uint32_t* l, uint32_t* u) {
+  uint32_t p13 = 0;
+  for (uint32_t q = 0; q < PRECALC; ++q) p13 |= (uint32_t)v[e - 1 - q] << (q << 1);
+  const uint64_t t = X->table ? X->table[p13] : kmer13(X, p13);
+  *l = (uint32_t)t;
+  *u = (uint32_t)(t >> 32);
+  for (uint32_t q = e - PRECALC; q-- > b && *l <= *u;) step(X, l, u, v[q]);
+}
+
+/* whether the clean 19-base window v[e-19, e) occurs in the text */
+static int window19(const or_index* X, const uint8_t* v, uint32_t e) {
+  uint32_t l, u;
+  interval_of(X, v, e - 19, e, &l, &u);
+  return l <= u;
+}
+
+/* the smallest period p <= pmax of the 20 codes v[e-20, e), all clean:
+ * v[z] == v[z + p] for every z in [e-20, e-p); 0 if none */
+static uint32_t tandem_period(const uint8_t* v, uint32_t e, uint32_t pmax) {
+  for (uint32_t z = e - 20; z < e; ++z)
+    if (v[z] > 3) return 0;
+  for (uint32_t p = 1; p <= pmax; ++p) {
+    uint32_t z = e - 20;
+    while (z + p < e && v[z] == v[z + p]) ++z;
+    if (z + p == e) return p;
+  }
+  return 0;
+}
 
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
   return row - (row >= X->primary ? 1u : 0u);
@@ -351,10 +384,18 @@ static void count_blocks(const or_index* X, uint32_t l, uint32_t u, work_t* w) {
  * change which steps run and are counted (see or_opts). */
 static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t L, uint32_t k,
                                   int prune, int smem_stop, uint32_t ext_len, uint32_t sat,
-                                  uint32_t direct_min, int miss_skip, int pair_probe, int smem, cand_t* out,
-                                  uint32_t* A, work_t* w) {
+                                  uint32_t direct_min, int miss_skip, int pair_probe, int smem,
+                                  const or_opts* o, cand_t* out, uint32_t* A, work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
+  /* tandem-jump model (or_opts.tandem): the open stretch [tR, tE) of period
+   * tp (0: none), and per walker j the recorded positions [hlo[j], hhi[j]]
+   * (empty while hhi < hlo) with their intervals in H */
+  const uint32_t TP = (o->tandem && prune) ? (o->tandem > 8 ? 8 : o->tandem) : 0;
+  const uint32_t cap = o->tandem_cap > TANDEM_CAP_MAX ? TANDEM_CAP_MAX : o->tandem_cap;
+  uint32_t tp = 0, tR = 0, tE = 0, prev_reach = 0;
+  int32_t hlo[8], hhi[8];
+  static __thread uint64_t H[8 * TANDEM_CAP_MAX];
   /* pair-table model (or_opts.pair_probe), with the 16-base extended table:
    * probe is set while the last looked-up clean 19-base window was absent;
    * known_e is the task end whose window the last pair byte also answered */
@@ -377,7 +418,56 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     if (bad) continue;
     uint32_t l = 0, u = 0, b = 0;
     int have = 0;
-    if (ext_len > PRECALC && e >= ext_len) {
+    /* a task whose window the last pair byte found absent is skipped before
+     * anything else (the device's next_task); it leaves prev_reach alone */
+    const int known_absent = pair && known_e == (int64_t)e && !window19(X, v, e);
+    /* tandem model: this task walks for the stretch (walker >= 0), or jumps */
+    int walker = -1, jump = 0;
+    if (TP && !known_absent) {
+      if (tp) {
+        if (tE - e < tp) {
+          walker = (int)(tE - e);
+        } else {
+          const uint32_t j = (tE - e) % tp, d = tE - j - e;
+          if (hhi[j] >= hlo[j]) {
+            const int32_t ys = hlo[j] - (int32_t)d > (int32_t)tR ? hlo[j] - (int32_t)d : (int32_t)tR;
+            if (ys + (int32_t)d <= hhi[j]) {
+              const uint64_t iv = H[j * cap + (uint32_t)ys + d - tR];
+              l = (uint32_t)iv;
+              u = (uint32_t)(iv >> 32);
+              b = (uint32_t)ys;
+              jump = have = 1;
+              ++w->hist;
+              if (o->tandem_check) {
+                uint32_t cl, cu;
+                interval_of(X, v, b, e, &cl, &cu);
+                if (cl != l || cu != u) ++w->tandem_bad;
+              }
+              if (pair) probe = 0;
+            }
+          }
+          if (!jump) tp = 0; /* the stretch closes */
+        }
+      }
+      if (!tp && prev_reach >= o->tandem_gate && e >= 20) {
+        const uint32_t q = tandem_period(v, e, TP);
+        if (q) {
+          int32_t z = (int32_t)e - 21;
+          while (z >= 0 && v[z] <= 3 && v[z] == v[z + q]) --z;
+          if (e - (uint32_t)(z + 1) >= o->tandem_span) {
+            tp = q;
+            tR = (uint32_t)(z + 1);
+            tE = e;
+            walker = 0;
+            for (uint32_t j = 0; j < 8; ++j) hlo[j] = 0, hhi[j] = -1;
+            ++w->stretches;
+          }
+        }
+      }
+    }
+    if (jump) {
+      /* the recorded interval stands for the lookup and the steps above b */
+    } else if (ext_len > PRECALC && e >= ext_len) {
       int clean = 1;
       for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
       if (clean) { /* one lookup answers the first ext_len bases when they occur */
@@ -411,7 +501,10 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         } else {
           ++w->ext;
         }
-        if (absent3) continue; /* absent: as below, with k >= ext_len + 3 */
+        if (absent3) { /* absent: as below, with k >= ext_len + 3 */
+          if (!known_absent) prev_reach = 0;
+          continue;
+        }
         if (l <= u) {
           have = 1;
           b = e - ext_len;
@@ -420,6 +513,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
            * reaches ext_len bases, let alone k: it can emit nothing, and what
            * else it leaves (its left end for the in-lane SMEM filter, its
            * size records) cannot change any output (see or_opts.miss_skip) */
+          if (!known_absent) prev_reach = 0;
           continue;
         }
       }
@@ -429,9 +523,23 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
       l = (uint32_t)iv;
       u = (uint32_t)(iv >> 32);
-      if (l > u) continue; /* 2145 */
+      if (l > u) { /* 2145 */
+        prev_reach = 0;
+        continue;
+      }
       b = e - PRECALC;
     }
+    /* a walker records the interval at every position it computes one for,
+     * while the positions stay contiguous (not after a direct run) */
+#define TANDEM_RECORD()                                                                  \
+  do {                                                                                   \
+    if (walker >= 0 && b >= tR + tp && b - tR < cap && (hhi[walker] < hlo[walker] || (int32_t)b == hlo[walker] - 1)) { \
+      H[(uint32_t)walker * cap + b - tR] = (uint64_t)l | ((uint64_t)u << 32);            \
+      if (hhi[walker] < hlo[walker]) hhi[walker] = (int32_t)b;                           \
+      hlo[walker] = (int32_t)b;                                                          \
+    }                                                                                    \
+  } while (0)
+    if (!jump) TANDEM_RECORD();
     int pruned = 0, reached = 1;
     if (prune) {
       uint32_t s = u - l + 1;
@@ -476,6 +584,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       u = u2;
       b = (uint32_t)x; /* 2252-2255 */
       if (run) --tq;
+      if (!run && c <= 3) TANDEM_RECORD();
       if (prune) {
         uint32_t s = u - l + 1;
         if (sat && s >= sat) s = sat;
@@ -483,6 +592,8 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         A[b] = s;
       }
     }
+#undef TANDEM_RECORD
+    prev_reach = e - b;
     if (pruned) continue;
     const int emit = e - b >= k && (!smem || b < last_b);
     if (run && emit) ++w->direct; /* the row where the run ended */
@@ -548,13 +659,13 @@ static void phase_candidates(void* vctx, uint64_t i, int tid) {
   const uint32_t sat = C->o->prune_sat;
   const uint32_t dmin = C->o->direct_min;
   uint32_t nf = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip,
-                                  C->o->pair_probe, C->o->smem, tmp + ntask, A, &C->work[tid]);
+                                  C->o->pair_probe, C->o->smem, C->o, tmp + ntask, A, &C->work[tid]);
   /* forward: decreasing end -> slot order (increasing end) */
   for (uint32_t q = 0; q < nf; ++q) Ls->F[q] = tmp[ntask + nf - 1 - q];
   Ls->nF = nf;
   for (uint32_t q = 0; q < L; ++q) v[q] = (uint8_t)or_code_rev(s[L - 1 - q]);
   uint32_t nr = strand_candidates(C->X, v, L, C->o->k, C->o->prune, stop, ext, sat, dmin, C->o->miss_skip,
-                                  C->o->pair_probe, C->o->smem, Ls->R, A, &C->work[tid]);
+                                  C->o->pair_probe, C->o->smem, C->o, Ls->R, A, &C->work[tid]);
   for (uint32_t q = 0; q < nr; ++q) { /* virtual [b,e) -> reverse (L-e, L-b) with strand bits */
     cand_t c = Ls->R[q];
     Ls->R[q] = (cand_t){(L - c.e) | 0x80000000u, (L - c.b) | 0x80000000u, c.l, c.u};
@@ -846,6 +957,9 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_direct += work[t].direct;
     res->w_text += work[t].text;
     res->w_pair += work[t].pair;
+    res->w_hist += work[t].hist;
+    res->w_tandem_bad += work[t].tandem_bad;
+    res->w_stretches += work[t].stretches;
     free(scratch[t]);
   }
   if (!o->prune) res->w_steps_unpruned = res->w_steps;

@@ -113,6 +113,29 @@ typedef struct {
    * is absent.  A present window is then looked up as usual (w_ext).  Output
    * unchanged; counters only.  0: off. */
   int pair_probe;
+  /* tandem: work model of the device's tandem jumps (maximum period, at most
+   * 8; 0: off).  Inside a stretch [R, E) of the read with period p (code(z)
+   * == code(z + p) <= 3 for every z in [R, E - p)), the pattern read[y, e)
+   * equals read[y + d, e + d) for d a multiple of p and y >= R, e + d <= E,
+   * so its interval is the one task e + d had at position y + d.  A task
+   * that starts when the previous task's left end lay at least tandem_gate
+   * positions below its end, and whose 20 codes left of its end have a
+   * period p <= tandem (the smallest one), finds R by scanning left; if E - R
+   * >= tandem_span (E = its end) it opens a stretch: it and the next p - 1
+   * task ends walk as usual and record the interval at every position they
+   * compute it for in [R + p, R + tandem_cap) (not in direct runs).  Any
+   * later task e of the stretch, with walker j = (E - e) mod p and d = E - j
+   * - e, reads one recorded interval (w_hist) instead of its table lookup
+   * and the steps down to y = max(R, lowest recorded position of j - d), if
+   * y + d was recorded, and extends from y as after a lookup; else the
+   * stretch closes.  Output unchanged (intervals are exact; the skipped
+   * size records only prune); counters only.  Needs prune. */
+  uint32_t tandem;
+  uint32_t tandem_gate, tandem_span, tandem_cap;
+  /* tandem_check: recompute every jumped-to interval from the text of the
+   * read (table lookup and steps) and count disagreements in
+   * w_tandem_bad. */
+  int tandem_check;
 } or_opts;
 
 typedef struct {
@@ -146,6 +169,9 @@ typedef struct {
                              suffix-array and inverse reads and, of them,     */
   uint64_t w_text;        /* 16-byte text reads                               */
   uint64_t w_pair;        /* pair-table bytes read (opts.pair_probe)          */
+  uint64_t w_hist;        /* recorded intervals read (opts.tandem)            */
+  uint64_t w_tandem_bad;  /* jumped-to intervals that disagree (tandem_check) */
+  uint64_t w_stretches;   /* tandem stretches opened                          */
 } or_result;
 
 /* Seeds reads[0..n): bases+offsets[i] holds lens[i] bytes of read i (raw line

@@ -24,7 +24,7 @@ static double now(void) {
 }
 
 int main(int argc, char** argv) {
-  or_opts o = {20, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  or_opts o = {20, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   int print = 0, c;
   while ((c = getopt(argc, argv, "k:sot:p:m:T")) >= 0) {
     switch (c) {

@@ -47,6 +47,44 @@ def repetitive_text(rng, n, n_units=4, unit_len=40, copies=6, div=0.03, tandem=2
     return "".join(t)
 
 
+def tandem_text(rng, n, n_arrays=60, pmax=6, len_min=30, len_max=400, n_units=12):
+    """Random text with exact tandem arrays of periods 1..pmax; arrays draw
+    their unit from a small pool, so a unit recurs in several arrays (large
+    intervals, as in a genome's microsatellites).  Returns the text and the
+    (start, length) of every array."""
+    t = list(random_text(rng, n))
+    units = [random_text(rng, rng.randint(1, pmax)) for _ in range(n_units)]
+    arrays = []
+    for _ in range(n_arrays):
+        unit = rng.choice(units)
+        ln = rng.randint(len_min, len_max)
+        p = rng.randrange(0, n - ln)
+        for j in range(ln):
+            t[p + j] = unit[j % len(unit)]
+        arrays.append((p, ln))
+    return "".join(t), arrays
+
+
+def reads_over(rng, text, arrays, nreads, lmin, lmax, sub=0.01, n_frac=0.05):
+    """Reads (both strands) placed over the given (start, length) stretches:
+    each read covers part of a stretch and often a flank, with substitutions
+    and, in a fraction of reads, an N."""
+    comp = {"A": "T", "C": "G", "G": "C", "T": "A", "N": "N"}
+    out = []
+    n = len(text)
+    for _ in range(nreads):
+        L = rng.randint(lmin, lmax)
+        p0, ln = rng.choice(arrays)
+        p = min(max(0, p0 + rng.randint(-L // 2, ln - L // 2)), n - L)
+        s = [rng.choice("ACGT".replace(ch, "")) if rng.random() < sub else ch for ch in text[p:p + L]]
+        if rng.random() < n_frac:
+            s[rng.randrange(L)] = "N"
+        if rng.random() < 0.5:
+            s = [comp[c] for c in reversed(s)]
+        out.append("".join(s).encode())
+    return out
+
+
 def sample_reads(rng, text, nreads, lmin, lmax, sub=0.02, indel=0.005, n_frac=0.1, extra=True):
     """Reads as raw line bytes, sampled from both strands with errors."""
     out = []

@@ -18,7 +18,9 @@ class OrOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("smem", C.c_int), ("prune", C.c_int),
                 ("max_occ", C.c_uint32), ("threads", C.c_int), ("use_table", C.c_int),
                 ("smem_stop", C.c_int), ("ext_len", C.c_int), ("prune_sat", C.c_uint32),
-                ("direct_min", C.c_uint32), ("miss_skip", C.c_int), ("pair_probe", C.c_int)]
+                ("direct_min", C.c_uint32), ("miss_skip", C.c_int), ("pair_probe", C.c_int),
+                ("tandem", C.c_uint32), ("tandem_gate", C.c_uint32), ("tandem_span", C.c_uint32),
+                ("tandem_cap", C.c_uint32), ("tandem_check", C.c_int)]
 
 
 class OrSeed(C.Structure):
@@ -35,10 +37,18 @@ class OrResult(C.Structure):
                 ("w_tasks", C.c_uint64), ("w_steps", C.c_uint64), ("w_blocks64", C.c_uint64),
                 ("w_blocks96", C.c_uint64), ("w_lf", C.c_uint64), ("w_sa", C.c_uint64),
                 ("w_steps_unpruned", C.c_uint64), ("n_short", C.c_uint64), ("w_ext", C.c_uint64),
-                ("w_direct", C.c_uint64), ("w_text", C.c_uint64), ("w_pair", C.c_uint64)]
+                ("w_direct", C.c_uint64), ("w_text", C.c_uint64), ("w_pair", C.c_uint64),
+                ("w_hist", C.c_uint64), ("w_tandem_bad", C.c_uint64), ("w_stretches", C.c_uint64)]
 
 
 _lib = None
+
+# the device's tandem-jump settings (gs_persist.hip): gate, span, capacity
+TANDEM_GATE, TANDEM_SPAN, TANDEM_CAP = 24, 32, 128
+
+
+def tandem_opts(tandem, check, gate=TANDEM_GATE, span=TANDEM_SPAN, cap=TANDEM_CAP):
+    return tandem, gate, span, cap, check
 
 
 def lib():
@@ -138,11 +148,11 @@ def pack_reads(reads):
 
 
 def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem_stop=0, ext_len=0, prune_sat=0,
-         direct_min=0, miss_skip=0, pair_probe=0):
+         direct_min=0, miss_skip=0, pair_probe=0, tandem=0, tandem_check=0, **tkw):
     """Returns (per-read list of (qbeg, qend, l, u, [positions])), counters dict)."""
     buf, offs, lens = pack_reads(reads)
     o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len, prune_sat, direct_min,
-               miss_skip, pair_probe)
+               miss_skip, pair_probe, *tandem_opts(tandem, tandem_check, **tkw))
     res = OrResult()
     rc = lib().or_seed_reads(index.h, buf if buf else b"\0",
                              offs.ctypes.data_as(C.POINTER(C.c_uint64)),
@@ -164,21 +174,22 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem
             out.append(lst)
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair")}
+                                                 "w_direct", "w_text", "w_pair", "w_hist",
+                                                 "w_tandem_bad", "w_stretches")}
         return out, counters
     finally:
         lib().or_result_free(C.byref(res))
 
 
 def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, use_table=1, smem_stop=0,
-                ext_len=0, prune_sat=0, direct_min=0, miss_skip=0, pair_probe=0):
+                ext_len=0, prune_sat=0, direct_min=0, miss_skip=0, pair_probe=0, tandem=0, tandem_check=0, **tkw):
     """Flat numpy copies of the oracle's result: (seeds[n,6] u32, pos u32,
     read_seed_off u64[n+1], read_pos_off u64[n+1]) plus the work counters."""
     buf = np.ascontiguousarray(buf, dtype=np.uint8)
     offs = np.ascontiguousarray(offs, dtype=np.uint64)
     lens = np.ascontiguousarray(lens, dtype=np.uint32)
     o = OrOpts(k, 1 if smem else 0, prune, max_occ, threads, use_table, smem_stop, ext_len, prune_sat, direct_min,
-               miss_skip, pair_probe)
+               miss_skip, pair_probe, *tandem_opts(tandem, tandem_check, **tkw))
     res = OrResult()
     rc = lib().or_seed_reads(index.h, C.cast(buf.ctypes.data, C.c_char_p),
                              offs.ctypes.data_as(C.POINTER(C.c_uint64)),
@@ -196,7 +207,8 @@ def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, 
         po = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair")}
+                                                 "w_direct", "w_text", "w_pair", "w_hist",
+                                                 "w_tandem_bad", "w_stretches")}
         return seeds, pos, so, po, counters
     finally:
         lib().or_result_free(C.byref(res))

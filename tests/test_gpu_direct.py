@@ -136,7 +136,7 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, smem, k, kernel, 
     ox = ob.Index(prefix)
     arrays = g.pack_reads(reads)
     _, _, _, _, c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1, ext_len=16, prune_sat=sat,
-                                   miss_skip=1, direct_min=3)
+                                   miss_skip=1, direct_min=3, tandem=6)
     with g.Index(prefix) as X:
         X.set_kmer_len(16)
         X.set_direct(True)
@@ -150,3 +150,4 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, smem, k, kernel, 
     assert st.n_table_reads == c["w_tasks"] + c["w_ext"]
     assert st.n_occ_reads == c["w_blocks96"]
     assert st.n_direct_reads == c["w_direct"]
+    assert st.n_hist_reads == c["w_hist"]

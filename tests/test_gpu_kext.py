@@ -106,7 +106,7 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k, 
     ox = ob.Index(prefix)
     arrays = g.pack_reads(reads)
     _, _, _, _, c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1,
-                                   ext_len=length if length > 13 else 0, prune_sat=sat, miss_skip=1)
+                                   ext_len=length if length > 13 else 0, prune_sat=sat, miss_skip=1, tandem=6)
     with g.Index(prefix) as X:
         if length > 13:
             X.set_kmer_len(length)
@@ -117,3 +117,4 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k, 
         (length, smem, st.n_table_reads, st.n_occ_reads, c["w_tasks"] + c["w_ext"], c["w_blocks96"]))
     assert st.n_table_reads == c["w_tasks"] + c["w_ext"]
     assert st.n_occ_reads == c["w_blocks96"]
+    assert st.n_hist_reads == c["w_hist"]
