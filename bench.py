@@ -246,10 +246,10 @@ def serve_bench(g, X, ref, k, max_occ, args):
                          "from a pool of %d reads; Poisson arrivals; latency = submit to results complete" % qp.n_reads)
 
 
-def profiled_counters(config, kernel, kmer_len, direct, pairs):
+def profiled_counters(config, kernel, kmer_len, direct, pairs, tandem):
     """The committed rocprofv3 counter summary of the extension kernel
     (profiles/seed_persistent_config<N>.txt) if it was measured on this
-    workload and index shape, else None."""
+    workload, index shape and kernel options, else None."""
     path = os.path.join(ROOT, "profiles", "seed_persistent_config%d.txt" % config)
     if not os.path.exists(path):
         return None
@@ -259,7 +259,7 @@ def profiled_counters(config, kernel, kmer_len, direct, pairs):
         if len(parts) == 2 and not line.startswith("#"):
             kv[parts[0]] = float(parts[1])
     if (kv.get("kernel") != kernel or kv.get("kmer_len") != kmer_len or kv.get("direct", 0) != direct
-            or kv.get("pairs", 0) != pairs):
+            or kv.get("pairs", 0) != pairs or kv.get("tandem", 0) != tandem):
         return None
     kv["path"] = os.path.relpath(path, ROOT)
     return kv
@@ -658,7 +658,7 @@ def main():
         req_per_read = wm["requests_per_read"]
         issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads + st.n_hist_reads
                      for st in stats) / (len(stats) * args.batch)
-        prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct, pairs)
+        prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct, pairs, 0 if args.no_tandem else 6)
         traffic, traffic_src = profiled_traffic(prof, args.batch)
         achieved = wm["bytes_per_read"] * args.batch / seed_s
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": 8000.0, "unit": "GB/s",
