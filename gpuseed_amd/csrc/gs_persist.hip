@@ -52,6 +52,26 @@ __device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, 
 }
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// 8 4-bit codes (code i in bits 4i..4i+3) -> their low two bits packed (code
+// i in bits 2i, 2i+1) and a bit per code above 3 (bit i)
+__device__ __forceinline__ uint32_t nib_codes(uint32_t x) {
+  uint32_t c = x & 0x33333333u;
+  c = (c | (c >> 2)) & 0x0F0F0F0Fu;
+  c = (c | (c >> 4)) & 0x00FF00FFu;
+  return (c | (c >> 8)) & 0x0000FFFFu;
+}
+__device__ __forceinline__ uint32_t nib_bad(uint32_t x) {
+  uint32_t b = ((x >> 2) | (x >> 3)) & 0x11111111u;
+  b = (b | (b >> 3)) & 0x03030303u;
+  b = (b | (b >> 6)) & 0x000F000Fu;
+  return (b | (b >> 12)) & 0x000000FFu;
+}
+// 32 2-bit fields in reverse order
+__device__ __forceinline__ uint64_t rev2(uint64_t x) {
+  const uint64_t r = __builtin_bitreverse64(x);
+  return ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+}
 constexpr uint32_t kBad13 = (1u << gs::kPrecalc) - 1u;  // the 13-mer window's codes in a 19-code window
 
 __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
@@ -90,16 +110,24 @@ seed_persistent(PersistArgs P) {
   // size records of RB bits per position: 4 (Nib, two per byte, low nibble
   // first), 8 or 16; a record saturates at 2^RB - 1 and then never prunes
   constexpr int RB = std::is_same<AT, Nib>::value ? 4 : 8 * (int)sizeof(AT);
-  constexpr int CODE_BYTES = LMAX / 2;
+  // A lane's LDS slot: the size records (RB bits per position of the
+  // window), then the window's codes at 2 bits in reverse order (position j
+  // at index r = w0 + LMAX - 1 - j, 16 per word, two words of padding), then
+  // a bit per position set for a code above 3 (same index, one padding word
+  // with every bit set: positions left of the read start).  In this order
+  // the 22 codes left of any task end are one run of bits, read with three
+  // word loads.
   constexpr int A_BYTES = LMAX * RB / 8;
-  constexpr int SLOT = CODE_BYTES + A_BYTES;
-  static_assert(SLOT % 16 == 0, "slot must be 16-byte aligned");
+  constexpr int CW = LMAX / 16 + 2;  // code words
+  constexpr int BW = LMAX / 32 + 1;  // words of the code-above-3 bits
+  constexpr int SLOT = (A_BYTES + 4 * (CW + BW) + 15) & ~15;
+  static_assert(A_BYTES % 16 == 0, "records must keep 16-byte alignment");
   constexpr uint32_t SAT = (1u << RB) - 1u;
   __shared__ __align__(16) uint8_t lds[BS * SLOT];
   const uint32_t sbase = threadIdx.x * SLOT;
   // record j of the slot starting at byte `slot`
   auto rec_get = [&](uint32_t slot, int32_t j) __attribute__((always_inline)) -> uint32_t {
-    uint8_t* a = lds + slot + CODE_BYTES;
+    uint8_t* a = lds + slot;
     if constexpr (RB == 4)
       return (a[j >> 1] >> ((j & 1) << 2)) & 15u;
     else if constexpr (RB == 8)
@@ -108,7 +136,7 @@ seed_persistent(PersistArgs P) {
       return reinterpret_cast<uint16_t*>(a)[j];
   };
   auto rec_set = [&](uint32_t slot, int32_t j, uint32_t v) __attribute__((always_inline)) {
-    uint8_t* a = lds + slot + CODE_BYTES;
+    uint8_t* a = lds + slot;
     if constexpr (RB == 4) {
       const uint32_t sh = (j & 1) << 2;
       a[j >> 1] = (uint8_t)((a[j >> 1] & ~(15u << sh)) | (v << sh));
@@ -134,13 +162,13 @@ seed_persistent(PersistArgs P) {
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
   uint32_t r = 0, strand = 0, L = 0, nout = 0, last_b = 0;
-  // the 20 codes left of the task end e: code(e-1-j) at bits 2j of w20 for
-  // j = 0..19, and bit j of bad20 set if that code is above 3 or its position
+  // the 22 codes left of the task end e: code(e-1-j) at bits 2j of wcode for
+  // j = 0..21, and bit j of wbad set if that code is above 3 or its position
   // is left of the read start; the 13-mer pattern is the low 26 bits, an
   // extended window and its 3 presence bases the bits above, and code(e-20)
-  // (the base left of the 19-base window) the top two
-  uint64_t w20 = 0;
-  uint32_t bad20 = 0;
+  // (the base left of the 19-base window) bits 38-39
+  uint64_t wcode = 0;
+  uint32_t wbad = 0;
   // pair table: probe is set while the last clean 19-base window looked up
   // was absent; known_e is the task end whose window the last pair byte
   // also answered, known_p whether that window occurs
@@ -189,11 +217,45 @@ seed_persistent(PersistArgs P) {
   // an extension may run left of it and then reads device memory
   auto code = [&](int32_t i) __attribute__((always_inline)) -> uint32_t {
     const int32_t j = i - w0;
-    if (j >= 0) return (lds[sbase + (j >> 1)] >> ((j & 1) << 2)) & 15u;
+    if (j >= 0) {
+      const uint32_t rr = (uint32_t)(LMAX - 1 - j);
+      const uint32_t* cw = reinterpret_cast<const uint32_t*>(lds + sbase + A_BYTES);
+      if ((cw[CW + (rr >> 5)] >> (rr & 31u)) & 1u) return 4u;
+      return (cw[rr >> 4] >> ((rr & 15u) << 1)) & 3u;
+    }
     const uint32_t w = reinterpret_cast<const uint32_t*>(P.codes + gcodes + (uint32_t)(i >> 5))[(i >> 3) & 3];
     return (w >> ((i & 7) << 2)) & 15u;
   };
 
+  // the 22 codes left of task end ee into wcode / wbad (code(ee-1-j) at bits
+  // 2j, bit j set for a code above 3 or a position left of the read start):
+  // one run of bits of the slot, unless the window reaches left of a
+  // segment's LDS window (then position by position, from device memory)
+  auto load_window = [&](int32_t ee) __attribute__((always_inline)) {
+    if (w0 > 0 && ee - 22 < w0) {
+      uint64_t w = 0;
+      uint32_t bad = 0;
+#pragma unroll 1
+      for (int j = 0; j < 22; ++j) {
+        const int32_t pos = ee - 1 - j;
+        const uint32_t c = pos >= 0 ? code(pos) : 15u;
+        bad |= (c > 3 ? 1u : 0u) << j;
+        w |= (uint64_t)(c & 3u) << (2 * j);
+      }
+      wcode = w;
+      wbad = bad;
+      return;
+    }
+    const uint32_t r1 = (uint32_t)(w0 + LMAX - ee);  // index of position ee - 1
+    const uint32_t* cw = reinterpret_cast<const uint32_t*>(lds + sbase + A_BYTES);
+    const uint32_t wi = r1 >> 4, o = (r1 & 15u) << 1;
+    const uint64_t lo = (uint64_t)cw[wi] | ((uint64_t)cw[wi + 1] << 32);
+    const uint64_t hi = cw[wi + 2];
+    wcode = (o ? (lo >> o) | (hi << (64 - o)) : lo) & ((1ull << 44) - 1ull);
+    const uint32_t bi = CW + (r1 >> 5), bo = r1 & 31u;
+    const uint64_t bb = (uint64_t)cw[bi] | ((uint64_t)cw[bi + 1] << 32);
+    wbad = (uint32_t)(bb >> bo) & 0x3FFFFFu;
+  };
   auto end_stream = [&]() __attribute__((always_inline)) {
     const uint4 meta = make_uint4(nout, first_b, last_b, flags);
     if (kind == 2) {
@@ -211,15 +273,6 @@ seed_persistent(PersistArgs P) {
     }
     have = false;
     --drun;  // folded into `running` once per wave and iteration
-  };
-  auto advance = [&]() __attribute__((always_inline)) -> bool {
-    if (e - 1 < e_min) return false;
-    const int32_t in = e - 21;  // enters the window as position e' - 20 of the next end e' = e - 1
-    const uint32_t cn = in >= 0 ? code(in) : 15u;
-    w20 = (w20 >> 2) | ((uint64_t)(cn & 3u) << 38);
-    bad20 = (bad20 >> 1) | ((cn > 3 ? 1u : 0u) << 19);
-    --e;
-    return true;
   };
   // Hands the lower half of the task ends still to do (e-1 .. e_min) to the
   // overflow queue as one piece and keeps the upper half, whose size records
@@ -280,10 +333,10 @@ seed_persistent(PersistArgs P) {
       }
       tp = 0;  // the stretch ends
     }
-    if (gate && e >= 20 && bad20 == 0) {
+    if (gate && e >= 20 && !(wbad & 0xFFFFFu)) {
       uint32_t q = 0;
       for (uint32_t p = 1; p <= P.tandem; ++p) {
-        if (!((w20 ^ (w20 >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) {
+        if (!((wcode ^ (wcode >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) {
           q = p;
           break;
         }
@@ -317,18 +370,25 @@ seed_persistent(PersistArgs P) {
         __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
         __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
       split();
+    // the next task end: past the one whose window the last pair byte found
+    // absent (its window is clean), then past 13-base windows holding a code
+    // above 3, jumping below the highest such code at once
+    int32_t ne = e - 1;
+    if (known_e == ne && !known_p) {
+      --ne;
+      probe = true;
+    }
     while (true) {
-      if (!advance()) {
+      if (ne < e_min) {
         end_stream();
         return;
       }
-      if (bad20 & kBad13) continue;
-      if (known_e == e && !known_p) {  // the last pair byte said this window is absent: no lookup
-        probe = true;
-        continue;
-      }
-      break;
+      load_window(ne);
+      const uint32_t m13 = wbad & kBad13;
+      if (!m13) break;
+      ne -= 32 - (int32_t)__clz(m13);
     }
+    e = ne;
     b = (uint32_t)e;  // the task's left end until its lookup
     if (tandem_on) t_start(gate);
   };
@@ -413,11 +473,27 @@ seed_persistent(PersistArgs P) {
       top = ((L - t0) + 31u) & ~31u;
       w0 = top > (uint32_t)LMAX ? (int32_t)(top - LMAX) : 0;
     }
+    // each 32-code chunk of the window, reversed: chunk c holds positions
+    // 32c .. 32c + 31, which get the slot indices 32q + 31 .. 32q with
+    // q = LMAX/32 - 1 - (c - c0)
     const uint32_t c0 = (uint32_t)w0 >> 5, c1 = top >> 5;
-    for (uint32_t c = c0; c < c1; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * (c - c0)) = src[c];
+    uint32_t* cw = reinterpret_cast<uint32_t*>(lds + sbase + A_BYTES);
+    for (uint32_t c = c0; c < c1; ++c) {
+      const uint4 v = src[c];
+      const uint64_t f = (uint64_t)nib_codes(v.x) | ((uint64_t)nib_codes(v.y) << 16) |
+                         ((uint64_t)nib_codes(v.z) << 32) | ((uint64_t)nib_codes(v.w) << 48);
+      const uint32_t fb = nib_bad(v.x) | (nib_bad(v.y) << 8) | (nib_bad(v.z) << 16) | (nib_bad(v.w) << 24);
+      const uint32_t q = (uint32_t)(LMAX / 32 - 1) - (c - c0);
+      const uint64_t rv = rev2(f);
+      cw[2 * q] = (uint32_t)rv;
+      cw[2 * q + 1] = (uint32_t)(rv >> 32);
+      cw[CW + q] = __builtin_bitreverse32(fb);
+    }
+    cw[CW - 2] = 0;
+    cw[CW - 1] = 0;
+    cw[CW + BW - 1] = 0xFFFFFFFFu;  // left of the read start (a segment's reads left of w0 go to device memory)
     const uint32_t za = ((top - (uint32_t)w0) * (uint32_t)RB / 8 + 15) >> 4;
-    for (uint32_t c = 0; c < za; ++c)
-      *reinterpret_cast<uint4*>(lds + sbase + CODE_BYTES + 16 * c) = make_uint4(0, 0, 0, 0);
+    for (uint32_t c = 0; c < za; ++c) *reinterpret_cast<uint4*>(lds + sbase + 16 * c) = make_uint4(0, 0, 0, 0);
     out = C->cand + 2 * C->task_off[r];
     nout = 0;
     last_b = 0xFFFFFFFFu;
@@ -427,14 +503,7 @@ seed_persistent(PersistArgs P) {
     link = link0;  // set before the item can end (a piece whose windows all hold an N ends here)
     e = (int32_t)(L - t0);
     e_min = e - (int32_t)tcount + 1;
-    w20 = 0;
-    bad20 = 0;
-    for (int j = 0; j < 20; ++j) {
-      const int32_t pos = e - 1 - j;
-      const uint32_t c = pos >= 0 ? code(pos) : 15u;
-      bad20 |= (c > 3 ? 1u : 0u) << j;
-      w20 |= (uint64_t)(c & 3u) << (2 * j);
-    }
+    load_window(e);
     have = true;
     ++drun;
     phase = 0;
@@ -444,7 +513,7 @@ seed_persistent(PersistArgs P) {
     known_e = -1;
     b = (uint32_t)e;
     tp = 0;
-    need_next = (bad20 & kBad13) != 0;  // no work done yet, so this never splits
+    need_next = (wbad & kBad13) != 0;  // no work done yet, so this never splits
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
     item = (uint32_t)it;
@@ -583,23 +652,23 @@ seed_persistent(PersistArgs P) {
     bool pair_read = false;  // op 1 reads the pair byte instead of a table entry
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
-    uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)w20 & 0x3FFFFFFu;  // table lookup: pattern length and entry
+    uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)wcode & 0x3FFFFFFu;  // table lookup: pattern length and entry
     uint32_t pbit = 64;  // presence bit of the clean 3 bases left of an extended window (64: none)
     if (op == 1) {
       a1 = reinterpret_cast<const uint4*>(P.X.kmer) + (tidx >> 1);  // two 8-byte entries per load
       const uint32_t el = P.X.ext_len;
       // the extended window: codes 13 .. el-1 clean and inside the read
-      if (el && !ext_miss && !(bad20 & (((1u << el) - 1u) & ~kBad13))) {
+      if (el && !ext_miss && !(wbad & (((1u << el) - 1u) & ~kBad13))) {
         tlen = el;
-        tidx = (uint32_t)(w20 & ((1ull << (2 * el)) - 1ull));
+        tidx = (uint32_t)(wcode & ((1ull << (2 * el)) - 1ull));
         a1 = P.X.kext + tidx;
         // presence bit c1 + 4*c2 + 16*c3 of the next three bases, if clean
-        if (k >= tlen + 3 && !(bad20 & (7u << el))) pbit = (uint32_t)(w20 >> (2 * el)) & 63u;
+        if (k >= tlen + 3 && !(wbad & (7u << el))) pbit = (uint32_t)(wcode >> (2 * el)) & 63u;
       }
       // after an absent window, a task whose 20 codes are clean reads the
       // pair byte of its 18-base core read[e-19, e-1) instead
-      if (pair_on && probe && known_e != e && !(bad20 & 0xFFFFFu)) {
-        const uint64_t pi = (w20 >> 2) & ((1ull << 36) - 1ull);
+      if (pair_on && probe && known_e != e && !(wbad & 0xFFFFFu)) {
+        const uint64_t pi = (wcode >> 2) & ((1ull << 36) - 1ull);
         a1 = reinterpret_cast<const uint4*>(P.X.pair) + (pi >> 4);
         sub = (uint32_t)pi & 15u;
         pair_read = true;
@@ -673,7 +742,9 @@ seed_persistent(PersistArgs P) {
         const int32_t j = jx - lane;
         uint32_t cc = 0, rec = 0, tb = 0;
         if (valid) {
-          cc = (lds[rs + (j >> 1)] >> ((j & 1) << 2)) & 15u;
+          const uint32_t rr = (uint32_t)(LMAX - 1 - j);
+          const uint32_t* rcw = reinterpret_cast<const uint32_t*>(lds + rs + A_BYTES);
+          cc = ((rcw[CW + (rr >> 5)] >> (rr & 31u)) & 1u) ? 4u : (rcw[rr >> 4] >> ((rr & 15u) << 1)) & 3u;
           rec = rec_get(rs, j);
           const uint32_t t = tp - (uint32_t)lane;
           tb = (pick4((t >> 4) & 3u, t0, t1, t2, t3) >> ((t & 15u) << 1)) & 3u;
@@ -690,7 +761,7 @@ seed_persistent(PersistArgs P) {
           // the first lane, at an even top position) writes the whole byte,
           // keeping the nibble of a position outside the matched range
           if ((uint32_t)lane < g && ((j & 1) || lane == 0)) {
-            uint8_t* a = lds + rs + CODE_BYTES + (j >> 1);
+            uint8_t* a = lds + rs + (j >> 1);
             if ((j & 1) && (uint32_t)lane + 1 < g)
               *a = 0x11;  // positions j and j - 1 both matched
             else
@@ -750,8 +821,8 @@ seed_persistent(PersistArgs P) {
       // window read[e-20, e-1) does; an absent window ends the task as an
       // absent presence bit would, a present one is looked up next
       known_e = e - 1;
-      known_p = (v4 >> (4u + ((uint32_t)(w20 >> 38) & 3u))) & 1u;
-      if (!((v4 >> ((uint32_t)w20 & 3u)) & 1u))
+      known_p = (v4 >> (4u + ((uint32_t)(wcode >> 38) & 3u))) & 1u;
+      if (!((v4 >> ((uint32_t)wcode & 3u)) & 1u))
         res = 1;
       else
         probe = false;
