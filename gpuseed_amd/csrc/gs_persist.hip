@@ -165,16 +165,19 @@ seed_persistent(PersistArgs P) {
   // the 22 codes left of the task end e: code(e-1-j) at bits 2j of wcode for
   // j = 0..21, and bit j of wbad set if that code is above 3 or its position
   // is left of the read start; the 13-mer pattern is the low 26 bits, an
-  // extended window and its 3 presence bases the bits above, and code(e-20)
-  // (the base left of the 19-base window) bits 38-39
+  // extended window and its 3 presence bases the bits above, code(e-20)
+  // (the base left of the 19-base window) bits 38-39, and code(e-21),
+  // code(e-22) (for a second pair byte) the top four
   uint64_t wcode = 0;
   uint32_t wbad = 0;
   // pair table: probe is set while the last clean 19-base window looked up
-  // was absent; known_e is the task end whose window the last pair byte
-  // also answered, known_p whether that window occurs
+  // was absent; the last probe also answered the windows of the task ends
+  // known_e, known_e - 1, ... (their number in bits 4-5 of known_m; bit i
+  // set iff the window of known_e - i occurs)
   const bool pair_on = P.X.pair && P.X.ext_len == 16 && k >= 19;
-  bool probe = false, known_p = false;
+  bool probe = false;
   int32_t known_e = -1;
+  uint32_t known_m = 0;
   uint32_t l = 0, u = 0, b = 0, cx = 0;
   int32_t e = 0, x = 0, e_min = 0, w0 = 0;
   uint32_t t0 = 0, first_b = 0, flags = 0, work = 0;
@@ -370,11 +373,11 @@ seed_persistent(PersistArgs P) {
         __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
         __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
       split();
-    // the next task end: past the one whose window the last pair byte found
-    // absent (its window is clean), then past 13-base windows holding a code
-    // above 3, jumping below the highest such code at once
+    // the next task end: past those whose windows the last probe found
+    // absent (their windows are clean), then past 13-base windows holding a
+    // code above 3, jumping below the highest such code at once
     int32_t ne = e - 1;
-    if (known_e == ne && !known_p) {
+    for (uint32_t ki = (uint32_t)(known_e - ne); ki < (known_m >> 4) && !((known_m >> ki) & 1u); ++ki) {
       --ne;
       probe = true;
     }
@@ -511,6 +514,7 @@ seed_persistent(PersistArgs P) {
     ones = 0;
     probe = false;
     known_e = -1;
+    known_m = 0;
     b = (uint32_t)e;
     tp = 0;
     need_next = (wbad & kBad13) != 0;  // no work done yet, so this never splits
@@ -648,8 +652,9 @@ seed_persistent(PersistArgs P) {
     // for every kind, the same single random access each.
     const uint4* a1 = nullptr;
     const uint4* a2 = nullptr;
-    uint32_t sub = 0;
+    uint32_t sub = 0, sub2 = 0;
     bool pair_read = false;  // op 1 reads the pair byte instead of a table entry
+    bool dual = false;       // and a second one
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
     uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)wcode & 0x3FFFFFFu;  // table lookup: pattern length and entry
@@ -665,16 +670,24 @@ seed_persistent(PersistArgs P) {
         // presence bit c1 + 4*c2 + 16*c3 of the next three bases, if clean
         if (k >= tlen + 3 && !(wbad & (7u << el))) pbit = (uint32_t)(wcode >> (2 * el)) & 63u;
       }
-      // after an absent window, a task whose 20 codes are clean reads the
-      // pair byte of its 18-base core read[e-19, e-1) instead
-      if (pair_on && probe && known_e != e && !(wbad & 0xFFFFFu)) {
+      // after an absent window, a task whose 20 codes are clean and whose
+      // window no probe answered reads the pair byte of its 18-base core
+      // read[e-19, e-1) instead, and with read[e-22, e-20) clean too that of
+      // read[e-21, e-3) in the same operation
+      if (pair_on && probe && (uint32_t)(known_e - e) >= (known_m >> 4) && !(wbad & 0xFFFFFu)) {
         const uint64_t pi = (wcode >> 2) & ((1ull << 36) - 1ull);
         a1 = reinterpret_cast<const uint4*>(P.X.pair) + (pi >> 4);
         sub = (uint32_t)pi & 15u;
         pair_read = true;
+        if (!(wbad & (3u << 20))) {
+          const uint64_t pj = (wcode >> 6) & ((1ull << 36) - 1ull);
+          a2 = reinterpret_cast<const uint4*>(P.X.pair) + (pj >> 4);
+          sub2 = (uint32_t)pj & 15u;
+          dual = true;
+        }
       }
-      ++n_table;
-      ++work;
+      n_table += dual ? 2 : 1;
+      work += dual ? 2 : 1;
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
       have1 = r1 != 0xFFFFFFFFu;
@@ -820,8 +833,16 @@ seed_persistent(PersistArgs P) {
       // read[e-19, e) occurs, bit 4 + code(e-20) whether the next task's
       // window read[e-20, e-1) does; an absent window ends the task as an
       // absent presence bit would, a present one is looked up next
+      // a second byte's bit code(e-3) and bit 4 + code(e-22) answer the
+      // windows of e-2 and e-3
       known_e = e - 1;
-      known_p = (v4 >> (4u + ((uint32_t)(wcode >> 38) & 3u))) & 1u;
+      uint32_t m = (v4 >> (4u + ((uint32_t)(wcode >> 38) & 3u))) & 1u;
+      if (dual) {
+        const uint32_t vb = (pick4(sub2 >> 2, d2.x, d2.y, d2.z, d2.w) >> ((sub2 & 3u) << 3)) & 255u;
+        m |= ((vb >> ((uint32_t)(wcode >> 4) & 3u)) & 1u) << 1;
+        m |= ((vb >> (4u + ((uint32_t)(wcode >> 42) & 3u))) & 1u) << 2;
+      }
+      known_m = m | ((dual ? 3u : 1u) << 4);
       if (!((v4 >> ((uint32_t)wcode & 3u)) & 1u))
         res = 1;
       else

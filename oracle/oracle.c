@@ -327,7 +327,7 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair, hist, tandem_bad, stretches;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair, pair_ops, hist, tandem_bad, stretches;
 } work_t;
 
 #define TANDEM_CAP_MAX 256u
@@ -398,10 +398,15 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
   static __thread uint64_t H[8 * TANDEM_CAP_MAX];
   /* pair-table model (or_opts.pair_probe), with the 16-base extended table:
    * probe is set while the last looked-up clean 19-base window was absent;
-   * known_e is the task end whose window the last pair byte also answered */
+   * the last probe also answered the windows of the known_n task ends
+   * known_e, known_e - 1, ...: bit i of known_m set iff that of known_e - i
+   * occurs */
   const int pair = pair_probe && miss_skip && ext_len == 16 && k >= 19;
   int probe = 0;
   int64_t known_e = -1;
+  uint32_t known_n = 0, known_m = 0;
+#define KNOWN(t) (known_e - (int64_t)(t) >= 0 && known_e - (int64_t)(t) < (int64_t)known_n)
+#define KNOWN_PRESENT(t) ((known_m >> (uint32_t)(known_e - (int64_t)(t))) & 1u)
   if (prune) memset(A, 0, sizeof(uint32_t) * L);
   uint32_t last_b = 0xFFFFFFFFu; /* left end of the last unpruned task (the device's in-lane SMEM filter) */
   for (uint32_t e = L; e >= k; --e) {
@@ -420,7 +425,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     int have = 0;
     /* a task whose window the last pair byte found absent is skipped before
      * anything else (the device's next_task); it leaves prev_reach alone */
-    const int known_absent = pair && known_e == (int64_t)e && !window19(X, v, e);
+    const int known_absent = pair && KNOWN(e) && !KNOWN_PRESENT(e);
     /* tandem model: this task walks for the stretch (walker >= 0), or jumps */
     int walker = -1, jump = 0;
     if (TP && !known_absent) {
@@ -488,11 +493,18 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         }
         if (pair && win3) { /* the 19-base window read[e-19, e) is clean */
           const int present = l <= u && !absent3;
-          if (known_e == (int64_t)e) { /* the last pair byte answered this window */
+          if (KNOWN(e)) { /* the last probe answered this window */
             if (present) ++w->ext;
-          } else if (probe && e >= 20 && v[e - 20] <= 3) { /* this task's pair byte */
-            ++w->pair;
+          } else if (probe && e >= 20 && v[e - 20] <= 3) { /* this task's pair byte(s) */
+            /* with 22 clean codes (pair_probe 2) a second byte, of the core
+             * read[e-21, e-3), answers the windows of e-2 and e-3 */
+            const int dual = pair_probe >= 2 && e >= 22 && v[e - 21] <= 3 && v[e - 22] <= 3;
+            w->pair += dual ? 2 : 1;
+            ++w->pair_ops;
             known_e = (int64_t)e - 1;
+            known_n = dual ? 3 : 1;
+            known_m = window19(X, v, e - 1) ? 1u : 0u;
+            if (dual) known_m |= (window19(X, v, e - 2) ? 2u : 0u) | (window19(X, v, e - 3) ? 4u : 0u);
             if (present) ++w->ext;
           } else {
             ++w->ext;
@@ -601,6 +613,8 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     if (e - b >= k) out[n++] = (cand_t){b, e, l, u}; /* 2259-2264 */
     if (smem_stop && reached) break; /* later tasks end at 0 too and fail the SMEM filter */
   }
+#undef KNOWN
+#undef KNOWN_PRESENT
   return n;
 }
 
@@ -957,6 +971,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_direct += work[t].direct;
     res->w_text += work[t].text;
     res->w_pair += work[t].pair;
+    res->w_pair_ops += work[t].pair_ops;
     res->w_hist += work[t].hist;
     res->w_tandem_bad += work[t].tandem_bad;
     res->w_stretches += work[t].stretches;

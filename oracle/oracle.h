@@ -110,8 +110,11 @@ typedef struct {
    * the next task's window read[e-20, e-1) does (a preceding base).  A task
    * whose window is absent ends without its extended-table lookup; the next
    * task, answered by the same byte, needs no lookup at all when its window
-   * is absent.  A present window is then looked up as usual (w_ext).  Output
-   * unchanged; counters only.  0: off. */
+   * is absent.  A present window is then looked up as usual (w_ext).
+   * pair_probe 2 (the device's): a task whose 22 bases read[e-22, e) are
+   * clean reads a second byte in the same operation, that of the core
+   * read[e-21, e-3), which answers the windows of task ends e-2 and e-3
+   * likewise.  Output unchanged; counters only.  0: off. */
   int pair_probe;
   /* tandem: work model of the device's tandem jumps (maximum period, at most
    * 8; 0: off).  Inside a stretch [R, E) of the read with period p (code(z)
@@ -169,6 +172,7 @@ typedef struct {
                              suffix-array and inverse reads and, of them,     */
   uint64_t w_text;        /* 16-byte text reads                               */
   uint64_t w_pair;        /* pair-table bytes read (opts.pair_probe)          */
+  uint64_t w_pair_ops;    /* operations that read them (one or two bytes)     */
   uint64_t w_hist;        /* recorded intervals read (opts.tandem)            */
   uint64_t w_tandem_bad;  /* jumped-to intervals that disagree (tandem_check) */
   uint64_t w_stretches;   /* tandem stretches opened                          */

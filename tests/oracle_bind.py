@@ -38,7 +38,7 @@ class OrResult(C.Structure):
                 ("w_blocks96", C.c_uint64), ("w_lf", C.c_uint64), ("w_sa", C.c_uint64),
                 ("w_steps_unpruned", C.c_uint64), ("n_short", C.c_uint64), ("w_ext", C.c_uint64),
                 ("w_direct", C.c_uint64), ("w_text", C.c_uint64), ("w_pair", C.c_uint64),
-                ("w_hist", C.c_uint64), ("w_tandem_bad", C.c_uint64), ("w_stretches", C.c_uint64)]
+                ("w_pair_ops", C.c_uint64), ("w_hist", C.c_uint64), ("w_tandem_bad", C.c_uint64), ("w_stretches", C.c_uint64)]
 
 
 _lib = None
@@ -174,7 +174,7 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem
             out.append(lst)
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair", "w_hist",
+                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_hist",
                                                  "w_tandem_bad", "w_stretches")}
         return out, counters
     finally:
@@ -207,7 +207,7 @@ def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, 
         po = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair", "w_hist",
+                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_hist",
                                                  "w_tandem_bad", "w_stretches")}
         return seeds, pos, so, po, counters
     finally:

@@ -1,4 +1,5 @@
 """Oracle seeding (SMEM and MEM) against the brute-force definitions."""
+import itertools
 import os
 import random
 
@@ -202,13 +203,14 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
             else:  # k below the window length: the option must not apply
                 assert cg == noskip[(stop, ext)], (smem, k, stop, ext)
     # the pair-table model: with the 16-mer table and k >= 19, a lookup of an
-    # 18-base core answers two adjacent windows; same seeds, fewer table
-    # reads; with k < 19 (or no 16-mer table) it must not apply
-    for smem, k in ((True, 19), (False, 20), (True, 16)):
+    # 18-base core answers two adjacent windows (pair_probe 2: two cores in
+    # one operation answer four); same seeds, fewer table reads; with k < 19
+    # (or no 16-mer table) it must not apply
+    for (smem, k), pp in itertools.product(((True, 19), (False, 20), (True, 16)), (1, 2)):
         off = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=16, miss_skip=1)
         for ext in (16, 14):
             on = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=ext, miss_skip=1,
-                                pair_probe=1)
+                                pair_probe=pp)
             for a, b in zip(off[:4], on[:4]):
                 assert (a == b).all(), (smem, k, ext)
             if ext != 16 or k < 19:
@@ -218,6 +220,8 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
                 continue
             co, cn = off[4], on[4]
             assert cn["w_pair"] > 0 and cn["w_ext"] + cn["w_pair"] < co["w_ext"], (smem, k)
+            # one operation per byte, or fewer operations than bytes with two per operation
+            assert (cn["w_pair_ops"] == cn["w_pair"]) == (pp == 1), (smem, k, pp)
             assert cn["w_ext"] < co["w_ext"]
             # everything after the table lookups is the same work
             assert all(cn[f] == co[f] for f in ("w_tasks", "w_steps", "w_blocks96", "w_direct")), (smem, k)
