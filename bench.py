@@ -64,8 +64,8 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, 
     oracle on `sample` reads spread over the batch: 13-mer table lookups (8 bytes),
     extended-table lookups (16 bytes: interval and presence bits), occ
     entries (16 bytes) and direct-extension reads (4-byte suffix-array and
-    inverse words, 16-byte text chunks), pair-table bytes and recorded
-    tandem intervals (8 bytes), with distance-1 pruning, the SMEM read-start
+    inverse words, 16-byte text chunks), pair-table bytes and tandem-table
+    entries (16 bytes), with distance-1 pruning, the SMEM read-start
     stop and tasks that cannot reach k bases ending at once; size records
     saturate at prune_sat like the kernel's (0: exact)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -80,13 +80,13 @@ def work_model(prefix, arrays, sample, threads, k, max_occ, ext_len, smem=True, 
     table = (c["w_tasks"] + c["w_ext"] + c["w_pair"]) / n
     occ = c["w_blocks96"] / n
     direct, text = c["w_direct"] / n, c["w_text"] / n
-    hist = c["w_hist"] / n
+    tandem_rd = c["w_tandem"] / n
     ext_bytes = 16.0 if ext_len > 13 else 8.0
-    return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct, hist_per_read=hist,
-                requests_per_read=table + occ + direct + hist,
+    return dict(reads=n, table_per_read=table, occ_per_read=occ, direct_per_read=direct, tandem_per_read=tandem_rd,
+                requests_per_read=table + occ + direct + tandem_rd,
                 pair_per_read=c["w_pair"] / n,
                 bytes_per_read=8.0 * c["w_tasks"] / n + ext_bytes * c["w_ext"] / n + 1.0 * c["w_pair"] / n + 16.0 * occ
-                + 4.0 * (direct - text) + 16.0 * text + 8.0 * hist)
+                + 4.0 * (direct - text) + 16.0 * text + 16.0 * tandem_rd)
 
 
 def cpu_baseline(prefix, arrays, gpu_out, sample, threads, k, max_occ, smem=True):
@@ -368,9 +368,9 @@ def timed_steps(batches, warmup, kernel):
                                         s.t_write * 1e3, s.t_locate * 1e3, s.n_seeds, s.n_pos))
     if kernel and stats and stats[0].n_reads:
         s = stats[0]
-        log("kernel %d requests per read: %.1f table, %.1f occ, %.1f direct, %.1f tandem history" %
+        log("kernel %d requests per read: %.1f table, %.1f occ, %.1f direct, %.1f tandem table" %
             (kernel, s.n_table_reads / s.n_reads, s.n_occ_reads / s.n_reads, s.n_direct_reads / s.n_reads,
-             s.n_hist_reads / s.n_reads))
+             s.n_tandem_reads / s.n_reads))
     return t1 - t0, stats
 
 
@@ -571,7 +571,7 @@ def main():
                                    "count": sum(x.t_count for x in r_stats) / len(r_stats) * 1e3,
                                    "write": sum(x.t_write for x in r_stats) / len(r_stats) * 1e3,
                                    "locate": sum(x.t_locate for x in r_stats) / len(r_stats) * 1e3},
-                     "requests_per_read_issued": sum(x.n_table_reads + x.n_occ_reads + x.n_direct_reads + x.n_hist_reads
+                     "requests_per_read_issued": sum(x.n_table_reads + x.n_occ_reads + x.n_direct_reads + x.n_tandem_reads
                                                      for x in r_stats) / (len(r_stats) * args.batch),
                      "same_output_as_headline": bool(same)}
         log("reference-shaped index (13-mer table, suffix-array walk): %.0f reads/s, seed %.1f ms, locate %.1f ms, "
@@ -596,7 +596,7 @@ def main():
         st_s = sb.run(args.kernel)
         sb.close()
         sample_counts = dict(reads=len(sa[2]), table=st_s.n_table_reads, occ=st_s.n_occ_reads,
-                             direct=st_s.n_direct_reads, hist=st_s.n_hist_reads)
+                             direct=st_s.n_direct_reads, tandem=st_s.n_tandem_reads)
     for _, bt in batches:
         bt.close()
     del batches
@@ -656,7 +656,7 @@ def main():
         # reads, counted by the oracle; W for the reference's procedure
         # (13-mer table, distance-1 pruning) is reported beside it
         req_per_read = wm["requests_per_read"]
-        issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads + st.n_hist_reads
+        issued = sum(st.n_table_reads + st.n_occ_reads + st.n_direct_reads + st.n_tandem_reads
                      for st in stats) / (len(stats) * args.batch)
         prof = profiled_counters(args.config, args.kernel, info.kmer_len, direct, pairs, 0 if args.no_tandem else 6)
         traffic, traffic_src = profiled_traffic(prof, args.batch)
@@ -673,7 +673,7 @@ def main():
               "measured_ceiling_requests_per_s": r_max}
         if sample_counts:
             sc = sample_counts
-            ra["requests_per_read_issued_on_model_sample"] = (sc["table"] + sc["occ"] + sc["direct"] + sc["hist"]) / sc["reads"]
+            ra["requests_per_read_issued_on_model_sample"] = (sc["table"] + sc["occ"] + sc["direct"] + sc["tandem"]) / sc["reads"]
             ra["model_sample_reads"] = sc["reads"]
         if prof and "tcc_hit" in prof:
             miss = prof["tcc_miss"] / (prof["tcc_hit"] + prof["tcc_miss"])

@@ -148,6 +148,7 @@ def lib():
         L.gs_index_read_occ.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_read_kmer.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_read_kext.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
+        L.gs_index_read_tandem.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, u32p]
         L.gs_index_set_kmer_len.argtypes = [C.c_void_p, C.c_int]
         L.gs_index_set_direct.argtypes = [C.c_void_p, C.c_int]
         L.gs_index_set_pairs.argtypes = [C.c_void_p, C.c_int]
@@ -347,6 +348,12 @@ class Index:
         _check("gs_index_read_kext", lib().gs_index_read_kext(self.h, first, count, _ptr(out, C.c_uint32)))
         return out
 
+    def read_tandem(self, first, count):
+        """Tandem-table entries: rows of (l, u, m, 0), 237 per unit key."""
+        out = np.empty((count, 4), dtype=np.uint32)
+        _check("gs_index_read_tandem", lib().gs_index_read_tandem(self.h, first, count, _ptr(out, C.c_uint32)))
+        return out
+
     def set_direct(self, on=True):
         """Adds or drops the direct-extension structures (full suffix array,
         its inverse, 2-bit text)."""
@@ -522,7 +529,7 @@ class BatchStats(C.Structure):
                 ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
                 ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
                 ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64), ("n_direct_reads", C.c_uint64),
-                ("n_hist_reads", C.c_uint64)]
+                ("n_tandem_reads", C.c_uint64)]
 
 
 def _bind_batch(L):

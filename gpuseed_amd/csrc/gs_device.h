@@ -59,6 +59,8 @@ struct DevIndex {
   // 4 + c iff base c followed by the core occurs; core index as the interval
   // tables' (sum of code_j * 4^j, the core's last base as j = 0)
   const uint8_t* pair;
+  // tandem table (kTandemKeys * kTandemRows 16-byte entries), see gs_internal.h
+  const uint4* tandem;
   uint32_t n_quirk, locate_direct;
   uint32_t primary, seq_len, sa_intv, ext_len;  // ext_len 0: no extended table
   uint64_t sa_div;       // ceil(2^64 / sa_intv) (0 when sa_intv == 1)

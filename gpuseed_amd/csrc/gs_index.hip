@@ -38,6 +38,7 @@ DevIndex gs_index::dev() const {
   d.text = reinterpret_cast<const uint4*>(text);
   d.quirk = quirk;
   d.pair = pair;
+  d.tandem = tandem;
   d.n_quirk = n_quirk;
   d.locate_direct = sa_full && n_quirk <= gs::kMaxQuirk ? 1u : 0u;
   d.primary = primary;
@@ -254,6 +255,36 @@ __global__ void pair_bits(const uint32_t* __restrict__ text, uint64_t n, uint64_
   if (bits) atomicOr(table + (idx >> 2), bits << (8 * (idx & 3)));
 }
 
+// One row of the tandem table per thread: key -> unit (p bases, its last
+// base the lowest code), then backward search of the unit repeated, one base
+// per length, from the interval of the empty string (rows 0..seq_len).  Once
+// a length is absent every longer one is too, so m stops growing.
+__global__ void tandem_rows(const DevIndex X, uint4* __restrict__ out) {
+  const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= gs::kTandemKeys) return;
+  uint32_t p = 1;
+  while (p < gs::kTandemMax && key >= gs::tandem_key_base(p + 1)) ++p;
+  const uint32_t unit = key - gs::tandem_key_base(p);
+  uint32_t l = 0, u = X.seq_len, m = 0;
+  bool alive = true;
+  uint32_t j = 0;  // code j of the unit comes next (base n - 1 - j of a string of length n)
+  for (uint32_t n = 1; n <= gs::kTandemLen; ++n) {
+    if (alive) {
+      uint32_t nl = l, nu = u;
+      gs::extend(X, (unit >> (2 * j)) & 3u, nl, nu);
+      if (nl > nu) {
+        alive = false;
+      } else {
+        l = nl;
+        u = nu;
+        m = n;
+      }
+    }
+    if (n >= gs::kTandemMin) out[(size_t)key * gs::kTandemRows + (n - gs::kTandemMin)] = make_uint4(l, u, m, 0);
+    j = j + 1 == p ? 0 : j + 1;
+  }
+}
+
 inline unsigned grid_for(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // One table level in launches of at most launch_chunk() work-items (level
@@ -305,7 +336,16 @@ int gs_build_kmer_table(gs_index* X) {
   hipError_t e = hipStreamSynchronize(X->stream);
   (void)hipFree(a);
   (void)hipFree(b);
-  return e == hipSuccess ? GS_OK : GS_EHIP;
+  if (e != hipSuccess) return GS_EHIP;
+  // the tandem table (about 21 MB), from the occ entries alone
+  const uint64_t tbytes = (uint64_t)gs::kTandemKeys * gs::kTandemRows * sizeof(uint4);
+  if (hipMalloc(&X->tandem, tbytes) != hipSuccess) {
+    X->tandem = nullptr;
+    return GS_ENOMEM;
+  }
+  X->device_bytes += tbytes;
+  hipLaunchKernelGGL(tandem_rows, dim3(grid_for(gs::kTandemKeys, 64)), dim3(64), 0, X->stream, X->dev(), X->tandem);
+  return hipStreamSynchronize(X->stream) == hipSuccess ? GS_OK : GS_EHIP;
 }
 
 // The extended table: the intervals of all 4^len patterns (len 14..16),
@@ -594,6 +634,7 @@ static void free_index(gs_index* X) {
   (void)hipFree(X->kmer);
   (void)hipFree(X->kext);
   (void)hipFree(X->pair);
+  (void)hipFree(X->tandem);
   free_direct(X);
   if (X->stream) (void)hipStreamDestroy(X->stream);
   delete X;
@@ -750,6 +791,15 @@ extern "C" GS_API int gs_index_read_occ(const gs_index* X, uint64_t first, uint6
   if (first > entries || count > entries - first) return GS_EINVAL;
   if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
   GS_HIP(hipMemcpy(out4, X->occ + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+extern "C" GS_API int gs_index_read_tandem(const gs_index* X, uint64_t first, uint64_t count, uint32_t* out4) {
+  if (!X || !out4 || !X->tandem) return GS_EINVAL;
+  const uint64_t entries = (uint64_t)gs::kTandemKeys * gs::kTandemRows;
+  if (first > entries || count > entries - first) return GS_EINVAL;
+  if (hipSetDevice(X->device) != hipSuccess) return GS_EHIP;
+  GS_HIP(hipMemcpy(out4, X->tandem + first, count * sizeof(uint4), hipMemcpyDeviceToHost));
   return GS_OK;
 }
 

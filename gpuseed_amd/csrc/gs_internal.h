@@ -39,6 +39,7 @@ struct gs_index {
   uint2* quirk = nullptr;
   uint32_t n_quirk = 0;
   uint8_t* pair = nullptr;  // pair table (gs_index_set_pairs), 4^18 bytes
+  uint4* tandem = nullptr;  // tandem table, built with the 13-mer table
   double t_pairs = 0;
   uint64_t device_bytes = 0;
   double t_read = 0, t_upload = 0, t_repack = 0, t_table = 0, t_sa = 0, t_write = 0, t_ext = 0, t_direct = 0;
@@ -121,7 +122,7 @@ struct PersistParams {
   uint32_t* seg_link;        // per item of a segmented read: first piece split off it, or ~0
   uint32_t* queue;           // work-queue counter
   uint64_t* counters;        // optional: [0] table reads, [1] occ entry reads, [2] direct-extension reads,
-                             // [3] tandem-history reads
+                             // [3] tandem-table reads
   uint32_t direct_min;       // size-1 steps before a task reads the text (0: no direct extension)
   // work splitting (dyn_cap 0: off)
   uint4* dyn_items;          // pieces {read, strand, first task, tasks}
@@ -134,8 +135,7 @@ struct PersistParams {
   uint32_t dyn_cap, split_work, split_tasks, epoch;
   uint32_t split_below;           // split only while fewer items than this run (set at launch)
   uint32_t refill_group;          // idle lanes of a wave that trigger a refill (set at launch)
-  uint32_t tandem;                // tandem jumps: largest period (0: off; needs hist)
-  uint2* hist;                    // tandem history: kTandemMax * kTandemCap intervals per lane of the grid
+  uint32_t tandem;                // tandem jumps: largest period (0: off; needs the index's tandem table)
 };
 
 // Joins pieces (segments of long reads, split-off pieces) into the per-read
@@ -168,16 +168,24 @@ constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that tri
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
 // Tandem jumps of the persistent kernel: periods 1..kTandemMax; a stretch
-// opens at a task whose previous task's left end lay kTandemGate or more
-// below its end and spans at least kTandemSpan positions; a walker records
-// the positions R + p .. R + kTandemCap - 1 of the stretch [R, E).  With the
-// 20 codes of the rolling window a period is at most 19; the configurations'
-// tandem repeats have periods 1-6.
+// [R, E) opens at a task end E whose 20 codes to the left have a period and
+// spans at least kTandemSpan positions; a task end e of the stretch with
+// e - R >= kTandemMin reads the tandem table's row of length
+// min(e - R, kTandemLen).  With the 20 codes of the rolling window a period
+// is at most 19; the configurations' tandem repeats have periods 1-6.
 constexpr uint32_t kTandemMax = 6;
-constexpr uint32_t kTandemCap = 128;
-constexpr uint32_t kTandemGate = 24;
-constexpr uint32_t kTandemSpan = 32;
-constexpr uint64_t kTandemLaneBytes = 8ull * kTandemMax * kTandemCap;
+constexpr uint32_t kTandemSpan = 24;
+constexpr uint32_t kTandemMin = 20;
+constexpr uint32_t kTandemLen = 256;
+// The tandem table: for every unit of 1..kTandemMax bases (key base(p) +
+// sum of code_j * 4^j, the unit's last base as j = 0, base(p) = (4^p - 4) / 3)
+// and every length n in [kTandemMin, kTandemLen], the 16-byte entry
+// {l, u, m, 0} of the string of length n that ends with the unit repeated
+// (the unit's last base at its end): m = the longest length <= n whose
+// string occurs, [l, u] the interval of that length.
+constexpr uint32_t kTandemKeys = (4096u * 4u - 4u) / 3u;  // 5460
+constexpr uint32_t kTandemRows = kTandemLen - kTandemMin + 1;
+__host__ __device__ constexpr uint32_t tandem_key_base(uint32_t p) { return ((1u << (2 * p)) - 4u) / 3u; }
 constexpr uint32_t kSegTasksMax = 112;  // task ends per segment: the window keeps every 13-base start
 
 // segments of a read of length L (1 for reads held whole in the LDS window)

@@ -90,7 +90,6 @@ struct PersistArgs {
   const uint4* codes;
   const gs::PersistCold* cold;
   uint32_t* queue;
-  uint2* hist;
   uint64_t n_items;
   uint32_t k, smem, direct_min, dyn_cap, split_work, split_tasks, split_below, refill_group, tandem;
 };
@@ -197,24 +196,20 @@ seed_persistent(PersistArgs P) {
   int rout = 0;
   uint4* out = nullptr;
   uint32_t gcodes = 0;  // the stream's first 16-byte code chunk (a batch has fewer than 2^32)
-  uint32_t n_table = 0, n_occ = 0, n_direct = 0, n_hist = 0;  // this lane's memory operations
-  // Tandem jumps.  In a stretch [tR, tE) of period tp (code(z) == code(z + tp)
-  // <= 3 for z in [tR, tE - tp)), read[y, e) equals read[y + d, e + d) for d
-  // a multiple of tp, y >= tR and e + d <= tE, so its interval is the one
-  // task end e + d had at position y + d.  The task ends tE .. tE - tp + 1
-  // (walker j = tE - e) extend as usual and record their intervals at the
-  // positions tR + tp .. tR + kTandemCap - 1 they compute them for, while
-  // those stay contiguous (a direct run or an N leaves a gap); any later
-  // task end e of the stretch, walker j = (tE - e) mod tp, d = tE - j - e,
-  // reads the interval at max(tR, lowest record of j - d) + d (phase 6) in
-  // place of its table lookup and of the steps above that position, and
-  // extends from there as after a lookup.  Skipped size records only prune.
-  // Byte j of t_lo / t_hi: offset from tR of walker j's lowest / highest
-  // record (t_hi < t_lo: none).  tp 0: no stretch.
+  uint32_t n_table = 0, n_occ = 0, n_direct = 0, n_tandem = 0;  // this lane's memory operations
+  // Tandem jumps.  In a stretch [tR, ..) of period tp (code(z) == code(z + tp)
+  // <= 3 for z >= tR up to the stretch's end), read[y, e) for tR <= y < e
+  // inside it is its last tp codes repeated, so its interval is the tandem
+  // table's for that unit and length e - y.  A task end e of the stretch
+  // with e - tR >= kTandemMin reads the entry of length n = min(e - tR,
+  // kTandemLen) (phase 6, the entry index in tq) in place of its table lookup
+  // and of the steps down to e - n, and extends from there as after a lookup;
+  // if the entry's longest occurring length m is below n, the backward
+  // extension ends at e - m (below 13: as a 13-mer table miss).  Skipped
+  // size records only prune.  tp 0: no stretch.
   uint32_t tp = 0;
-  int32_t tR = 0, tE = 0;
-  uint64_t t_lo = 0, t_hi = 0;
-  const bool tandem_on = P.tandem && P.hist;
+  int32_t tR = 0;
+  const bool tandem_on = P.tandem != 0;
 
   // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
   // an extension may run left of it and then reads device memory
@@ -298,45 +293,13 @@ seed_persistent(PersistArgs P) {
     work = 0;
     return true;
   };
-  // the lane's tandem history: kTandemMax walkers of kTandemCap intervals
-  auto hist_base = [&]() __attribute__((always_inline)) -> uint2* {
-    return P.hist + (size_t)(blockIdx.x * BS + threadIdx.x) * (gs::kTandemMax * gs::kTandemCap);
-  };
-  // a walker records the interval it just reached at b
-  auto t_record = [&]() __attribute__((always_inline)) {
-    const uint32_t j = (uint32_t)(tE - e);
-    if (j >= tp) return;  // not a walker (or no stretch: tp 0)
-    const int32_t off = (int32_t)b - tR;
-    if (off < (int32_t)tp || off >= (int32_t)gs::kTandemCap) return;
-    const uint32_t sh = 8u * j;
-    const uint32_t lo = (uint32_t)(t_lo >> sh) & 255u, hi = (uint32_t)(t_hi >> sh) & 255u;
-    if (hi >= lo && (uint32_t)off + 1u != lo) return;  // not next to the records so far
-    hist_base()[j * gs::kTandemCap + (uint32_t)off] = make_uint2(l, u);
-    if (hi < lo) t_hi = (t_hi & ~(255ull << sh)) | ((uint64_t)off << sh);
-    t_lo = (t_lo & ~(255ull << sh)) | ((uint64_t)off << sh);
-  };
-  // At a task's start: a jump (phase 6, the position in x, the record in
-  // tq), a walker, or none; with no stretch open, opens one when `gate`
-  // (the previous task's left end lay kTandemGate or more below its end)
-  // and the 20 codes left of e have a period.
-  auto t_start = [&](bool gate) __attribute__((always_inline)) {
-    if (tp) {
-      const uint32_t dd = (uint32_t)(tE - e);
-      if (dd < tp) return;  // a walker
-      const uint32_t j = dd % tp, d = dd - j;
-      const uint32_t lo = (uint32_t)(t_lo >> (8u * j)) & 255u, hi = (uint32_t)(t_hi >> (8u * j)) & 255u;
-      if (hi >= lo) {
-        const int32_t yo = (int32_t)lo - (int32_t)d > 0 ? (int32_t)lo - (int32_t)d : 0;
-        if (yo + (int32_t)d <= (int32_t)hi) {
-          x = tR + yo;
-          tq = j * gs::kTandemCap + (uint32_t)yo + d;
-          phase = 6;
-          return;
-        }
-      }
-      tp = 0;  // the stretch ends
-    }
-    if (gate && e >= 20 && !(wbad & 0xFFFFFu)) {
+  // At a task's start: a jump (phase 6, the table entry in tq) while the
+  // open stretch reaches kTandemMin codes left of e; with no stretch open,
+  // opens one when the 20 codes left of e have a period and the repeat runs
+  // on for kTandemSpan codes or more.
+  auto t_start = [&]() __attribute__((always_inline)) {
+    if (tp && e - tR < (int32_t)gs::kTandemMin) tp = 0;  // the stretch ends
+    if (!tp && e >= 20 && !(wbad & 0xFFFFFu)) {
       uint32_t q = 0;
       for (uint32_t p = 1; p <= P.tandem; ++p) {
         if (!((wcode ^ (wcode >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) {
@@ -354,16 +317,18 @@ seed_persistent(PersistArgs P) {
         if (e - (z + 1) >= (int32_t)gs::kTandemSpan) {
           tp = q;
           tR = z + 1;
-          tE = e;
-          t_lo = 0xFFFFFFFFFFFFull;  // every walker without records
-          t_hi = 0;
         }
       }
+    }
+    if (tp) {
+      const uint32_t n = min((uint32_t)(e - tR), gs::kTandemLen);
+      const uint32_t key = gs::tandem_key_base(tp) + ((uint32_t)wcode & ((1u << (2 * tp)) - 1u));
+      tq = key * gs::kTandemRows + (n - gs::kTandemMin);
+      phase = 6;
     }
   };
   // next task with a clean 13-base window, or end of the item
   auto next_task = [&]() __attribute__((always_inline)) {
-    const bool gate = tandem_on && e - (int32_t)b >= (int32_t)gs::kTandemGate;
     phase = 0;
     ext_miss = false;
     ones = 0;
@@ -393,7 +358,7 @@ seed_persistent(PersistArgs P) {
     }
     e = ne;
     b = (uint32_t)e;  // the task's left end until its lookup
-    if (tandem_on) t_start(gate);
+    if (tandem_on) t_start();
   };
   // size record at pos (positions left of the window keep none: pruning is
   // only an early exit, so a missing record never changes the output)
@@ -518,6 +483,7 @@ seed_persistent(PersistArgs P) {
     b = (uint32_t)e;
     tp = 0;
     need_next = (wbad & kBad13) != 0;  // no work done yet, so this never splits
+    if (tandem_on && !need_next) t_start();
   };
   auto start_main = [&](uint64_t it) __attribute__((always_inline)) {
     item = (uint32_t)it;
@@ -707,10 +673,9 @@ seed_persistent(PersistArgs P) {
     } else if (op == 5) {  // the inverse word of text position tq
       a1 = reinterpret_cast<const uint4*>(P.X.isa) + (tq >> 2);
       sub = tq & 3u;
-    } else if (op == 6) {  // the recorded interval tq of this lane (an even number of them per lane)
-      a1 = reinterpret_cast<const uint4*>(hist_base()) + (tq >> 1);
-      sub = tq & 1u;
-      ++n_hist;
+    } else if (op == 6) {  // tandem-table entry tq
+      a1 = P.X.tandem + tq;
+      ++n_tandem;
       ++work;
     }
     if (op >= 3 && op <= 5) {
@@ -817,10 +782,10 @@ seed_persistent(PersistArgs P) {
       nl = base + o1 + 1u;
       nu = base + o2;
       nb = (uint32_t)x;
-    } else if (op == 6) {  // the interval at x, recorded by the stretch's walker
-      nl = sub ? d1.z : d1.x;
-      nu = sub ? d1.w : d1.y;
-      nb = (uint32_t)x;
+    } else if (op == 6) {  // the entry's interval of length m = d1.z
+      nl = d1.x;
+      nu = d1.y;
+      nb = (uint32_t)e - d1.z;
     }
     // How the task goes on: 0 continues, 1 pruned or a table miss (no
     // candidate, 2145), 2 its interval emptied, 3 it reached the read start;
@@ -888,7 +853,9 @@ seed_persistent(PersistArgs P) {
           res = rout == 1 ? 2 : 3;
       }
     } else if (op != 0) {
-      if (nl > nu) {
+      // a tandem entry whose repeat occurs for fewer than 13 bases ends the
+      // task as a 13-mer table miss would
+      if (nl > nu || (op == 6 && d1.z < (uint32_t)gs::kPrecalc)) {
         // An absent clean tlen-base window: the task's candidate could only be
         // shorter than tlen.  With k >= tlen it can emit nothing, and what
         // else it would leave cannot change the output (left ends of finished
@@ -911,9 +878,10 @@ seed_persistent(PersistArgs P) {
         l = nl;
         u = nu;
         b = nb;
-        if (tandem_on) t_record();
         if (prune_at((int32_t)b, u - l + 1u)) {
           res = 1;
+        } else if (op == 6 && d1.z < min((uint32_t)(e - tR), gs::kTandemLen)) {
+          res = 2;  // the repeat one base longer does not occur: the extension ends here
         } else {
           x = (int32_t)b - 1;
           prep = true;
@@ -938,7 +906,7 @@ seed_persistent(PersistArgs P) {
   }
   if (C->counters) {
     // wave totals, one atomic per wave
-    uint64_t c0 = n_table, c1 = n_occ, c2 = n_direct, c3 = n_hist;
+    uint64_t c0 = n_table, c1 = n_occ, c2 = n_direct, c3 = n_tandem;
     for (int d = 32; d > 0; d >>= 1) {
       c0 += __shfl_down(c0, d, 64);
       c1 += __shfl_down(c1, d, 64);
@@ -1132,7 +1100,6 @@ int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
   A.codes = P.codes;
   A.cold = dst;
   A.queue = P.queue;
-  A.hist = P.hist;
   A.n_items = P.n_items;
   A.k = P.k;
   A.smem = P.smem;

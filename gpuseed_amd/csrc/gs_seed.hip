@@ -791,7 +791,7 @@ struct gs_batch {
   uint64_t n_reads = 0, span = 0, tasks = 0;
   uint32_t maxlen = 0;
   DevBuf bases, offs, lens, toff, batch, cand, nF, nR, ns, np, soff, poff, tiles, tot, seeds, rows, scratch;
-  DevBuf codes, code_off, queue, counters, hist;  // persistent kernel (hist: tandem history of its grid)
+  DevBuf codes, code_off, queue, counters;  // persistent kernel
   DevBuf items, seg_meta, long_reads, item0, item_tmp;  // item_tmp: counts and scans of the item plan
   DevBuf seg_link, dyn_items, dyn_meta, dyn_link, dyn_ready, dyn_ctl, wmeta, wlink;  // work splitting
   uint64_t code_chunks = 0, n_items = 0;
@@ -1119,17 +1119,6 @@ int gs::batch_load(gs_batch* B, const BatchPlan& P) {
   B->dyn_cap = split_on ? (uint32_t)std::min<uint64_t>(16u << 20, std::max<uint64_t>(65536, B->n_items)) : 0u;
   if (ensure_buffers(B, B->span, n, B->tasks, B->code_chunks, B->n_items, B->n_long, B->maxlen, B->dyn_cap))
     return GS_ENOMEM;
-  // the tandem history of the largest persistent grid (any batched kernel
-  // may run the batch); a plain-kernel batch has none
-  if (P.opts.kernel != 0 && P.opts.tandem != 0xFFFF && n) {
-    uint64_t need = 0;
-    for (int v = 0; v < 3; ++v) {
-      unsigned grid = 0;
-      if (gs::persistent_grid(v, P.opts.wg, &grid) == GS_OK)
-        need = std::max<uint64_t>(need, (uint64_t)grid * (P.opts.wg ? P.opts.wg : 128) * gs::kTandemLaneBytes);
-    }
-    if (need && B->hist.ensure(need)) return GS_ENOMEM;
-  }
   hipStream_t st = B->stream;
   if (hipEventRecord(B->ev[0], st) != hipSuccess) return GS_EHIP;
   if ((rc = staged_upload(B, B->bases.as<char>(), P.pieces, P.span))) return rc;
@@ -1279,11 +1268,9 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   PP.split_tasks = B->opts.split_tasks ? B->opts.split_tasks : 8u;
   PP.split_work = 64u * (B->opts.split_work ? B->opts.split_work : 32u);
   PP.epoch = 1;  // dyn_ready is cleared by launch_persistent before every launch
-  // tandem jumps where the history holds the launch's grid (batch_load sizes it)
-  const uint64_t hist_need = (uint64_t)pgrid * (B->opts.wg ? B->opts.wg : 128) * gs::kTandemLaneBytes;
-  const bool tandem = B->opts.tandem != 0xFFFF && B->hist.p && B->hist.cap >= hist_need;
+  // tandem jumps read the index's tandem table
+  const bool tandem = B->opts.tandem != 0xFFFF && B->X->tandem;
   PP.tandem = tandem ? (B->opts.tandem ? std::min<uint32_t>(B->opts.tandem, gs::kTandemMax) : gs::kTandemMax) : 0u;
-  PP.hist = tandem ? B->hist.as<uint2>() : nullptr;
   gs::JoinParams J;
   J.long_reads = B->long_reads.as<uint32_t>();
   J.item0 = B->item0.as<uint64_t>();
@@ -1423,7 +1410,7 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
         stats->n_table_reads = c[0];
         stats->n_occ_reads = c[1];
         stats->n_direct_reads = c[2];
-        stats->n_hist_reads = c[3];
+        stats->n_tandem_reads = c[3];
       }
     }
   }

@@ -236,6 +236,14 @@ GS_API int gs_index_read_direct(const gs_index* idx, uint32_t first, uint32_t co
  * c1 + 4*c2 + 16*c3 set iff the pattern extended to the left by c1, then c2,
  * then c3 occurs). */
 GS_API int gs_index_read_kext(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
+/* Entries [first, first+count) of the tandem table (tests): four words per
+ * entry, l, u, m, 0.  Entry key * 237 + (n - 20), n in [20, 256], of unit
+ * key (p bases for key in [(4^p - 4)/3, (4^(p+1) - 4)/3), p = 1..6, code
+ * key - (4^p - 4)/3 = sum of code_j * 4^j with the unit's last base as j = 0)
+ * describes the string of length n ending with the unit repeated: m is the
+ * longest length <= n whose string occurs in the text and [l, u] its
+ * interval.  Built with every index.  No counterpart in the reference. */
+GS_API int gs_index_read_tandem(const gs_index* idx, uint64_t first, uint64_t count, uint32_t* out4);
 
 /* Builds the index of codes[0..n) (values 0..3) on the device: suffix array
  * by prefix doubling, then the same device structures gs_index_open makes.
@@ -281,12 +289,14 @@ typedef struct {
                        0xFFFF: never                                         */
   uint16_t tandem;     /* batched kernels: tandem jumps for repeat periods up
                        to this, 1..6 (0 = 6); 0xFFFF: off.  Inside a stretch
-                       of the read with period p, the interval of read[y, e)
-                       is the one read[y + p, e + p) had, so after p task
-                       ends have extended through the stretch and recorded
-                       their intervals in device memory (about 1.6 GB per
-                       batch), each later task end of the stretch reads one
-                       recorded interval instead of its lookup and steps.
+                       read[R, E) of the read with period p, read[y, e) for
+                       y >= R is the last p bases of read[.., e) repeated,
+                       so its interval depends only on that unit and on
+                       e - y: a task end e of the stretch reads the index's
+                       tandem table (about 21 MB, built with the index) for
+                       length min(e - R, 256) instead of its lookup and the
+                       steps down to R, and goes on from there (or ends,
+                       where the table says the repeat stops occurring).
                        The output depends on none of the tuning fields
                        (kernel, seg_tasks, wg, walk_rows, split_*,
                        direct_min, tandem)                                   */
@@ -367,7 +377,7 @@ typedef struct {
   uint64_t n_table_reads, n_occ_reads; /* memory operations of the batched
                       extension kernel (0 for the plain kernel)                */
   uint64_t n_direct_reads; /* its suffix-array, text and inverse reads       */
-  uint64_t n_hist_reads;   /* its recorded tandem intervals read (gs_opts.tandem) */
+  uint64_t n_tandem_reads; /* its tandem-table entries read (gs_opts.tandem) */
 } gs_batch_stats;
 
 GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,

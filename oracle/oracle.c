@@ -327,10 +327,11 @@ typedef struct {
 } cand_t;
 
 typedef struct {
-  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair, pair_ops, hist, tandem_bad, stretches;
+  uint64_t tasks, steps, blocks64, blocks96, lf, sa, ext, direct, text, pair, pair_ops, tandem, tandem_bad, stretches;
 } work_t;
 
-#define TANDEM_CAP_MAX 256u
+#define TANDEM_MIN 20u  /* shortest stretch length a jump reads (the table's first length) */
+#define TANDEM_LEN 256u /* the table's longest length */
 
 /* the interval of the clean pattern v[b, e), e - b >= 13: its 13-mer table
  * entry, then one backward step per further base */
@@ -363,6 +364,48 @@ static uint32_t tandem_period(const uint8_t* v, uint32_t e, uint32_t pmax) {
   return 0;
 }
 
+/* Backward search of unit (p codes, code j = (unit >> 2j) & 3 at distance j
+ * from the end) repeated, one base per length from the empty string's
+ * interval [0, seq_len]: the longest m <= n whose string occurs, its
+ * interval in *l, *u. */
+static uint32_t tandem_walk(const or_index* X, uint32_t unit, uint32_t p, uint32_t n, uint32_t* l, uint32_t* u) {
+  uint32_t cl = 0, cu = X->seq_len, m = 0, j = 0;
+  for (uint32_t len = 1; len <= n; ++len) {
+    uint32_t nl = cl, nu = cu;
+    step(X, &nl, &nu, (unit >> (2 * j)) & 3u);
+    if (nl > nu) break;
+    cl = nl;
+    cu = nu;
+    m = len;
+    j = j + 1 == p ? 0 : j + 1;
+  }
+  *l = cl;
+  *u = cu;
+  return m;
+}
+
+int or_tandem_row(const or_index* X, uint32_t key, uint32_t* out4) {
+  if (!X || !out4 || key >= 5460u) return -1;
+  uint32_t p = 1;
+  while (p < 6 && key >= ((1u << (2 * (p + 1))) - 4u) / 3u) ++p;
+  const uint32_t unit = key - ((1u << (2 * p)) - 4u) / 3u;
+  uint32_t l, u;
+  const uint32_t m = tandem_walk(X, unit, p, TANDEM_LEN, &l, &u);
+  for (uint32_t n = TANDEM_MIN; n <= TANDEM_LEN; ++n) {
+    uint32_t* o = out4 + 4 * (n - TANDEM_MIN);
+    if (n < m) { /* the repeat of length n: the same walk, stopped at n */
+      tandem_walk(X, unit, p, n, &o[0], &o[1]);
+      o[2] = n;
+    } else {
+      o[0] = l;
+      o[1] = u;
+      o[2] = m;
+    }
+    o[3] = 0;
+  }
+  return 0;
+}
+
 static inline uint32_t row_to_idx(const or_index* X, uint32_t row) {
   return row - (row >= X->primary ? 1u : 0u);
 }
@@ -388,14 +431,11 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
                                   const or_opts* o, cand_t* out, uint32_t* A, work_t* w) {
   uint32_t n = 0;
   if (L < k) return 0;
-  /* tandem-jump model (or_opts.tandem): the open stretch [tR, tE) of period
-   * tp (0: none), and per walker j the recorded positions [hlo[j], hhi[j]]
-   * (empty while hhi < hlo) with their intervals in H */
+  /* tandem-jump model (or_opts.tandem): the open stretch [tR, ..) of period
+   * tp (0: none) */
   const uint32_t TP = (o->tandem && prune) ? (o->tandem > 8 ? 8 : o->tandem) : 0;
-  const uint32_t cap = o->tandem_cap > TANDEM_CAP_MAX ? TANDEM_CAP_MAX : o->tandem_cap;
-  uint32_t tp = 0, tR = 0, tE = 0, prev_reach = 0;
-  int32_t hlo[8], hhi[8];
-  static __thread uint64_t H[8 * TANDEM_CAP_MAX];
+  const uint32_t tlen = o->tandem_len < TANDEM_MIN ? TANDEM_MIN : o->tandem_len;
+  uint32_t tp = 0, tR = 0;
   /* pair-table model (or_opts.pair_probe), with the 16-base extended table:
    * probe is set while the last looked-up clean 19-base window was absent;
    * the last probe also answered the windows of the known_n task ends
@@ -424,37 +464,13 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     uint32_t l = 0, u = 0, b = 0;
     int have = 0;
     /* a task whose window the last pair byte found absent is skipped before
-     * anything else (the device's next_task); it leaves prev_reach alone */
+     * anything else (the device's next_task) */
     const int known_absent = pair && KNOWN(e) && !KNOWN_PRESENT(e);
-    /* tandem model: this task walks for the stretch (walker >= 0), or jumps */
-    int walker = -1, jump = 0;
+    /* tandem model: this task jumps, or not */
+    int jump = 0, tend = 0;
     if (TP && !known_absent) {
-      if (tp) {
-        if (tE - e < tp) {
-          walker = (int)(tE - e);
-        } else {
-          const uint32_t j = (tE - e) % tp, d = tE - j - e;
-          if (hhi[j] >= hlo[j]) {
-            const int32_t ys = hlo[j] - (int32_t)d > (int32_t)tR ? hlo[j] - (int32_t)d : (int32_t)tR;
-            if (ys + (int32_t)d <= hhi[j]) {
-              const uint64_t iv = H[j * cap + (uint32_t)ys + d - tR];
-              l = (uint32_t)iv;
-              u = (uint32_t)(iv >> 32);
-              b = (uint32_t)ys;
-              jump = have = 1;
-              ++w->hist;
-              if (o->tandem_check) {
-                uint32_t cl, cu;
-                interval_of(X, v, b, e, &cl, &cu);
-                if (cl != l || cu != u) ++w->tandem_bad;
-              }
-              if (pair) probe = 0;
-            }
-          }
-          if (!jump) tp = 0; /* the stretch closes */
-        }
-      }
-      if (!tp && prev_reach >= o->tandem_gate && e >= 20) {
+      if (tp && !(e >= tR + TANDEM_MIN)) tp = 0; /* the stretch closes */
+      if (!tp && e >= 20) {
         const uint32_t q = tandem_period(v, e, TP);
         if (q) {
           int32_t z = (int32_t)e - 21;
@@ -462,16 +478,38 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
           if (e - (uint32_t)(z + 1) >= o->tandem_span) {
             tp = q;
             tR = (uint32_t)(z + 1);
-            tE = e;
-            walker = 0;
-            for (uint32_t j = 0; j < 8; ++j) hlo[j] = 0, hhi[j] = -1;
             ++w->stretches;
+          }
+        }
+      }
+      if (tp) {
+        const uint32_t n = e - tR < tlen ? e - tR : tlen;
+        uint32_t unit = 0;
+        for (uint32_t j = 0; j < tp; ++j) unit |= (uint32_t)v[e - 1 - j] << (2 * j);
+        const uint32_t m = tandem_walk(X, unit, tp, n, &l, &u);
+        ++w->tandem;
+        if (m < PRECALC) continue; /* as a 13-mer table miss */
+        b = e - m;
+        jump = have = 1;
+        tend = m < n;
+        if (pair) probe = 0;
+        if (o->tandem_check) {
+          uint32_t cl, cu;
+          interval_of(X, v, b, e, &cl, &cu);
+          if (cl != l || cu != u) ++w->tandem_bad;
+          if (tend) { /* one more base of the read must empty the interval */
+            if (b == 0 || v[b - 1] > 3) {
+              ++w->tandem_bad;
+            } else {
+              step(X, &cl, &cu, v[b - 1]);
+              if (cl <= cu) ++w->tandem_bad;
+            }
           }
         }
       }
     }
     if (jump) {
-      /* the recorded interval stands for the lookup and the steps above b */
+      /* the table's interval stands for the lookup and the steps above b */
     } else if (ext_len > PRECALC && e >= ext_len) {
       int clean = 1;
       for (uint32_t j = PRECALC; j < ext_len; ++j) clean &= v[e - 1 - j] <= 3;
@@ -514,7 +552,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
           ++w->ext;
         }
         if (absent3) { /* absent: as below, with k >= ext_len + 3 */
-          if (!known_absent) prev_reach = 0;
           continue;
         }
         if (l <= u) {
@@ -525,7 +562,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
            * reaches ext_len bases, let alone k: it can emit nothing, and what
            * else it leaves (its left end for the in-lane SMEM filter, its
            * size records) cannot change any output (see or_opts.miss_skip) */
-          if (!known_absent) prev_reach = 0;
           continue;
         }
       }
@@ -535,24 +571,10 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       uint64_t iv = X->table ? X->table[p] : kmer13(X, p);
       l = (uint32_t)iv;
       u = (uint32_t)(iv >> 32);
-      if (l > u) { /* 2145 */
-        prev_reach = 0;
-        continue;
-      }
+      if (l > u) continue; /* 2145 */
       b = e - PRECALC;
     }
-    /* a walker records the interval at every position it computes one for,
-     * while the positions stay contiguous (not after a direct run) */
-#define TANDEM_RECORD()                                                                  \
-  do {                                                                                   \
-    if (walker >= 0 && b >= tR + tp && b - tR < cap && (hhi[walker] < hlo[walker] || (int32_t)b == hlo[walker] - 1)) { \
-      H[(uint32_t)walker * cap + b - tR] = (uint64_t)l | ((uint64_t)u << 32);            \
-      if (hhi[walker] < hlo[walker]) hhi[walker] = (int32_t)b;                           \
-      hlo[walker] = (int32_t)b;                                                          \
-    }                                                                                    \
-  } while (0)
-    if (!jump) TANDEM_RECORD();
-    int pruned = 0, reached = 1;
+    int pruned = 0, reached = !tend; /* a jump that ends the task: the next base empties it */
     if (prune) {
       uint32_t s = u - l + 1;
       if (sat && s >= sat) s = sat; /* a saturated record never prunes */
@@ -564,7 +586,7 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
     int run = 0;
     uint32_t tq = 0, ones = 0;
     uint64_t chunk = ~0ull;
-    for (int64_t x = (int64_t)b - 1; !pruned && x >= 0; --x) { /* 2154-2257 */
+    for (int64_t x = (int64_t)b - 1; !pruned && !tend && x >= 0; --x) { /* 2154-2257 */
       uint32_t c = v[x], l2 = l, u2 = u;
       if (direct_min && prune && !run && c <= 3 && l == u && ones >= direct_min && A[x] != 1) {
         run = 1; /* suffix-array read: the text position of the single row */
@@ -596,7 +618,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
       u = u2;
       b = (uint32_t)x; /* 2252-2255 */
       if (run) --tq;
-      if (!run && c <= 3) TANDEM_RECORD();
       if (prune) {
         uint32_t s = u - l + 1;
         if (sat && s >= sat) s = sat;
@@ -604,8 +625,6 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
         A[b] = s;
       }
     }
-#undef TANDEM_RECORD
-    prev_reach = e - b;
     if (pruned) continue;
     const int emit = e - b >= k && (!smem || b < last_b);
     if (run && emit) ++w->direct; /* the row where the run ended */
@@ -972,7 +991,7 @@ int or_seed_reads(const or_index* Xc, const char* bases, const uint64_t* offsets
     res->w_text += work[t].text;
     res->w_pair += work[t].pair;
     res->w_pair_ops += work[t].pair_ops;
-    res->w_hist += work[t].hist;
+    res->w_tandem += work[t].tandem;
     res->w_tandem_bad += work[t].tandem_bad;
     res->w_stretches += work[t].stretches;
     free(scratch[t]);

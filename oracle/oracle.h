@@ -117,27 +117,28 @@ typedef struct {
    * likewise.  Output unchanged; counters only.  0: off. */
   int pair_probe;
   /* tandem: work model of the device's tandem jumps (maximum period, at most
-   * 8; 0: off).  Inside a stretch [R, E) of the read with period p (code(z)
-   * == code(z + p) <= 3 for every z in [R, E - p)), the pattern read[y, e)
-   * equals read[y + d, e + d) for d a multiple of p and y >= R, e + d <= E,
-   * so its interval is the one task e + d had at position y + d.  A task
-   * that starts when the previous task's left end lay at least tandem_gate
-   * positions below its end, and whose 20 codes left of its end have a
-   * period p <= tandem (the smallest one), finds R by scanning left; if E - R
-   * >= tandem_span (E = its end) it opens a stretch: it and the next p - 1
-   * task ends walk as usual and record the interval at every position they
-   * compute it for in [R + p, R + tandem_cap) (not in direct runs).  Any
-   * later task e of the stretch, with walker j = (E - e) mod p and d = E - j
-   * - e, reads one recorded interval (w_hist) instead of its table lookup
-   * and the steps down to y = max(R, lowest recorded position of j - d), if
-   * y + d was recorded, and extends from y as after a lookup; else the
-   * stretch closes.  Output unchanged (intervals are exact; the skipped
-   * size records only prune); counters only.  Needs prune. */
+   * 8; 0: off).  Inside a stretch [R, ..) of the read with period p (code(z)
+   * == code(z + p) <= 3 for every z >= R up to the stretch's end), the
+   * pattern read[y, e) with R <= y < e is its last p codes (the unit)
+   * repeated, so its interval depends only on the unit and on e - y: the
+   * device reads it from a table of every unit of at most 6 bases and every
+   * length in [20, 256] (or_tandem_row).  A task with no stretch open whose
+   * 20 codes left of its end e have a period p <= tandem (the smallest one)
+   * finds R by scanning left and opens a stretch if e - R >= tandem_span.  A
+   * task of an open stretch with e - R >= 20 (else the stretch closes) takes
+   * n = min(e - R, tandem_len) and the longest m <= n whose repeat occurs
+   * (one table read, w_tandem): with m < 13 it ends as a 13-mer table miss
+   * would (no candidate, left end unchanged); else its interval is the
+   * repeat's of length m at y = e - m, in place of its table lookup and the
+   * steps down to y, and it goes on from y as after a lookup, or ends there
+   * (its interval emptying) when m < n.  Output unchanged (intervals are
+   * exact; the skipped size records only prune); counters only.  Needs
+   * prune. */
   uint32_t tandem;
-  uint32_t tandem_gate, tandem_span, tandem_cap;
+  uint32_t tandem_span, tandem_len, tandem_pad;
   /* tandem_check: recompute every jumped-to interval from the text of the
-   * read (table lookup and steps) and count disagreements in
-   * w_tandem_bad. */
+   * read (table lookup and steps), and check that one more base empties it
+   * where the jump ends the task; count disagreements in w_tandem_bad. */
   int tandem_check;
 } or_opts;
 
@@ -173,7 +174,7 @@ typedef struct {
   uint64_t w_text;        /* 16-byte text reads                               */
   uint64_t w_pair;        /* pair-table bytes read (opts.pair_probe)          */
   uint64_t w_pair_ops;    /* operations that read them (one or two bytes)     */
-  uint64_t w_hist;        /* recorded intervals read (opts.tandem)            */
+  uint64_t w_tandem;      /* tandem-table entries read (opts.tandem)          */
   uint64_t w_tandem_bad;  /* jumped-to intervals that disagree (tandem_check) */
   uint64_t w_stretches;   /* tandem stretches opened                          */
 } or_result;
@@ -189,6 +190,15 @@ void or_result_free(or_result* res);
 /* The -o text output of one run (seed_gen.cu:3625-3648), reads numbered from
  * first_read_no (1-based). */
 int or_format_text(const or_result* res, int smem, uint64_t first_read_no, FILE* out);
+
+/* Entries [20, 256] of the tandem table's row of unit `key` (p bases for
+ * key in [(4^p - 4)/3, (4^(p+1) - 4)/3), p = 1..6; code key - (4^p - 4)/3 =
+ * sum of code_j * 4^j, the unit's last base as j = 0): 237 entries of four
+ * words {l, u, m, 0}, for the string of length n = 20 + index that ends
+ * with the unit repeated, m the longest length <= n that occurs and [l, u]
+ * its interval (the empty string's [0, seq_len] for m = 0).  No counterpart
+ * in the reference. */
+int or_tandem_row(const or_index* X, uint32_t key, uint32_t* out4);
 
 #ifdef __cplusplus
 }

@@ -19,8 +19,8 @@ class OrOpts(C.Structure):
                 ("max_occ", C.c_uint32), ("threads", C.c_int), ("use_table", C.c_int),
                 ("smem_stop", C.c_int), ("ext_len", C.c_int), ("prune_sat", C.c_uint32),
                 ("direct_min", C.c_uint32), ("miss_skip", C.c_int), ("pair_probe", C.c_int),
-                ("tandem", C.c_uint32), ("tandem_gate", C.c_uint32), ("tandem_span", C.c_uint32),
-                ("tandem_cap", C.c_uint32), ("tandem_check", C.c_int)]
+                ("tandem", C.c_uint32), ("tandem_span", C.c_uint32), ("tandem_len", C.c_uint32),
+                ("tandem_pad", C.c_uint32), ("tandem_check", C.c_int)]
 
 
 class OrSeed(C.Structure):
@@ -38,17 +38,27 @@ class OrResult(C.Structure):
                 ("w_blocks96", C.c_uint64), ("w_lf", C.c_uint64), ("w_sa", C.c_uint64),
                 ("w_steps_unpruned", C.c_uint64), ("n_short", C.c_uint64), ("w_ext", C.c_uint64),
                 ("w_direct", C.c_uint64), ("w_text", C.c_uint64), ("w_pair", C.c_uint64),
-                ("w_pair_ops", C.c_uint64), ("w_hist", C.c_uint64), ("w_tandem_bad", C.c_uint64), ("w_stretches", C.c_uint64)]
+                ("w_pair_ops", C.c_uint64), ("w_tandem", C.c_uint64), ("w_tandem_bad", C.c_uint64), ("w_stretches", C.c_uint64)]
 
 
 _lib = None
 
-# the device's tandem-jump settings (gs_persist.hip): gate, span, capacity
-TANDEM_GATE, TANDEM_SPAN, TANDEM_CAP = 24, 32, 128
+# the device's tandem-jump settings (gs_internal.h): the shortest stretch
+# that opens and the tandem table's longest length
+TANDEM_SPAN, TANDEM_LEN = 24, 256
+TANDEM_KEYS, TANDEM_ROWS = 5460, 237
 
 
-def tandem_opts(tandem, check, gate=TANDEM_GATE, span=TANDEM_SPAN, cap=TANDEM_CAP):
-    return tandem, gate, span, cap, check
+def tandem_opts(tandem, check, span=TANDEM_SPAN, length=TANDEM_LEN):
+    return tandem, span, length, 0, check
+
+
+def tandem_row(index, key):
+    """The oracle's tandem-table row of unit `key`: uint32 [237, 4]."""
+    out = np.zeros((TANDEM_ROWS, 4), np.uint32)
+    if lib().or_tandem_row(index.h, key, out.ctypes.data_as(C.POINTER(C.c_uint32))) != 0:
+        raise RuntimeError("or_tandem_row failed")
+    return out
 
 
 def lib():
@@ -81,6 +91,7 @@ def lib():
                                     C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(OrOpts),
                                     C.POINTER(OrResult)]
         L.or_result_free.argtypes = [C.POINTER(OrResult)]
+        L.or_tandem_row.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
 
@@ -174,7 +185,7 @@ def seed(index, reads, k, smem, prune=0, max_occ=0, threads=4, use_table=0, smem
             out.append(lst)
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_hist",
+                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_tandem",
                                                  "w_tandem_bad", "w_stretches")}
         return out, counters
     finally:
@@ -207,7 +218,7 @@ def seed_arrays(index, buf, offs, lens, k, smem, prune=1, max_occ=0, threads=0, 
         po = np.ctypeslib.as_array(res.read_pos_off, (n + 1,)).copy()
         counters = {f: getattr(res, f) for f in ("w_tasks", "w_steps", "w_blocks64", "w_blocks96",
                                                  "w_lf", "w_sa", "w_steps_unpruned", "n_short", "w_ext",
-                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_hist",
+                                                 "w_direct", "w_text", "w_pair", "w_pair_ops", "w_tandem",
                                                  "w_tandem_bad", "w_stretches")}
         return seeds, pos, so, po, counters
     finally:

@@ -117,4 +117,4 @@ def test_kernel_work_equals_the_oracle_work_model(g, tmp_path, length, smem, k, 
         (length, smem, st.n_table_reads, st.n_occ_reads, c["w_tasks"] + c["w_ext"], c["w_blocks96"]))
     assert st.n_table_reads == c["w_tasks"] + c["w_ext"]
     assert st.n_occ_reads == c["w_blocks96"]
-    assert st.n_hist_reads == c["w_hist"]
+    assert st.n_tandem_reads == c["w_tandem"]

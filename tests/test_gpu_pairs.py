@@ -113,7 +113,7 @@ def test_seeds_unchanged_and_work_equals_the_model(g, tmp_path, smem, k):
             assert st.n_table_reads == c["w_tasks"] + c["w_ext"] + c["w_pair"]
             assert st.n_occ_reads == c["w_blocks96"]
             assert st.n_direct_reads == c["w_direct"]
-            assert st.n_hist_reads == c["w_hist"]
+            assert st.n_tandem_reads == c["w_tandem"]
     ox.close()
 
 

@@ -1,9 +1,10 @@
 """Tandem jumps of the batched kernels (gs_opts.tandem) on a text full of
-tandem arrays: the seeds equal the oracle's, and the kernels' memory
-operations (table lookups, occ entries, direct reads and recorded intervals
-read) equal the oracle's work model of the same shortcut (or_opts.tandem),
-for reads held whole in the LDS window with no work splitting.  Long reads
-(segments), work splitting and graph replays only need equal seeds."""
+tandem arrays: the device's tandem table equals the oracle's rows, the seeds
+equal the oracle's, and the kernels' memory operations (table lookups, occ
+entries, direct reads and tandem-table entries read) equal the oracle's
+work model of the same shortcut (or_opts.tandem), for reads held whole in
+the LDS window with no work splitting.  Long reads (segments), work
+splitting and graph replays only need equal seeds."""
 import random
 import sys
 
@@ -35,6 +36,20 @@ def case(g, tmp_path_factory):
     long_reads = reads_over(rng, text, arrays, 30, 400, 1500)
     d = tmp_path_factory.mktemp("tandem")
     return dict(prefix=build_index(d, text, 7), short=g.pack_reads(reads), mixed=g.pack_reads(reads + long_reads))
+
+
+def test_tandem_table_equals_the_oracle(g, case):
+    ox = ob.Index(case["prefix"])
+    with g.Index(case["prefix"]) as X:
+        got = X.read_tandem(0, ob.TANDEM_KEYS * ob.TANDEM_ROWS).reshape(ob.TANDEM_KEYS, ob.TANDEM_ROWS, 4)
+    long_rows = 0
+    for key in range(ob.TANDEM_KEYS):
+        want = ob.tandem_row(ox, key)
+        assert np.array_equal(got[key], want), key
+        long_rows += int(want[-1, 2] >= 100)
+    ox.close()
+    assert long_rows >= 10  # the text's arrays give rows that occur for 100 bases and more
+    say("tandem table: %d keys, %d with repeats of 100 bases or more" % (ob.TANDEM_KEYS, long_rows))
 
 
 def compare(out, want, what):
@@ -71,15 +86,15 @@ def test_tandem_seeds_and_work_equal_the_model(g, case, index, smem, k):
                                    miss_skip=1 if ext else 0, direct_min=3 if direct else 0,
                                    pair_probe=2 if direct else 0, tandem=tp, tandem_check=1)[4]
                 say("  kernel table %d occ %d direct %d hist %d; model %d %d %d %d (stretches %d)" %
-                    (st.n_table_reads, st.n_occ_reads, st.n_direct_reads, st.n_hist_reads,
-                     c["w_tasks"] + c["w_ext"] + c["w_pair"], c["w_blocks96"], c["w_direct"], c["w_hist"],
+                    (st.n_table_reads, st.n_occ_reads, st.n_direct_reads, st.n_tandem_reads,
+                     c["w_tasks"] + c["w_ext"] + c["w_pair"], c["w_blocks96"], c["w_direct"], c["w_tandem"],
                      c["w_stretches"]))
                 assert c["w_tandem_bad"] == 0
-                assert (c["w_hist"] > 0) == (tp > 0)
+                assert (c["w_tandem"] > 0) == (tp > 0)
                 assert st.n_table_reads == c["w_tasks"] + c["w_ext"] + c["w_pair"]
                 assert st.n_occ_reads == c["w_blocks96"]
                 assert st.n_direct_reads == c["w_direct"]
-                assert st.n_hist_reads == c["w_hist"]
+                assert st.n_tandem_reads == c["w_tandem"]
     ox.close()
 
 
@@ -98,7 +113,7 @@ def test_tandem_with_segments_splitting_and_replays(g, case):
         say("graph replays")
         b = g.Batch(X, arrays, k=19, smem=True, max_occ=200, kernel=3)
         st = b.run()
-        assert st.n_hist_reads > 0
+        assert st.n_tandem_reads > 0
         compare(b.fetch(), want, "run")
         b.capture()
         for i in range(3):

@@ -1,14 +1,15 @@
 """The tandem-jump work model (or_opts.tandem), the device kernel's shortcut
-through tandem repeats.
+through tandem repeats, and the tandem table it reads (or_tandem_row).
 
-Inside a stretch of the read with period p, the interval of read[y, e) is
-the one read[y + d, e + d) had, for d a multiple of p: the first p task ends
-of the stretch extend as usual and record their intervals, later ones read
-one record instead of a lookup and the steps above it.  The model must not
-change the seeds (checked against the brute-force SMEM and MEM definitions
-on a text full of tandem arrays), every jumped-to interval must equal the
-one recomputed from the read (tandem_check), and the shortcut must actually
-run and cut the work."""
+Inside a stretch of the read with period p, read[y, e) is its last p bases
+repeated, so its interval depends only on that unit and on e - y: a task
+reads it from the table instead of a lookup and the steps down to y.  The
+table rows must equal the brute-force intervals of the repeated units; the
+model must not change the seeds (checked against the brute-force SMEM and
+MEM definitions on a text full of tandem arrays), every jumped-to interval
+must equal the one recomputed from the read, and a jump that ends its task
+must end it where one more base of the read empties the interval
+(tandem_check); and the shortcut must actually run and cut the work."""
 import random
 
 import numpy as np
@@ -43,22 +44,62 @@ def small(built, tmp_path_factory):
     return dict(bi=brute.BruteIndex(text, 5), ox=ob.Index(prefix), reads=reads)
 
 
+def unit_of(key):
+    p = 1
+    while p < 6 and key >= (4 ** (p + 1) - 4) // 3:
+        p += 1
+    code = key - (4 ** p - 4) // 3
+    # code j is the base at distance j from the string's end
+    return "".join("ACGT"[(code >> (2 * j)) & 3] for j in reversed(range(p)))
+
+
+def repeated(unit, n):
+    """the string of length n that ends with unit repeated"""
+    s = unit * (n // len(unit) + 2)
+    return s[len(s) - n:]
+
+
+def test_tandem_row_definition(small):
+    """Every length of the rows of units that occur as arrays in the text,
+    of units that never occur and of short units, against brute force."""
+    bi = small["bi"]
+    text = bi.T
+    keys = set(range(0, 20)) | {5459, 1364, 340, 84}
+    for key in range(5460):
+        u = unit_of(key)
+        if len(u) >= 3 and u * 4 in text:
+            keys.add(key)
+    assert len(keys) > 40
+    for key in sorted(keys):
+        unit = unit_of(key)
+        row = ob.tandem_row(small["ox"], key)
+        m = 0
+        for n in range(1, ob.TANDEM_LEN + 1):
+            if bi.interval(repeated(unit, n)) is None:
+                break
+            m = n
+        for n in range(20, ob.TANDEM_LEN + 1):
+            want_m = min(m, n)
+            iv = bi.interval(repeated(unit, want_m)) if want_m else (0, bi.n)
+            assert tuple(row[n - 20]) == (iv[0], iv[1], want_m, 0), (key, unit, n)
+
+
 @pytest.mark.parametrize("smem,k", [(True, 19), (False, 19), (True, 16), (False, 20)])
-@pytest.mark.parametrize("gate,span", [(24, 32), (0, 20)])
-def test_tandem_model_matches_definition(small, smem, k, gate, span):
-    got, c = ob.seed(small["ox"], small["reads"], k, smem=smem, prune=1, tandem=6, tandem_check=1, gate=gate,
-                     span=span, **DEVICE)
+@pytest.mark.parametrize("span,length", [(24, 256), (20, 40)])
+def test_tandem_model_matches_definition(small, smem, k, span, length):
+    got, c = ob.seed(small["ox"], small["reads"], k, smem=smem, prune=1, tandem=6, tandem_check=1, span=span,
+                     length=length, **DEVICE)
     want = (brute.smem if smem else brute.mem)(small["bi"], small["reads"], k)
     for i, (g, w) in enumerate(zip(as_tuples(got), want)):
         assert g == w, "read %d %r" % (i, small["reads"][i])
     assert c["w_tandem_bad"] == 0
-    assert c["w_stretches"] > 0 and c["w_hist"] > 0, c
+    assert c["w_stretches"] > 0 and c["w_tandem"] > 0, c
 
 
 def test_tandem_needs_pruning(small):
     """Without size records (prune 0) the model is off: no stretch opens."""
     _, c = ob.seed(small["ox"], small["reads"], 19, smem=True, prune=0, tandem=6, **DEVICE)
-    assert c["w_stretches"] == 0 and c["w_hist"] == 0
+    assert c["w_stretches"] == 0 and c["w_tandem"] == 0
 
 
 @pytest.fixture(scope="module")
@@ -73,8 +114,8 @@ def large(built, tmp_path_factory):
 
 @pytest.mark.parametrize("opts", [dict(), dict(ext_len=0, miss_skip=0, pair_probe=0), dict(direct_min=0),
                                   dict(prune_sat=0)])
-@pytest.mark.parametrize("tk", [dict(tandem=6), dict(tandem=1), dict(tandem=6, cap=8), dict(tandem=6, gate=0, span=14),
-                                dict(tandem=8, cap=256)])
+@pytest.mark.parametrize("tk", [dict(tandem=6), dict(tandem=1), dict(tandem=6, length=24), dict(tandem=6, span=20),
+                                dict(tandem=8)])
 def test_tandem_cuts_work_and_keeps_seeds(large, opts, tk):
     o = dict(DEVICE, **opts)
     buf, offs, lens = large["arrays"]
@@ -86,8 +127,8 @@ def test_tandem_cuts_work_and_keeps_seeds(large, opts, tk):
         assert np.array_equal(a, b)
     co, cn = off[4], on[4]
     assert cn["w_tandem_bad"] == 0 and cn["w_stretches"] > 0
-    work = lambda c: c["w_tasks"] + c["w_ext"] + c["w_pair"] + c["w_blocks96"] + c["w_direct"] + c["w_hist"]
-    if tk.get("tandem") != 1 and tk.get("cap", 128) >= 64:
+    work = lambda c: c["w_tasks"] + c["w_ext"] + c["w_pair"] + c["w_blocks96"] + c["w_direct"] + c["w_tandem"]
+    if tk.get("tandem") != 1 and tk.get("length", 256) >= 64:
         assert work(cn) < 0.8 * work(co), (work(cn), work(co))
     else:
         assert work(cn) < work(co)
