@@ -300,13 +300,15 @@ seed_persistent(PersistArgs P) {
   auto t_start = [&]() __attribute__((always_inline)) {
     if (tp && e - tR < (int32_t)gs::kTandemMin) tp = 0;  // the stretch ends
     if (!tp && e >= 20 && !(wbad & 0xFFFFFu)) {
+      // the smallest period up to kTandemMax (every shift a constant, one
+      // select per period), kept if it is at most P.tandem (a smaller
+      // period would be the smallest)
+      const uint64_t w40 = wcode & ((1ull << 40) - 1ull);
       uint32_t q = 0;
-      for (uint32_t p = 1; p <= P.tandem; ++p) {
-        if (!((wcode ^ (wcode >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) {
-          q = p;
-          break;
-        }
-      }
+#pragma unroll
+      for (uint32_t p = gs::kTandemMax; p >= 1; --p)
+        if (!((w40 ^ (w40 >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) q = p;
+      if (q > P.tandem) q = 0;
       if (q) {
         int32_t z = e - 21;
         while (z >= 0) {
@@ -496,14 +498,15 @@ seed_persistent(PersistArgs P) {
       (strand ? C->nR : C->nF)[r] = 0;
       return;
     }
-    if (L <= (uint32_t)LMAX) {
-      kind = 0;
-      start_item(0, L - k + 1, kNone);
-    } else {
+    // one call site of start_item for both kinds (it is long)
+    uint32_t ts = 0, tc = L - k + 1;
+    kind = 0;
+    if (L > (uint32_t)LMAX) {
       kind = 1;
-      const uint32_t ts = seg * C->seg_tasks;
-      start_item(ts, min(C->seg_tasks, L - k + 1 - ts), kNone);
+      ts = seg * C->seg_tasks;
+      tc = min(C->seg_tasks, L - k + 1 - ts);
     }
+    start_item(ts, tc, kNone);
   };
   auto start_piece = [&](uint32_t id) __attribute__((always_inline)) {
     const uint4 d = C->dyn_items[id];
