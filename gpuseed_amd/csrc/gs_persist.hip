@@ -229,6 +229,20 @@ seed_persistent(PersistArgs P) {
   // 2j, bit j set for a code above 3 or a position left of the read start):
   // one run of bits of the slot, unless the window reaches left of a
   // segment's LDS window (then position by position, from device memory)
+  // the same 22 codes and bits from the slot alone, for ee - 22 >= w0 or
+  // w0 == 0 (positions left of the read start read the zero code words and
+  // the all-ones bit word after the window)
+  auto window_bits = [&](int32_t ee, uint64_t& w, uint32_t& bad) __attribute__((always_inline)) {
+    const uint32_t r1 = (uint32_t)(w0 + LMAX - ee);  // index of position ee - 1
+    const uint32_t* cw = reinterpret_cast<const uint32_t*>(lds + sbase + A_BYTES);
+    const uint32_t wi = r1 >> 4, o = (r1 & 15u) << 1;
+    const uint64_t lo = (uint64_t)cw[wi] | ((uint64_t)cw[wi + 1] << 32);
+    const uint64_t hi = cw[wi + 2];
+    w = (o ? (lo >> o) | (hi << (64 - o)) : lo) & ((1ull << 44) - 1ull);
+    const uint32_t bi = CW + (r1 >> 5), bo = r1 & 31u;
+    const uint64_t bb = (uint64_t)cw[bi] | ((uint64_t)cw[bi + 1] << 32);
+    bad = (uint32_t)(bb >> bo) & 0x3FFFFFu;
+  };
   auto load_window = [&](int32_t ee) __attribute__((always_inline)) {
     if (w0 > 0 && ee - 22 < w0) {
       uint64_t w = 0;
@@ -244,15 +258,7 @@ seed_persistent(PersistArgs P) {
       wbad = bad;
       return;
     }
-    const uint32_t r1 = (uint32_t)(w0 + LMAX - ee);  // index of position ee - 1
-    const uint32_t* cw = reinterpret_cast<const uint32_t*>(lds + sbase + A_BYTES);
-    const uint32_t wi = r1 >> 4, o = (r1 & 15u) << 1;
-    const uint64_t lo = (uint64_t)cw[wi] | ((uint64_t)cw[wi + 1] << 32);
-    const uint64_t hi = cw[wi + 2];
-    wcode = (o ? (lo >> o) | (hi << (64 - o)) : lo) & ((1ull << 44) - 1ull);
-    const uint32_t bi = CW + (r1 >> 5), bo = r1 & 31u;
-    const uint64_t bb = (uint64_t)cw[bi] | ((uint64_t)cw[bi + 1] << 32);
-    wbad = (uint32_t)(bb >> bo) & 0x3FFFFFu;
+    window_bits(ee, wcode, wbad);
   };
   auto end_stream = [&]() __attribute__((always_inline)) {
     const uint4 meta = make_uint4(nout, first_b, last_b, flags);
@@ -303,18 +309,49 @@ seed_persistent(PersistArgs P) {
       // the smallest period up to kTandemMax (every shift a constant, one
       // select per period), kept if it is at most P.tandem (a smaller
       // period would be the smallest)
+      // A period p also makes every multiple of p below 20 a period, and
+      // each p <= 6 divides 4, 5 or 6: codes without those three periods
+      // (nearly all) skip the rest.
+      static_assert(gs::kTandemMax == 6, "the prefilter covers periods 1..6");
       const uint64_t w40 = wcode & ((1ull << 40) - 1ull);
+      auto period = [&](uint32_t p) __attribute__((always_inline)) -> bool {
+        return !((w40 ^ (w40 >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull));
+      };
       uint32_t q = 0;
+      if (period(4) | period(5) | period(6)) {
 #pragma unroll
-      for (uint32_t p = gs::kTandemMax; p >= 1; --p)
-        if (!((w40 ^ (w40 >> (2 * p))) & ((1ull << (40 - 2 * p)) - 1ull))) q = p;
-      if (q > P.tandem) q = 0;
+        for (uint32_t p = gs::kTandemMax; p >= 1; --p)
+          if (period(p)) q = p;
+        if (q > P.tandem) q = 0;
+      }
       if (q) {
+        // R: z falls while code(z) == code(z + q) <= 3, 22 positions per
+        // step from the slot (the first position that differs, holds a code
+        // above 3 or lies left of the read start stops it), one at a time
+        // near a segment window's left edge
         int32_t z = e - 21;
         while (z >= 0) {
-          const uint32_t c = code(z);
-          if (c > 3 || c != code(z + (int32_t)q)) break;
-          --z;
+          if (w0 > 0 && z - 21 < w0) {
+            const uint32_t c = code(z);
+            if (c > 3 || c != code(z + (int32_t)q)) break;
+            --z;
+            continue;
+          }
+          uint64_t wa, wb;
+          uint32_t ba, bq;
+          window_bits(z + 1, wa, ba);                // code(z - j) at bits 2j
+          window_bits(z + 1 + (int32_t)q, wb, bq);   // code(z + q - j)
+          const uint64_t d = wa ^ wb;
+          const uint64_t g = (d | (d >> 1)) & 0x5555555555555555ull;  // bit 2j: position z - j differs
+          const uint32_t jd = g ? (uint32_t)__builtin_ctzll(g) >> 1 : 32u;
+          const uint32_t bm = ba | bq;
+          const uint32_t jb = bm ? (uint32_t)__builtin_ctz(bm) : 32u;
+          const uint32_t j0 = jd < jb ? jd : jb;
+          if (j0 < 22u) {
+            z -= (int32_t)j0;
+            break;
+          }
+          z -= 22;
         }
         if (e - (z + 1) >= (int32_t)gs::kTandemSpan) {
           tp = q;
