@@ -160,7 +160,21 @@ struct JoinParams {
 };
 
 constexpr uint32_t kSplitStream = 0xFFFFFFFFu;  // nF/nR of a whole-read stream left for join_whole
-constexpr uint32_t kSplitItemsPerLane = 4;      // work splitting only in batches this small
+// Work splitting by default only in batches of at most this many items per
+// lane of the persistent grid: on the 3.1 Gbp index, 80,000 150 bp reads
+// (0.6 items per lane) 7.5 ms with splitting, 11 ms without; 160,000 (1.2 per
+// lane) 18.8 and 10.7 ms; 10,000 mixed-length reads (1.6 per lane) 7.5 and
+// 6.2 ms.
+constexpr uint32_t kSplitItemsPerLane = 1;
+// Split thresholds for reads held whole in the LDS window (memory operations,
+// task ends); the options' values apply when smaller.  A launch of 10,000
+// 150 bp reads on the 3.1 Gbp index: 8.8 ms with the defaults of long reads
+// (2048, 8), 3.7 ms with (128, 4), 3.6 ms with (64, 2); 40,000 reads: 8.8,
+// 4.9 and 9.7 ms.  Long reads' segments keep the options' values: with
+// (128, 4) for every item a mixed-length launch of 10,000 reads took 12.9
+// against 9.3 ms.
+constexpr uint32_t kSplitWorkWhole = 128;
+constexpr uint32_t kSplitTasksWhole = 4;
 constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that trigger a refill
                                                 // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
                                                 // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms;

@@ -24,7 +24,9 @@
 //    never stops a task and each task walks far).  Once the main queue is
 //    exhausted and fewer items run than half the lanes, an item that has used
 //    split_work memory operations since it started or last split, and still
-//    has at least 2 * split_tasks task ends to do, hands half of them to an
+//    has at least 2 * split_tasks task ends to do (for reads held whole in
+//    the LDS window at most kSplitWorkWhole and kSplitTasksWhole), hands half
+//    of them to an
 //    overflow queue, which idle lanes take (a piece starts without size
 //    records, so splitting while every lane is busy would only add work);
 //    the join kernels put the pieces back in task order and settle the
@@ -183,6 +185,8 @@ seed_persistent(PersistArgs P) {
   uint32_t pending = kNone;                                // claimed piece not visible yet
   int32_t drun = 0;                                         // items started minus ended this iteration
   uint32_t poll_wait = 0;                                   // wave-uniform: iterations to the next poll
+  uint32_t fold_wait = 0;                                   // wave-uniform: iterations since the count was folded
+  bool in_tail = false;                                     // wave-uniform: items split (work splitting on)
   uint32_t backoff = 1;                                     // wave-uniform: sleeps while all lanes wait
   uint32_t link = kNone;                                   // first piece split off this item
   uint32_t item = 0;  // index of the item (or piece) in its list
@@ -371,12 +375,14 @@ seed_persistent(PersistArgs P) {
     phase = 0;
     ext_miss = false;
     ones = 0;
-    // split only in the tail: the main queue is exhausted and fewer items run
-    // than half the lanes, so pieces find idle lanes instead of adding work
-    if (P.dyn_cap && work > P.split_work && e - e_min >= 2 * (int32_t)P.split_tasks &&
-        __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
-        __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below)
-      split();
+    // split only in the tail (in_tail: the main queue is exhausted and fewer
+    // items run than half the lanes), so pieces find idle lanes instead of
+    // adding work; reads held whole in the window (and pieces of them) split
+    // sooner: their pieces reload only the read's few code chunks
+    const bool whole = L <= (uint32_t)LMAX;
+    const uint32_t sw = whole ? min(P.split_work, gs::kSplitWorkWhole) : P.split_work;
+    const uint32_t stk = whole ? min(P.split_tasks, gs::kSplitTasksWhole) : P.split_tasks;
+    if (in_tail && work > sw && e - e_min >= 2 * (int32_t)stk) split();
     // the next task end: past those whose windows the last probe found
     // absent (their windows are clean), then past 13-base windows holding a
     // code above 3, jumping below the highest such code at once
@@ -555,9 +561,23 @@ seed_persistent(PersistArgs P) {
     L = C->lens[r];
     start_item(d.z, d.w, lk);  // pieces split off this one go in front of its successor
   };
-  // items started minus items ended in this iteration, one atomic per wave;
-  // a split pushes its pieces (and fences) before its item ends
+  // items started minus items ended since the wave's last fold, one atomic
+  // per wave every 32 iterations, and at once when the wave has no item
+  // (its count is then final); a split pushes its pieces (and fences) before
+  // its item ends.  A wave's folded total never falls below zero (its items'
+  // starts are folded no later than their ends), and a count that lags only
+  // delays the split trigger or keeps waiting lanes polling a little longer.
   auto fold_running = [&]() __attribute__((always_inline)) {
+    if ((++fold_wait & 31u) && __ballot(have) != 0) return;
+    // the split trigger's state, read by one lane for the wave: the main
+    // queue is exhausted and fewer items run than split_below
+    uint32_t t = 0;
+    if (lane == 0)
+      t = __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
+                  __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below
+              ? 1u
+              : 0u;
+    in_tail = __shfl(t, 0) != 0;
     if (__ballot(drun != 0) == 0) return;
     int d = drun;
     drun = 0;
