@@ -76,6 +76,17 @@ __device__ __forceinline__ uint64_t rev2(uint64_t x) {
 }
 constexpr uint32_t kBad13 = (1u << gs::kPrecalc) - 1u;  // the 13-mer window's codes in a 19-code window
 
+// A 16-byte chunk of 32 4-bit codes as the persistent kernel's LDS slot
+// holds it: {codes at 2 bits in reverse order (low, high word), a bit per
+// code above 3 in reverse order, 0}.
+__device__ __forceinline__ uint4 slot_chunk(uint4 v) {
+  const uint64_t f = (uint64_t)nib_codes(v.x) | ((uint64_t)nib_codes(v.y) << 16) |
+                     ((uint64_t)nib_codes(v.z) << 32) | ((uint64_t)nib_codes(v.w) << 48);
+  const uint32_t fb = nib_bad(v.x) | (nib_bad(v.y) << 8) | (nib_bad(v.z) << 16) | (nib_bad(v.w) << 24);
+  const uint64_t rv = rev2(f);
+  return make_uint4((uint32_t)rv, (uint32_t)(rv >> 32), __builtin_bitreverse32(fb), 0u);
+}
+
 __device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -108,6 +119,7 @@ template <typename AT, int LMAX, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kMinWaves<AT>)))
 seed_persistent(PersistArgs P) {
   static_assert(LMAX % 32 == 0, "LMAX must be a multiple of 32");
+  static_assert(LMAX == (int)gs::kPersistLmax, "pack_codes writes reads of up to kPersistLmax in the slot's form");
   // size records of RB bits per position: 4 (Nib, two per byte, low nibble
   // first), 8 or 16; a record saturates at 2^RB - 1 and then never prunes
   constexpr int RB = std::is_same<AT, Nib>::value ? 4 : 8 * (int)sizeof(AT);
@@ -216,7 +228,9 @@ seed_persistent(PersistArgs P) {
   const bool tandem_on = P.tandem != 0;
 
   // code of position i of the stream: the LDS window holds [w0, w0 + LMAX);
-  // an extension may run left of it and then reads device memory
+  // an extension may run left of it and then reads the 4-bit codes in
+  // device memory (only in segments of long reads: a read held whole has
+  // w0 = 0, and its chunks are stored in the slot's form)
   auto code = [&](int32_t i) __attribute__((always_inline)) -> uint32_t {
     const int32_t j = i - w0;
     if (j >= 0) {
@@ -491,16 +505,14 @@ seed_persistent(PersistArgs P) {
     // q = LMAX/32 - 1 - (c - c0)
     const uint32_t c0 = (uint32_t)w0 >> 5, c1 = top >> 5;
     uint32_t* cw = reinterpret_cast<uint32_t*>(lds + sbase + A_BYTES);
+    const bool whole = L <= (uint32_t)LMAX;  // pack_codes wrote its chunks in the slot's form
     for (uint32_t c = c0; c < c1; ++c) {
-      const uint4 v = src[c];
-      const uint64_t f = (uint64_t)nib_codes(v.x) | ((uint64_t)nib_codes(v.y) << 16) |
-                         ((uint64_t)nib_codes(v.z) << 32) | ((uint64_t)nib_codes(v.w) << 48);
-      const uint32_t fb = nib_bad(v.x) | (nib_bad(v.y) << 8) | (nib_bad(v.z) << 16) | (nib_bad(v.w) << 24);
+      uint4 v = src[c];
+      if (!whole) v = slot_chunk(v);
       const uint32_t q = (uint32_t)(LMAX / 32 - 1) - (c - c0);
-      const uint64_t rv = rev2(f);
-      cw[2 * q] = (uint32_t)rv;
-      cw[2 * q + 1] = (uint32_t)(rv >> 32);
-      cw[CW + q] = __builtin_bitreverse32(fb);
+      cw[2 * q] = v.x;
+      cw[2 * q + 1] = v.y;
+      cw[CW + q] = v.z;
     }
     cw[CW - 2] = 0;
     cw[CW - 1] = 0;
@@ -1075,7 +1087,9 @@ __device__ __forceinline__ uint4 pack_chunk(const uint8_t* s, uint32_t L, uint32
   return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-// reads of length <= kPersistLmax: one thread per (read, strand, chunk)
+// reads of length <= kPersistLmax: one thread per (read, strand, chunk),
+// each chunk stored in the form the persistent kernel's LDS slot holds
+// (slot_chunk), which a refill then copies without converting
 __global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ bases, const uint64_t* __restrict__ offs,
                                                   const uint32_t* __restrict__ lens,
                                                   const uint64_t* __restrict__ code_off, uint32_t n_reads,
@@ -1090,7 +1104,7 @@ __global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ ba
   if (L > gs::kPersistLmax) return;
   const uint32_t nch = (L + 31) >> 5;
   if (c >= nch) return;
-  codes[code_off[r] + strand * nch + c] = pack_chunk(bases + offs[r], L, strand, c);
+  codes[code_off[r] + strand * nch + c] = slot_chunk(pack_chunk(bases + offs[r], L, strand, c));
 }
 
 // longer reads: one workgroup per read
