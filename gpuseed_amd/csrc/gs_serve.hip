@@ -286,57 +286,83 @@ void device_main(gs_server* S, gs_batch* B) {
     if (!rc) rc = gs::batch_download(B, &res);
     const double t3 = S->now();
     if (!rc) {
-      // each chunk's results into its piece; large rounds copy on several
-      // threads (the pieces are disjoint)
-      std::atomic<int> alloc_failed{0};
-      auto hand_off = [&](size_t first, size_t step) {
-        for (size_t i = first; i < R->chunks.size(); i += step) {
-          const Chunk& c = R->chunks[i];
-          if (skip[i]) continue;
+      // each chunk's results into its piece: the pieces' arrays are allocated
+      // here, then filled by up to 8 threads that take fixed-size parts of
+      // them (per-read arrays, seed records, positions) from a shared list,
+      // so a round of a few large chunks copies, and takes its page faults,
+      // as fast as one of many small ones
+      bool alloc_failed = false;
+      for (size_t i = 0; i < R->chunks.size(); ++i) {
+        const Chunk& c = R->chunks[i];
+        if (skip[i]) continue;
+        gs_result& o = c.piece->r;
+        gs_result_free(&o);
+        const uint64_t a = c.at, m = c.r1 - c.r0, z = a + m;
+        o.n_reads = m;
+        o.n_seeds = res.soff[z] - res.soff[a];
+        o.n_pos = res.poff[z] - res.poff[a];
+        o.read_nseeds = (uint32_t*)gs::result_alloc(4 * (m + 1));
+        o.read_npos = (uint32_t*)gs::result_alloc(4 * (m + 1));
+        o.read_seed_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
+        o.read_pos_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
+        o.seeds = (gs_seed_rec*)gs::result_alloc(sizeof(gs_seed_rec) * (o.n_seeds + 1));
+        o.pos = (uint32_t*)gs::result_alloc(4 * (o.n_pos + 1));
+        if (!o.read_nseeds || !o.read_npos || !o.read_seed_off || !o.read_pos_off || !o.seeds || !o.pos) {
+          gs_result_free(&o);
+          skip[i] = 1;
+          alloc_failed = true;
+        }
+      }
+      // parts: kind 0 reads [a, z) of the per-read arrays (the offsets
+      // rebased to the chunk, the last one included with the chunk's last
+      // part), kind 1 seed records, kind 2 positions
+      struct Part {
+        uint32_t chunk, kind;
+        uint64_t a, z;
+      };
+      constexpr uint64_t kReadsPer = 1u << 17, kSeedsPer = 1u << 17, kPosPer = 1u << 20;
+      std::vector<Part> parts;
+      uint64_t bytes = 0;
+      for (size_t i = 0; i < R->chunks.size(); ++i) {
+        if (skip[i]) continue;
+        const gs_result& o = R->chunks[i].piece->r;
+        for (uint64_t a = 0; a < o.n_reads || a == 0; a += kReadsPer)
+          parts.push_back({(uint32_t)i, 0u, a, std::min<uint64_t>(o.n_reads, a + kReadsPer)});
+        for (uint64_t a = 0; a < o.n_seeds; a += kSeedsPer)
+          parts.push_back({(uint32_t)i, 1u, a, std::min<uint64_t>(o.n_seeds, a + kSeedsPer)});
+        for (uint64_t a = 0; a < o.n_pos; a += kPosPer)
+          parts.push_back({(uint32_t)i, 2u, a, std::min<uint64_t>(o.n_pos, a + kPosPer)});
+        bytes += 24 * o.n_reads + sizeof(gs_seed_rec) * o.n_seeds + 4 * o.n_pos;
+      }
+      std::atomic<size_t> next{0};
+      auto copier = [&]() {
+        for (size_t k = next++; k < parts.size(); k = next++) {
+          const Part& pt = parts[k];
+          const Chunk& c = R->chunks[pt.chunk];
           gs_result& o = c.piece->r;
-          gs_result_free(&o);  // a chunk copied twice (see below) keeps the last copy
-          const uint64_t a = c.at, m = c.r1 - c.r0, z = a + m;
-          const uint64_t s0 = res.soff[a], p0 = res.poff[a];
-          o.n_reads = m;
-          o.n_seeds = res.soff[z] - s0;
-          o.n_pos = res.poff[z] - p0;
-          o.read_nseeds = (uint32_t*)gs::result_alloc(4 * (m + 1));
-          o.read_npos = (uint32_t*)gs::result_alloc(4 * (m + 1));
-          o.read_seed_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
-          o.read_pos_off = (uint64_t*)gs::result_alloc(8 * (m + 1));
-          o.seeds = (gs_seed_rec*)gs::result_alloc(sizeof(gs_seed_rec) * (o.n_seeds + 1));
-          o.pos = (uint32_t*)gs::result_alloc(4 * (o.n_pos + 1));
-          if (!o.read_nseeds || !o.read_npos || !o.read_seed_off || !o.read_pos_off || !o.seeds || !o.pos) {
-            gs_result_free(&o);
-            alloc_failed = 1;
-            continue;
+          const uint64_t at = c.at, s0 = res.soff[at], p0 = res.poff[at];
+          if (pt.kind == 0) {
+            memcpy(o.read_nseeds + pt.a, res.nseeds + at + pt.a, 4 * (pt.z - pt.a));
+            memcpy(o.read_npos + pt.a, res.npos + at + pt.a, 4 * (pt.z - pt.a));
+            const uint64_t last = pt.z == o.n_reads ? pt.z : pt.z - 1;  // offsets have n_reads + 1 entries
+            for (uint64_t j = pt.a; j <= last; ++j) {
+              o.read_seed_off[j] = res.soff[at + j] - s0;
+              o.read_pos_off[j] = res.poff[at + j] - p0;
+            }
+          } else if (pt.kind == 1) {
+            memcpy(o.seeds + pt.a, res.seeds + s0 + pt.a, sizeof(gs_seed_rec) * (pt.z - pt.a));
+          } else {
+            memcpy(o.pos + pt.a, res.pos + p0 + pt.a, 4 * (pt.z - pt.a));
           }
-          memcpy(o.read_nseeds, res.nseeds + a, 4 * m);
-          memcpy(o.read_npos, res.npos + a, 4 * m);
-          for (uint64_t j = 0; j <= m; ++j) {
-            o.read_seed_off[j] = res.soff[a + j] - s0;
-            o.read_pos_off[j] = res.poff[a + j] - p0;
-          }
-          memcpy(o.seeds, res.seeds + s0, sizeof(gs_seed_rec) * o.n_seeds);
-          memcpy(o.pos, res.pos + p0, 4 * o.n_pos);
         }
       };
-      const uint64_t bytes = res.n_seeds * sizeof(gs_seed_rec) + 4 * res.n_pos;
-      const size_t nt = std::min<size_t>(R->chunks.size(), bytes > (32u << 20) ? 4 : 1);
+      const size_t nt = std::min<size_t>({parts.size(), (size_t)8, (size_t)(bytes / (4u << 20)) + 1});
       std::vector<std::thread> helpers;
-      size_t step = nt;
       try {
-        for (size_t t = 1; t < nt; ++t) helpers.emplace_back(hand_off, t, nt);
-      } catch (...) {  // fewer helpers than planned: the chunks they would take are done here
-        step = 1;
+        for (size_t t = 1; t < nt; ++t) helpers.emplace_back(copier);
+      } catch (...) {  // fewer helpers: the parts they would take stay in the list for this thread
       }
-      if (step == 1 && !helpers.empty()) {
-        for (auto& h : helpers) h.join();
-        helpers.clear();
-        // the helpers that started took chunks t, t+nt, ...; redo every chunk
-        // (assign replaces a piece's contents, so copying one twice is harmless)
-      }
-      hand_off(0, step);
+      copier();
       for (auto& h : helpers) h.join();
       if (alloc_failed) rc = GS_ENOMEM;
     }
