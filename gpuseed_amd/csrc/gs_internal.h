@@ -129,7 +129,8 @@ struct PersistParams {
   uint4* dyn_meta;           // per piece, as seg_meta
   uint32_t* dyn_link;        // per piece: the next piece in task order, or ~0
   uint32_t* dyn_ready;       // per piece: == epoch once its entry is written (cleared per launch)
-  uint32_t* dyn_ctl;         // [0] pieces reserved, [64] next piece to claim, [128] items running
+  uint32_t* dyn_ctl;         // [0] pieces reserved, [64] next piece to claim, [kRunBase + kRunStride * i]
+                             // items running, counter i of kRunSlots (kDynCtlWords words)
   uint4* wmeta;              // per whole-read stream (2 * read + strand) that was split
   uint32_t* wlink;
   uint32_t dyn_cap, split_work, split_tasks, epoch;
@@ -175,6 +176,22 @@ constexpr uint32_t kSplitItemsPerLane = 1;
 // against 9.3 ms.
 constexpr uint32_t kSplitWorkWhole = 128;
 constexpr uint32_t kSplitTasksWhole = 4;
+// The running-items count of work splitting, spread over kRunSlots counters
+// 128 bytes apart: waves add to counter (wave mod kRunSlots), readers sum
+// them.  With one counter, whose atomics serialise in one L2 channel, a
+// launch of 2,000,000 150 bp reads took 56 ms with splitting on against 24
+// ms with it off; with 64 counters, read on every poll of the waiting waves,
+// 45 ms.
+constexpr uint32_t kRunBase = 128, kRunStride = 32, kRunSlots = 16;
+constexpr uint32_t kDynCtlWords = kRunBase + kRunStride * kRunSlots;
+// Waves that wait for split-off pieces once they have nothing to do: one in
+// kPollerEvery (about one per CU); the others leave at their first poll
+// that finds no piece.  On the 3.1 Gbp index, 80,000 150 bp reads: 8.0 ->
+// 7.0 ms per launch; 2,000 mixed-length reads: 4.5 -> 3.8 ms.  (A launch
+// of 2,000,000 reads with splitting forced on still takes twice as long as
+// without, 51 against 23 ms, in the items still running after the main
+// queue is exhausted; splitting is off by default at that size.)
+constexpr uint32_t kPollerEvery = 16;
 constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that trigger a refill
                                                 // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
                                                 // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms;

@@ -170,7 +170,20 @@ seed_persistent(PersistArgs P) {
   const gs::PersistCold* const C = P.cold;
   auto reserved = [&]() __attribute__((always_inline)) { return C->dyn_ctl; };
   auto next_piece = [&]() __attribute__((always_inline)) { return C->dyn_ctl + 64; };
-  auto running = [&]() __attribute__((always_inline)) { return C->dyn_ctl + 128; };
+  // items running: one counter per group of waves (gs::kRunSlots of them,
+  // 128 bytes apart), so that no single address takes every wave's atomics;
+  // a wave adds to its own and reads the total with one load per lane
+  auto run_slot = [&]() __attribute__((always_inline)) -> uint32_t* {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * BS + threadIdx.x) >> 6);
+    return C->dyn_ctl + gs::kRunBase + gs::kRunStride * (wave % gs::kRunSlots);
+  };
+  auto running_total = [&]() __attribute__((always_inline)) -> uint32_t {
+    uint32_t v = lane < (int)gs::kRunSlots ? __hip_atomic_load(C->dyn_ctl + gs::kRunBase + gs::kRunStride * lane,
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                           : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  };
 
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
@@ -581,20 +594,15 @@ seed_persistent(PersistArgs P) {
   // delays the split trigger or keeps waiting lanes polling a little longer.
   auto fold_running = [&]() __attribute__((always_inline)) {
     if ((++fold_wait & 31u) && __ballot(have) != 0) return;
-    // the split trigger's state, read by one lane for the wave: the main
-    // queue is exhausted and fewer items run than split_below
-    uint32_t t = 0;
-    if (lane == 0)
-      t = __hip_atomic_load(P.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n_items &&
-                  __hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < P.split_below
-              ? 1u
-              : 0u;
-    in_tail = __shfl(t, 0) != 0;
+    // the split trigger's state for the wave: the main queue is exhausted (a
+    // lane of the wave found it so) and fewer items run than split_below
+    const uint32_t total = running_total();
+    in_tail = __ballot(main_done) != 0 && total < P.split_below;
     if (__ballot(drun != 0) == 0) return;
     int d = drun;
     drun = 0;
     for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
-    if (lane == 0 && d) atomicAdd(running(), (uint32_t)d);
+    if (lane == 0 && d) atomicAdd(run_slot(), (uint32_t)d);
   };
   // overflow queue, polled by one lane per wave: claims pieces for the
   // wave's idle lanes with one CAS, or finds that none can come any more.
@@ -609,7 +617,7 @@ seed_persistent(PersistArgs P) {
       return;
     }
     const int leader = __ffsll((unsigned long long)idlemask) - 1;
-    uint32_t got = 0, first = 0, none = 0;
+    uint32_t got = 0, first = 0, none = 0, empty = 0;
     if (lane == leader) {
       const uint32_t nx = __hip_atomic_load(next_piece(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t rs = __hip_atomic_load(reserved(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -620,13 +628,21 @@ seed_persistent(PersistArgs P) {
           got = want;
           first = nx;
         }
-      } else if (__hip_atomic_load(running(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        none = 1;  // nothing to claim and no item anywhere that could still split
+      } else {
+        empty = 1;
       }
     }
     got = __shfl(got, leader);
     first = __shfl(first, leader);
-    none = __shfl(none, leader);
+    // nothing to claim: a poller wave (one in kPollerEvery) reads the running
+    // count, which decides whether any piece can still come; the others
+    // leave at once, so that the waiting waves' loads of the few control
+    // lines stay rare (every wave polling them slowed the items still
+    // running 2.4 times)
+    if (__shfl(empty, leader)) {
+      const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * BS + threadIdx.x) >> 6);
+      if (wave % gs::kPollerEvery != 0 || running_total() == 0) none = 1;
+    }
     if (idle) {
       const uint32_t rank = (uint32_t)__popcll(idlemask & ((1ull << lane) - 1ull));
       if (rank < got)
@@ -1237,7 +1253,7 @@ int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid
   if (!grid) return GS_EINVAL;
   hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, P.queue, 1u);
   if (P.dyn_cap) {  // an empty overflow queue: no piece reserved, claimed or written
-    hipLaunchKernelGGL(zero_words, dim3(3), dim3(64), 0, st, P.dyn_ctl, 192u);
+    hipLaunchKernelGGL(zero_words, dim3((gs::kDynCtlWords + 63) / 64), dim3(64), 0, st, P.dyn_ctl, gs::kDynCtlWords);
     hipLaunchKernelGGL(zero_words, dim3((P.dyn_cap + 63) / 64), dim3(64), 0, st, P.dyn_ready, P.dyn_cap);
   }
   if (!P.counters) return GS_EINVAL;  // its buffer also holds the launch's PersistCold

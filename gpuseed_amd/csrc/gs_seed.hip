@@ -993,7 +993,7 @@ static int ensure_buffers(gs_batch* B, uint64_t span, uint64_t n, uint64_t tasks
       (dyn_cap &&
        (B->dyn_items.ensure(16ull * dyn_cap) || B->dyn_meta.ensure(16ull * dyn_cap) ||
         B->dyn_link.ensure(4ull * dyn_cap) || B->dyn_ready.ensure(4ull * dyn_cap) ||
-        B->dyn_ctl.ensure(1024) || B->wmeta.ensure(32 * nr1) || B->wlink.ensure(8 * nr1))))
+        B->dyn_ctl.ensure(4ull * gs::kDynCtlWords) || B->wmeta.ensure(32 * nr1) || B->wlink.ensure(8 * nr1))))
     return GS_ENOMEM;
   return GS_OK;
 }
