@@ -178,19 +178,18 @@ constexpr uint32_t kSplitWorkWhole = 128;
 constexpr uint32_t kSplitTasksWhole = 4;
 // The running-items count of work splitting, spread over kRunSlots counters
 // 128 bytes apart: waves add to counter (wave mod kRunSlots), readers sum
-// them.  With one counter, whose atomics serialise in one L2 channel, a
-// launch of 2,000,000 150 bp reads took 56 ms with splitting on against 24
-// ms with it off; with 64 counters, read on every poll of the waiting waves,
-// 45 ms.
+// them.  One counter, folded once per wave and iteration, serialised every
+// wave's atomics on one address: 10,000 mixed-length reads took 9.1 ms per
+// launch with splitting on against 6.3 ms with it off.
 constexpr uint32_t kRunBase = 128, kRunStride = 32, kRunSlots = 16;
 constexpr uint32_t kDynCtlWords = kRunBase + kRunStride * kRunSlots;
 // Waves that wait for split-off pieces once they have nothing to do: one in
 // kPollerEvery (about one per CU); the others leave at their first poll
 // that finds no piece.  On the 3.1 Gbp index, 80,000 150 bp reads: 8.0 ->
-// 7.0 ms per launch; 2,000 mixed-length reads: 4.5 -> 3.8 ms.  (A launch
-// of 2,000,000 reads with splitting forced on still takes twice as long as
-// without, 51 against 23 ms, in the items still running after the main
-// queue is exhausted; splitting is off by default at that size.)
+// 7.0 ms per launch; 2,000 mixed-length reads: 4.5 -> 3.8 ms.  (Forced on
+// in a launch of 2,000,000 150 bp reads, 15 items per lane, splitting after
+// 128 operations makes it twice as long, 51 against 23 ms: the pieces start
+// without size records; the default keeps it off at that size.)
 constexpr uint32_t kPollerEvery = 16;
 constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that trigger a refill
                                                 // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
