@@ -631,7 +631,7 @@ def main():
     if not args.no_cpu:
         wm = work_model(prefix, first_arrays, args.work_sample, threads, k, max_occ, info.kmer_len, smem,
                         direct_min=4 if direct else 0, prune_sat={1: 255, 2: 65535, 3: 15}.get(args.kernel, 0),
-                        pair_probe=3 if pairs else 0, tandem=0 if args.no_tandem else 6)
+                        pair_probe=2 if pairs else 0, tandem=0 if args.no_tandem else 6)
         log("work model: %.1f table + %.1f occ + %.1f direct requests per read (%.0f bytes)" %
             (wm["table_per_read"], wm["occ_per_read"], wm["direct_per_read"], wm["bytes_per_read"]))
     serving = None

@@ -75,7 +75,6 @@ __device__ __forceinline__ uint64_t rev2(uint64_t x) {
   return ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
 }
 constexpr uint32_t kBad13 = (1u << gs::kPrecalc) - 1u;  // the 13-mer window's codes in a 19-code window
-constexpr int kWin = 24;  // codes a task end keeps in registers (wcode, wbad)
 
 // A 16-byte chunk of 32 4-bit codes as the persistent kernel's LDS slot
 // holds it: {codes at 2 bits in reverse order (low, high word), a bit per
@@ -189,19 +188,18 @@ seed_persistent(PersistArgs P) {
   bool have = false, finished = false, main_done = false;
   int kind = 0;  // 0: a whole read's stream, 1: a segment of a long read, 2: a split-off piece
   uint32_t r = 0, strand = 0, L = 0, nout = 0, last_b = 0;
-  // the kWin (24) codes left of the task end e: code(e-1-j) at bits 2j of
-  // wcode for j = 0..23, and bit j of wbad set if that code is above 3 or its
-  // position is left of the read start; the 13-mer pattern is the low 26
-  // bits, an extended window and its 3 presence bases the bits above,
-  // code(e-20) (the base left of the 19-base window) bits 38-39, code(e-21),
-  // code(e-22) (for a second pair byte) bits 40-43 and code(e-23),
-  // code(e-24) (for a third) the top four
+  // the 22 codes left of the task end e: code(e-1-j) at bits 2j of wcode for
+  // j = 0..21, and bit j of wbad set if that code is above 3 or its position
+  // is left of the read start; the 13-mer pattern is the low 26 bits, an
+  // extended window and its 3 presence bases the bits above, code(e-20)
+  // (the base left of the 19-base window) bits 38-39, and code(e-21),
+  // code(e-22) (for a second pair byte) the top four
   uint64_t wcode = 0;
   uint32_t wbad = 0;
   // pair table: probe is set while the last clean 19-base window looked up
   // was absent; the last probe also answered the windows of the task ends
-  // known_e, known_e - 1, ... (their number in bits 8 and up of known_m;
-  // bit i set iff the window of known_e - i occurs)
+  // known_e, known_e - 1, ... (their number in bits 4-5 of known_m; bit i
+  // set iff the window of known_e - i occurs)
   const bool pair_on = P.X.pair && P.X.ext_len == 16 && k >= 19;
   bool probe = false;
   int32_t known_e = -1;
@@ -258,11 +256,11 @@ seed_persistent(PersistArgs P) {
     return (w >> ((i & 7) << 2)) & 15u;
   };
 
-  // the kWin codes left of task end ee into wcode / wbad (code(ee-1-j) at
-  // bits 2j, bit j set for a code above 3 or a position left of the read
-  // start): one run of bits of the slot, unless the window reaches left of a
+  // the 22 codes left of task end ee into wcode / wbad (code(ee-1-j) at bits
+  // 2j, bit j set for a code above 3 or a position left of the read start):
+  // one run of bits of the slot, unless the window reaches left of a
   // segment's LDS window (then position by position, from device memory)
-  // the same codes and bits from the slot alone, for ee - kWin >= w0 or
+  // the same 22 codes and bits from the slot alone, for ee - 22 >= w0 or
   // w0 == 0 (positions left of the read start read the zero code words and
   // the all-ones bit word after the window)
   auto window_bits = [&](int32_t ee, uint64_t& w, uint32_t& bad) __attribute__((always_inline)) {
@@ -271,17 +269,17 @@ seed_persistent(PersistArgs P) {
     const uint32_t wi = r1 >> 4, o = (r1 & 15u) << 1;
     const uint64_t lo = (uint64_t)cw[wi] | ((uint64_t)cw[wi + 1] << 32);
     const uint64_t hi = cw[wi + 2];
-    w = (o ? (lo >> o) | (hi << (64 - o)) : lo) & ((1ull << (2 * kWin)) - 1ull);
+    w = (o ? (lo >> o) | (hi << (64 - o)) : lo) & ((1ull << 44) - 1ull);
     const uint32_t bi = CW + (r1 >> 5), bo = r1 & 31u;
     const uint64_t bb = (uint64_t)cw[bi] | ((uint64_t)cw[bi + 1] << 32);
-    bad = (uint32_t)(bb >> bo) & ((1u << kWin) - 1u);
+    bad = (uint32_t)(bb >> bo) & 0x3FFFFFu;
   };
   auto load_window = [&](int32_t ee) __attribute__((always_inline)) {
-    if (w0 > 0 && ee - kWin < w0) {
+    if (w0 > 0 && ee - 22 < w0) {
       uint64_t w = 0;
       uint32_t bad = 0;
 #pragma unroll 1
-      for (int j = 0; j < kWin; ++j) {
+      for (int j = 0; j < 22; ++j) {
         const int32_t pos = ee - 1 - j;
         const uint32_t c = pos >= 0 ? code(pos) : 15u;
         bad |= (c > 3 ? 1u : 0u) << j;
@@ -358,13 +356,13 @@ seed_persistent(PersistArgs P) {
         if (q > P.tandem) q = 0;
       }
       if (q) {
-        // R: z falls while code(z) == code(z + q) <= 3, kWin positions per
+        // R: z falls while code(z) == code(z + q) <= 3, 22 positions per
         // step from the slot (the first position that differs, holds a code
         // above 3 or lies left of the read start stops it), one at a time
         // near a segment window's left edge
         int32_t z = e - 21;
         while (z >= 0) {
-          if (w0 > 0 && z + 1 - kWin < w0) {
+          if (w0 > 0 && z - 21 < w0) {
             const uint32_t c = code(z);
             if (c > 3 || c != code(z + (int32_t)q)) break;
             --z;
@@ -380,11 +378,11 @@ seed_persistent(PersistArgs P) {
           const uint32_t bm = ba | bq;
           const uint32_t jb = bm ? (uint32_t)__builtin_ctz(bm) : 32u;
           const uint32_t j0 = jd < jb ? jd : jb;
-          if (j0 < (uint32_t)kWin) {
+          if (j0 < 22u) {
             z -= (int32_t)j0;
             break;
           }
-          z -= kWin;
+          z -= 22;
         }
         if (e - (z + 1) >= (int32_t)gs::kTandemSpan) {
           tp = q;
@@ -416,7 +414,7 @@ seed_persistent(PersistArgs P) {
     // absent (their windows are clean), then past 13-base windows holding a
     // code above 3, jumping below the highest such code at once
     int32_t ne = e - 1;
-    for (uint32_t ki = (uint32_t)(known_e - ne); ki < (known_m >> 8) && !((known_m >> ki) & 1u); ++ki) {
+    for (uint32_t ki = (uint32_t)(known_e - ne); ki < (known_m >> 4) && !((known_m >> ki) & 1u); ++ki) {
       --ne;
       probe = true;
     }
@@ -711,9 +709,6 @@ seed_persistent(PersistArgs P) {
     uint32_t sub = 0, sub2 = 0;
     bool pair_read = false;  // op 1 reads the pair byte instead of a table entry
     bool dual = false;       // and a second one
-    bool triple = false;     // and a third
-    const uint4* a3 = nullptr;
-    uint32_t sub3 = 0;
     uint32_t i1 = 0, i2 = 0, b1 = 0, b2 = 0;
     bool have1 = false;
     uint32_t tlen = gs::kPrecalc, tidx = (uint32_t)wcode & 0x3FFFFFFu;  // table lookup: pattern length and entry
@@ -731,10 +726,9 @@ seed_persistent(PersistArgs P) {
       }
       // after an absent window, a task whose 20 codes are clean and whose
       // window no probe answered reads the pair byte of its 18-base core
-      // read[e-19, e-1) instead, with read[e-22, e-20) clean too that of
-      // read[e-21, e-3), and with read[e-24, e-22) clean too that of
-      // read[e-23, e-5), in the same operation
-      if (pair_on && probe && (uint32_t)(known_e - e) >= (known_m >> 8) && !(wbad & 0xFFFFFu)) {
+      // read[e-19, e-1) instead, and with read[e-22, e-20) clean too that of
+      // read[e-21, e-3) in the same operation
+      if (pair_on && probe && (uint32_t)(known_e - e) >= (known_m >> 4) && !(wbad & 0xFFFFFu)) {
         const uint64_t pi = (wcode >> 2) & ((1ull << 36) - 1ull);
         a1 = reinterpret_cast<const uint4*>(P.X.pair) + (pi >> 4);
         sub = (uint32_t)pi & 15u;
@@ -744,16 +738,10 @@ seed_persistent(PersistArgs P) {
           a2 = reinterpret_cast<const uint4*>(P.X.pair) + (pj >> 4);
           sub2 = (uint32_t)pj & 15u;
           dual = true;
-          if (!(wbad & (3u << 22))) {
-            const uint64_t pk = (wcode >> 10) & ((1ull << 36) - 1ull);
-            a3 = reinterpret_cast<const uint4*>(P.X.pair) + (pk >> 4);
-            sub3 = (uint32_t)pk & 15u;
-            triple = true;
-          }
         }
       }
-      n_table += 1u + (dual ? 1u : 0u) + (triple ? 1u : 0u);
-      work += 1u + (dual ? 1u : 0u) + (triple ? 1u : 0u);
+      n_table += dual ? 2 : 1;
+      work += dual ? 2 : 1;
     } else if (op == 2) {
       const uint32_t r1 = l - 1u;
       have1 = r1 != 0xFFFFFFFFu;
@@ -786,8 +774,6 @@ seed_persistent(PersistArgs P) {
     if (a1) d1 = *a1;
     uint4 d2 = d1;
     if (a2) d2 = *a2;
-    uint4 d3 = d1;
-    if (a3) d3 = *a3;
     // ops 3 and 5: word `sub`; the pair byte: byte `sub`
     const uint32_t word = pick4(pair_read ? sub >> 2 : sub, d1.x, d1.y, d1.z, d1.w);
     const uint32_t v4 = pair_read ? (word >> ((sub & 3u) << 3)) & 255u : word;
@@ -901,8 +887,7 @@ seed_persistent(PersistArgs P) {
       // window read[e-20, e-1) does; an absent window ends the task as an
       // absent presence bit would, a present one is looked up next
       // a second byte's bit code(e-3) and bit 4 + code(e-22) answer the
-      // windows of e-2 and e-3, a third byte's bit code(e-5) and bit 4 +
-      // code(e-24) those of e-4 and e-5
+      // windows of e-2 and e-3
       known_e = e - 1;
       uint32_t m = (v4 >> (4u + ((uint32_t)(wcode >> 38) & 3u))) & 1u;
       if (dual) {
@@ -910,12 +895,7 @@ seed_persistent(PersistArgs P) {
         m |= ((vb >> ((uint32_t)(wcode >> 4) & 3u)) & 1u) << 1;
         m |= ((vb >> (4u + ((uint32_t)(wcode >> 42) & 3u))) & 1u) << 2;
       }
-      if (triple) {
-        const uint32_t vc = (pick4(sub3 >> 2, d3.x, d3.y, d3.z, d3.w) >> ((sub3 & 3u) << 3)) & 255u;
-        m |= ((vc >> ((uint32_t)(wcode >> 8) & 3u)) & 1u) << 3;
-        m |= ((vc >> (4u + ((uint32_t)(wcode >> 46) & 3u))) & 1u) << 4;
-      }
-      known_m = m | ((triple ? 5u : (dual ? 3u : 1u)) << 8);
+      known_m = m | ((dual ? 3u : 1u) << 4);
       if (!((v4 >> ((uint32_t)wcode & 3u)) & 1u))
         res = 1;
       else

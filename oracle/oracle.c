@@ -535,17 +535,14 @@ static uint32_t strand_candidates(const or_index* X, const uint8_t* v, uint32_t 
             if (present) ++w->ext;
           } else if (probe && e >= 20 && v[e - 20] <= 3) { /* this task's pair byte(s) */
             /* with 22 clean codes (pair_probe 2) a second byte, of the core
-             * read[e-21, e-3), answers the windows of e-2 and e-3; with 24
-             * (pair_probe 3) a third, of read[e-23, e-5), those of e-4 and e-5 */
+             * read[e-21, e-3), answers the windows of e-2 and e-3 */
             const int dual = pair_probe >= 2 && e >= 22 && v[e - 21] <= 3 && v[e - 22] <= 3;
-            const int triple = pair_probe >= 3 && dual && e >= 24 && v[e - 23] <= 3 && v[e - 24] <= 3;
-            w->pair += 1 + dual + triple;
+            w->pair += dual ? 2 : 1;
             ++w->pair_ops;
             known_e = (int64_t)e - 1;
-            known_n = triple ? 5 : (dual ? 3 : 1);
+            known_n = dual ? 3 : 1;
             known_m = window19(X, v, e - 1) ? 1u : 0u;
             if (dual) known_m |= (window19(X, v, e - 2) ? 2u : 0u) | (window19(X, v, e - 3) ? 4u : 0u);
-            if (triple) known_m |= (window19(X, v, e - 4) ? 8u : 0u) | (window19(X, v, e - 5) ? 16u : 0u);
             if (present) ++w->ext;
           } else {
             ++w->ext;

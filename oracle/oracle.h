@@ -111,12 +111,10 @@ typedef struct {
    * whose window is absent ends without its extended-table lookup; the next
    * task, answered by the same byte, needs no lookup at all when its window
    * is absent.  A present window is then looked up as usual (w_ext).
-   * pair_probe 2: a task whose 22 bases read[e-22, e) are clean reads a
-   * second byte in the same operation, that of the core read[e-21, e-3),
-   * which answers the windows of task ends e-2 and e-3 likewise;
-   * pair_probe 3 (the device's): with 24 clean bases read[e-24, e) a third,
-   * of read[e-23, e-5), answers those of e-4 and e-5.  Output unchanged;
-   * counters only.  0: off. */
+   * pair_probe 2 (the device's): a task whose 22 bases read[e-22, e) are
+   * clean reads a second byte in the same operation, that of the core
+   * read[e-21, e-3), which answers the windows of task ends e-2 and e-3
+   * likewise.  Output unchanged; counters only.  0: off. */
   int pair_probe;
   /* tandem: work model of the device's tandem jumps (maximum period, at most
    * 8; 0: off).  Inside a stretch [R, ..) of the read with period p (code(z)

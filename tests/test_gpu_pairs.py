@@ -105,7 +105,7 @@ def test_seeds_unchanged_and_work_equals_the_model(g, tmp_path, smem, k):
             st = b.run()
             b.close()
             c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1, ext_len=16, prune_sat=sat, miss_skip=1,
-                               direct_min=3, pair_probe=3, tandem=6)[4]
+                               direct_min=3, pair_probe=2, tandem=6)[4]
             say("  kernel: table %d occ %d direct %d; model: %d + %d + %d pair, %d, %d" %
                 (st.n_table_reads, st.n_occ_reads, st.n_direct_reads, c["w_tasks"], c["w_ext"], c["w_pair"],
                  c["w_blocks96"], c["w_direct"]))

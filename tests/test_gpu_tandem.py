@@ -84,7 +84,7 @@ def test_tandem_seeds_and_work_equal_the_model(g, case, index, smem, k):
                 b.close()
                 c = ob.seed_arrays(ox, *arrays, k, smem, threads=8, smem_stop=1, ext_len=ext, prune_sat=sat,
                                    miss_skip=1 if ext else 0, direct_min=3 if direct else 0,
-                                   pair_probe=3 if direct else 0, tandem=tp, tandem_check=1)[4]
+                                   pair_probe=2 if direct else 0, tandem=tp, tandem_check=1)[4]
                 say("  kernel table %d occ %d direct %d tandem %d; model %d %d %d %d (stretches %d)" %
                     (st.n_table_reads, st.n_occ_reads, st.n_direct_reads, st.n_tandem_reads,
                      c["w_tasks"] + c["w_ext"] + c["w_pair"], c["w_blocks96"], c["w_direct"], c["w_tandem"],

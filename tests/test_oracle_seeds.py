@@ -204,10 +204,9 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
                 assert cg == noskip[(stop, ext)], (smem, k, stop, ext)
     # the pair-table model: with the 16-mer table and k >= 19, a lookup of an
     # 18-base core answers two adjacent windows (pair_probe 2: two cores in
-    # one operation answer four; 3: three cores, six); same seeds, fewer
-    # table reads; with k < 19 (or no 16-mer table) it must not apply
-    pair_ops = {}
-    for (smem, k), pp in itertools.product(((True, 19), (False, 20), (True, 16)), (1, 2, 3)):
+    # one operation answer four); same seeds, fewer table reads; with k < 19
+    # (or no 16-mer table) it must not apply
+    for (smem, k), pp in itertools.product(((True, 19), (False, 20), (True, 16)), (1, 2)):
         off = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=16, miss_skip=1)
         for ext in (16, 14):
             on = ob.seed_arrays(ox, buf, offs, lens, k, smem, threads=4, smem_stop=1, ext_len=ext, miss_skip=1,
@@ -224,9 +223,6 @@ def test_work_model_options_do_not_change_output(built, tmp_path):
             # one operation per byte, or fewer operations than bytes with two per operation
             assert (cn["w_pair_ops"] == cn["w_pair"]) == (pp == 1), (smem, k, pp)
             assert cn["w_ext"] < co["w_ext"]
-            pair_ops[(smem, k, pp)] = cn["w_pair_ops"]
-            if pp == 3:  # more windows per operation: fewer operations
-                assert cn["w_pair_ops"] < pair_ops[(smem, k, 2)] < pair_ops[(smem, k, 1)], (smem, k)
             # everything after the table lookups is the same work
             assert all(cn[f] == co[f] for f in ("w_tasks", "w_steps", "w_blocks96", "w_direct")), (smem, k)
     # saturating size records only lose pruning: same seeds, at least as

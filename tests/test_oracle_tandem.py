@@ -19,7 +19,7 @@ import brute
 import oracle_bind as ob
 from conftest import build_index, reads_over, tandem_text
 
-DEVICE = dict(smem_stop=1, ext_len=16, miss_skip=1, direct_min=4, prune_sat=15, pair_probe=3)
+DEVICE = dict(smem_stop=1, ext_len=16, miss_skip=1, direct_min=4, prune_sat=15, pair_probe=2)
 
 
 def as_tuples(res):
