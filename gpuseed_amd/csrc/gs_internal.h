@@ -191,10 +191,12 @@ constexpr uint32_t kDynCtlWords = kRunBase + kRunStride * kRunSlots;
 // 128 operations makes it twice as long, 51 against 23 ms: the pieces start
 // without size records; the default keeps it off at that size.)
 constexpr uint32_t kPollerEvery = 16;
-constexpr uint32_t kRefillGroup = 8;            // idle lanes of a wave that trigger a refill
-                                                // (2.5M 150 bp reads, 3.1 Gbp index: 1 lane
-                                                // 72.5 ms, 4: 58.0 ms, 8: 58.2 ms, 16: 59.6 ms;
-                                                // 2, 3, 4 and 6 within 1 %)
+constexpr uint32_t kRefillGroup = 12;           // idle lanes of a wave that trigger a refill
+                                                // (2.5M 150 bp reads, 3.1 Gbp index, an early
+                                                // form of the kernel: 1 lane 72.5 ms, 4: 58.0 ms,
+                                                // 8: 58.2 ms, 16: 59.6 ms; 10M reads with the
+                                                // current one, alternating runs on one box: 8:
+                                                // 68.7-69.0 ms, 12: 67.6-68.1 ms median)
 
 constexpr uint32_t kPersistLmax = 160;  // LDS code window of a persistent lane, in bases
 // Tandem jumps of the persistent kernel: periods 1..kTandemMax; a stretch
