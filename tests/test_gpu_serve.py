@@ -161,3 +161,24 @@ def test_failed_round_fails_its_requests_only(g, world):
             with pytest.raises(g.GsError) as ei:  # the server reports the failed round
                 S.close()
             assert ei.value.rc == g.GS_EHIP
+
+
+def test_large_request_handed_off_in_parts(g, world):
+    """One request of 300,000 150 bp reads in a round of its own: its results
+    are copied to the caller in parts (131,072 reads, 131,072 seed records
+    and 1,048,576 positions a part, on several threads) and must still equal
+    gs_seed on the request alone."""
+    rp, _ = g.preset(4)  # the reference of the world's index
+    rp.length = 3000000
+    _, qp = g.preset(3)  # 150 bp reads
+    qp.n_reads = 300000
+    ref = g.synth_reference(rp)
+    arrays = g.synth_reads(ref, qp)[:3]
+    with g.Index(world["prefix"]) as X:
+        alone = g.seed(X, arrays=arrays, k=19, smem=True, kernel=3)
+        say("alone: %d seeds, %d positions" % (len(alone.seeds), len(alone.pos)))
+        assert len(alone.seeds) > 131072 and len(alone.pos) > 1048576  # several parts of each kind
+        with g.Server(X, exclusive=True, k=19, smem=True) as S:
+            out, info = S.wait(S.submit(arrays))
+        assert info.n_reads == 300000 and info.n_rounds == 1
+        assert same(out, alone)
