@@ -529,7 +529,7 @@ class BatchStats(C.Structure):
                 ("t_write", C.c_double), ("t_locate", C.c_double), ("t_total", C.c_double),
                 ("n_reads", C.c_uint64), ("n_seeds", C.c_uint64), ("n_pos", C.c_uint64),
                 ("n_table_reads", C.c_uint64), ("n_occ_reads", C.c_uint64), ("n_direct_reads", C.c_uint64),
-                ("n_tandem_reads", C.c_uint64)]
+                ("n_tandem_reads", C.c_uint64), ("t_kernel", C.c_double)]
 
 
 def _bind_batch(L):

@@ -232,7 +232,9 @@ int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t
 // variant 0: 8-bit size records, 1: 16-bit size records; wg 64, 128 or 256
 // (0 = 128); grid from persistent_grid (called outside any graph capture)
 int persistent_grid(int variant, int wg, unsigned* grid);
-int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st);
+// e0, e1 (optional): recorded on st right before and after seed_persistent
+int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int launch_join(const JoinParams& J, hipStream_t st);
 
 }  // namespace gs

@@ -1162,7 +1162,7 @@ int grid_t(unsigned* grid) {
 __global__ void write_cold(gs::PersistCold c, gs::PersistCold* dst) { *dst = c; }
 
 template <typename AT, int LMAX, int BS>
-int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
+int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   gs::PersistCold c;
   c.code_off = P.code_off;
   c.lens = P.lens;
@@ -1200,8 +1200,10 @@ int launch_t(const gs::PersistParams& P, unsigned grid, hipStream_t st) {
   A.split_below = grid * (uint32_t)BS / 2;  // half the lanes of the launch
   A.refill_group = gs::kRefillGroup;
   A.tandem = P.tandem;
+  if (e0 && hipEventRecord(e0, st) != hipSuccess) return GS_EHIP;
   hipLaunchKernelGGL((seed_persistent<AT, LMAX, BS>), dim3(grid), dim3(BS), 0, st, A);
-  return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+  if (hipGetLastError() != hipSuccess) return GS_EHIP;
+  return e1 && hipEventRecord(e1, st) != hipSuccess ? GS_EHIP : GS_OK;
 }
 
 }  // namespace
@@ -1249,7 +1251,8 @@ int persistent_grid(int variant, int wg, unsigned* grid) {
   }
 }
 
-int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st) {
+int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid, hipStream_t st, hipEvent_t e0,
+                      hipEvent_t e1) {
   if (!grid) return GS_EINVAL;
   hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, P.queue, 1u);
   if (P.dyn_cap) {  // an empty overflow queue: no piece reserved, claimed or written
@@ -1260,15 +1263,15 @@ int launch_persistent(const PersistParams& P, int variant, int wg, unsigned grid
   hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, (uint32_t*)P.counters, 8u);
   constexpr int L = kPersistLmax;
   switch (variant * 1000 + (wg ? wg : 128)) {
-    case 64: return launch_t<uint8_t, L, 64>(P, grid, st);
-    case 128: return launch_t<uint8_t, L, 128>(P, grid, st);
-    case 256: return launch_t<uint8_t, L, 256>(P, grid, st);
-    case 1064: return launch_t<uint16_t, L, 64>(P, grid, st);
-    case 1128: return launch_t<uint16_t, L, 128>(P, grid, st);
-    case 1256: return launch_t<uint16_t, L, 256>(P, grid, st);
-    case 2064: return launch_t<Nib, L, 64>(P, grid, st);
-    case 2128: return launch_t<Nib, L, 128>(P, grid, st);
-    case 2256: return launch_t<Nib, L, 256>(P, grid, st);
+    case 64: return launch_t<uint8_t, L, 64>(P, grid, st, e0, e1);
+    case 128: return launch_t<uint8_t, L, 128>(P, grid, st, e0, e1);
+    case 256: return launch_t<uint8_t, L, 256>(P, grid, st, e0, e1);
+    case 1064: return launch_t<uint16_t, L, 64>(P, grid, st, e0, e1);
+    case 1128: return launch_t<uint16_t, L, 128>(P, grid, st, e0, e1);
+    case 1256: return launch_t<uint16_t, L, 256>(P, grid, st, e0, e1);
+    case 2064: return launch_t<Nib, L, 64>(P, grid, st, e0, e1);
+    case 2128: return launch_t<Nib, L, 128>(P, grid, st, e0, e1);
+    case 2256: return launch_t<Nib, L, 256>(P, grid, st, e0, e1);
     default: return GS_EINVAL;
   }
 }

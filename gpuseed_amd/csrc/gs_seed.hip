@@ -799,6 +799,7 @@ struct gs_batch {
   uint64_t n_seeds = 0, n_pos = 0;
   bool ran = false;
   hipEvent_t ev[7] = {};
+  hipEvent_t kev[2] = {};  // around the extension kernel's own launch (t_kernel)
   double t_h2d = 0;
   uint64_t* h_tot = nullptr;  // pinned: totals of the count stage
   // captured pipeline (gs_batch_capture) and its pinned staging
@@ -967,6 +968,8 @@ uint64_t gs::plan_device_bytes(const BatchPlan& P) {
 static int batch_events(gs_batch* B) {
   if (!B->stream && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) return GS_EHIP;
   for (auto& e : B->ev)
+    if (!e && hipEventCreate(&e) != hipSuccess) return GS_EHIP;
+  for (auto& e : B->kev)
     if (!e && hipEventCreate(&e) != hipSuccess) return GS_EHIP;
   if (!B->h_tot && hipHostMalloc((void**)&B->h_tot, 16, hipHostMallocDefault) != hipSuccess) return GS_ENOMEM;
   return GS_OK;
@@ -1228,14 +1231,20 @@ static void fill_params(gs_batch* B, SeedParams& S, PostParams& Q) {
   Q.long_reads = B->long_reads.as<uint32_t>();
 }
 
-static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsigned pgrid, hipStream_t st) {
+// kev (null under graph capture): two events recorded on st right before and
+// after the extension kernel itself, without the code packing, the queue's
+// set-up and the join of split streams around it
+static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsigned pgrid, hipStream_t st,
+                             hipEvent_t* kev = nullptr) {
   if (!B->n_reads) return GS_OK;
   if (kernel == 0) {
+    if (kev && hipEventRecord(kev[0], st) != hipSuccess) return GS_EHIP;
     if (B->maxlen <= 160)
       launch_seed<uint16_t, 160>(S, st);
     else
       launch_seed<uint16_t, 256>(S, st);
-    return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
+    if (hipGetLastError() != hipSuccess) return GS_EHIP;
+    return kev && hipEventRecord(kev[1], st) != hipSuccess ? GS_EHIP : GS_OK;
   }
   gs::PersistParams PP;
   PP.X = S.X;
@@ -1296,7 +1305,8 @@ static int enqueue_extension(gs_batch* B, const SeedParams& S, int kernel, unsig
   J.pad = 0;
   int rc = gs::launch_pack_codes(S.bases, S.offs, S.lens, PP.code_off, S.n_reads, J.long_reads, B->n_long,
                                  B->codes.as<uint4>(), st);
-  if (!rc) rc = gs::launch_persistent(PP, kernel - 1, B->opts.wg, pgrid, st);
+  if (!rc) rc = gs::launch_persistent(PP, kernel - 1, B->opts.wg, pgrid, st, kev ? kev[0] : nullptr,
+                                      kev ? kev[1] : nullptr);
   if (!rc) rc = gs::launch_join(J, st);
   return rc;
 }
@@ -1365,7 +1375,7 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
   fill_params(B, S, Q);
   B->h_tot[0] = B->h_tot[1] = 0;
   GS_HIP(hipEventRecord(B->ev[0], st));
-  rc = enqueue_extension(B, S, kernel, pgrid, st);
+  rc = enqueue_extension(B, S, kernel, pgrid, st, B->kev);
   if (rc) return rc;
   GS_HIP(hipEventRecord(B->ev[1], st));
   rc = enqueue_counts(B, Q, st);
@@ -1401,6 +1411,8 @@ extern "C" GS_API int gs_batch_run(gs_batch* B, int kernel_override, gs_batch_st
     stats->t_write = ms[2] * 1e-3;
     stats->t_locate = ms[3] * 1e-3;
     stats->t_total = ms[4] * 1e-3;
+    float kms = 0;
+    if (B->n_reads && hipEventElapsedTime(&kms, B->kev[0], B->kev[1]) == hipSuccess) stats->t_kernel = kms * 1e-3;
     stats->n_reads = B->n_reads;
     stats->n_seeds = B->n_seeds;
     stats->n_pos = B->n_pos;

@@ -378,6 +378,10 @@ typedef struct {
                       extension kernel (0 for the plain kernel)                */
   uint64_t n_direct_reads; /* its suffix-array, text and inverse reads       */
   uint64_t n_tandem_reads; /* its tandem-table entries read (gs_opts.tandem) */
+  double t_kernel; /* the extension kernel's launch alone (seed_persistent, or
+                      the plain kernel): t_seed less code packing, the work
+                      queue's set-up and the join of split streams; 0 from
+                      gs_batch_replay, whose graph holds no timing events)  */
 } gs_batch_stats;
 
 GS_API int gs_batch_create(gs_index* idx, const char* bases, const uint64_t* offsets,

@@ -88,4 +88,7 @@ def test_batch_api_matches_gs_seed(built, tmp_path):
     for o in (two, three, four):
         assert np.array_equal(one.seeds, o.seeds) and np.array_equal(one.pos, o.pos)
     assert s2.t_seed > 0 and s2.t_total >= s2.t_seed
+    # the extension kernel's own launch lies inside the seed stage's events
+    for s in (s1, s2, s3, s4):
+        assert 0 < s.t_kernel <= s.t_seed
     assert s3.n_table_reads > 0 and s3.n_occ_reads > 0
