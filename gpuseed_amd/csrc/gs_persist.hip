@@ -39,6 +39,7 @@
 //    yet checks again next iteration, and a lane leaves once the main queue
 //    is exhausted, no piece is left to claim and no item is running anywhere
 //    (so no piece can appear any more).
+#include <algorithm>
 #include <type_traits>
 
 #include "gs_internal.h"
@@ -54,6 +55,9 @@ __device__ __forceinline__ uint32_t pick4(uint32_t c, uint32_t a0, uint32_t a1, 
 }
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// workgroups (of four waves) of the helper kernels that take reads or
+// streams in turn: 16 per CU on 256 CUs
+constexpr uint32_t kHelperGrid = 4096;
 
 // 8 4-bit codes (code i in bits 4i..4i+3) -> their low two bits packed (code
 // i in bits 2i, 2i+1) and a bit per code above 3 (bit i)
@@ -1048,28 +1052,43 @@ __device__ __forceinline__ void join_chain(const gs::JoinParams& J, uint4* out, 
   }
 }
 
-// Streams of long reads, one wave per stream (a 20 kbp read has about 20,000
-// candidates per strand to move): segments in order, each followed by the
-// chain of pieces split off it; every lane follows the same metadata and the
-// lanes move 64 candidates at a time.
+// Streams of long reads, one wave per stream at a time (a 20 kbp read has
+// about 20,000 candidates per strand to move): segments in order, each
+// followed by the chain of pieces split off it; every lane follows the same
+// metadata and the lanes move 64 candidates at a time.  A fixed grid of
+// waves takes the streams in turn (a workgroup per four streams left the
+// launch bound by workgroup dispatch), and the metadata of 64 segments is
+// loaded at once, one segment per lane, then read from the lanes in order.
 __global__ void __launch_bounds__(256) join_segments(gs::JoinParams J) {
-  const uint64_t t = blockIdx.x * 4ull + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
-  if (t >= 2ull * J.n_long) return;
-  const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
-  const uint32_t L = J.lens[r];
-  const uint32_t ntask = L - J.k + 1;
-  const uint32_t S = J.seg_tasks;
-  const uint32_t nseg = (ntask + S - 1) / S;
-  uint4* out = J.cand + 2 * J.task_off[r];
-  const uint64_t i0 = J.item0[t >> 1] + strand;
-  uint32_t K = 0, LB = 0xFFFFFFFFu;
-  for (uint32_t g = 0; g < nseg; ++g) {
-    const uint64_t it = i0 + 2 * (uint64_t)g;
-    join_piece(out, ntask, strand, g * S, J.seg_meta[it], J.smem, K, LB, lane, 64);
-    if (J.dyn_cap) join_chain(J, out, ntask, strand, J.seg_link[it], K, LB, lane, 64);
+  const uint64_t n_streams = 2ull * J.n_long, n_waves = gridDim.x * 4ull;
+  for (uint64_t t = blockIdx.x * 4ull + (threadIdx.x >> 6); t < n_streams; t += n_waves) {
+    const uint32_t r = J.long_reads[t >> 1], strand = (uint32_t)(t & 1);
+    const uint32_t L = J.lens[r];
+    const uint32_t ntask = L - J.k + 1;
+    const uint32_t S = J.seg_tasks;
+    const uint32_t nseg = (ntask + S - 1) / S;
+    uint4* out = J.cand + 2 * J.task_off[r];
+    const uint64_t i0 = J.item0[t >> 1] + strand;
+    uint32_t K = 0, LB = 0xFFFFFFFFu;
+    for (uint32_t g0 = 0; g0 < nseg; g0 += 64) {
+      uint4 mine = make_uint4(0, 0, 0, 0);
+      uint32_t link = kNone;
+      if (g0 + lane < nseg) {
+        const uint64_t it = i0 + 2 * (uint64_t)(g0 + lane);
+        mine = J.seg_meta[it];
+        if (J.dyn_cap) link = J.seg_link[it];
+      }
+      const uint32_t ng = min(64u, nseg - g0);
+      for (uint32_t j = 0; j < ng; ++j) {
+        const uint4 m = make_uint4(__shfl(mine.x, j), __shfl(mine.y, j), __shfl(mine.z, j), __shfl(mine.w, j));
+        const uint32_t lk = __shfl(link, j);
+        join_piece(out, ntask, strand, (g0 + j) * S, m, J.smem, K, LB, lane, 64);
+        if (J.dyn_cap) join_chain(J, out, ntask, strand, lk, K, LB, lane, 64);
+      }
+    }
+    if (lane == 0) (strand ? J.nR : J.nF)[r] = K;
   }
-  if (lane == 0) (strand ? J.nR : J.nF)[r] = K;
 }
 
 // Whole-read streams that were split (one thread per stream of the batch;
@@ -1123,20 +1142,26 @@ __global__ void __launch_bounds__(256) pack_codes(const uint8_t* __restrict__ ba
   codes[code_off[r] + strand * nch + c] = slot_chunk(pack_chunk(bases + offs[r], L, strand, c));
 }
 
-// longer reads: one workgroup per read
+// longer reads: a fixed grid of waves taking the reads in turn, a wave's
+// lanes one chunk each (a workgroup per read left the launch bound by
+// workgroup dispatch: 1.4 M workgroups of about a hundred chunks each)
 __global__ void __launch_bounds__(256) pack_codes_long(const uint8_t* __restrict__ bases,
                                                        const uint64_t* __restrict__ offs,
                                                        const uint32_t* __restrict__ lens,
                                                        const uint64_t* __restrict__ code_off,
-                                                       const uint32_t* __restrict__ long_reads,
+                                                       const uint32_t* __restrict__ long_reads, uint32_t n_long,
                                                        uint4* __restrict__ codes) {
-  const uint32_t r = long_reads[blockIdx.x];
-  const uint32_t L = lens[r];
-  const uint32_t nch = (L + 31) >> 5;
-  const uint8_t* s = bases + offs[r];
-  for (uint32_t q = threadIdx.x; q < 2 * nch; q += 256) {
-    const uint32_t strand = q >= nch ? 1u : 0u;
-    codes[code_off[r] + q] = pack_chunk(s, L, strand, q - strand * nch);
+  const uint32_t lane = threadIdx.x & 63, n_waves = gridDim.x * 4;
+  for (uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < n_long; w += n_waves) {
+    const uint32_t r = long_reads[w];
+    const uint32_t L = lens[r];
+    const uint32_t nch = (L + 31) >> 5;
+    const uint8_t* s = bases + offs[r];
+    uint4* dst = codes + code_off[r];
+    for (uint32_t q = lane; q < 2 * nch; q += 64) {
+      const uint32_t strand = q >= nch ? 1u : 0u;
+      dst[q] = pack_chunk(s, L, strand, q - strand * nch);
+    }
   }
 }
 
@@ -1217,15 +1242,16 @@ int launch_pack_codes(const uint8_t* bases, const uint64_t* offs, const uint32_t
     hipLaunchKernelGGL(pack_codes, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, bases, offs, lens,
                        code_off, n_reads, codes);
   if (n_long)
-    hipLaunchKernelGGL(pack_codes_long, dim3(n_long), dim3(256), 0, st, bases, offs, lens, code_off, long_reads,
-                       codes);
+    hipLaunchKernelGGL(pack_codes_long, dim3(std::min<uint32_t>((n_long + 3) / 4, kHelperGrid)), dim3(256), 0, st,
+                       bases, offs, lens, code_off, long_reads, n_long, codes);
   return hipGetLastError() == hipSuccess ? GS_OK : GS_EHIP;
 }
 
 int launch_join(const JoinParams& J, hipStream_t st) {
   if (J.n_long) {
-    const uint64_t waves = 2ull * J.n_long;  // four streams per workgroup of 256
-    hipLaunchKernelGGL(join_segments, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, J);
+    const uint64_t waves = 2ull * J.n_long;  // one stream per wave at a time, four waves per workgroup
+    hipLaunchKernelGGL(join_segments, dim3((unsigned)std::min<uint64_t>((waves + 3) / 4, kHelperGrid)), dim3(256), 0,
+                       st, J);
   }
   if (J.dyn_cap && J.n_reads) {
     const uint64_t threads = 2ull * J.n_reads;
