@@ -126,7 +126,59 @@ __device__ __forceinline__ uint32_t div_sa(const DevIndex& X, uint32_t row) {
 }
 
 // read byte -> code, seed_gen.cu:2540 (forward) and 2562 (reverse complement)
-__device__ __forceinline__ uint32_t code_fwd(uint32_t byte) { return (0x24031005u >> ((byte & 7u) << 2)) & 15u; }
-__device__ __forceinline__ uint32_t code_rev(uint32_t byte) { return (0x14002035u >> ((byte & 7u) << 2)) & 15u; }
+constexpr uint32_t kLutFwd = 0x24031005u, kLutRev = 0x14002035u;
+__device__ __forceinline__ uint32_t code_fwd(uint32_t byte) { return (kLutFwd >> ((byte & 7u) << 2)) & 15u; }
+__device__ __forceinline__ uint32_t code_rev(uint32_t byte) { return (kLutRev >> ((byte & 7u) << 2)) & 15u; }
+
+// Chunk c (codes 32c .. 32c+31) of a read's virtual strand in 4-bit codes,
+// two per byte, even position in the low nibble; codes at or past the read's
+// length L are 0.  Forward code i is the code of byte i, reverse code i the
+// reverse-complement code of byte L-1-i (pack_4bit_fow / pack_4bit_rev,
+// seed_gen.cu:2536-2578).  The chunk's codes come from at most 32
+// consecutive bytes of the read, loaded as the 4-byte-aligned words that hold
+// them, all at once: a word is read only if it holds a byte of the read, so
+// no load leaves [s rounded down to 4 bytes, s + L + 3).
+__host__ __device__ __forceinline__ uint4 pack_chunk(const uint8_t* s, uint32_t L, uint32_t strand, uint32_t c) {
+  // the window of 32 bytes, in read positions: forward [32c, 32c + 32),
+  // reverse [L - 32c - 32, L - 32c) (its start may be negative)
+  const int32_t lo = strand ? (int32_t)L - 32 * (int32_t)c - 32 : 32 * (int32_t)c;
+  const uint32_t d = (uint32_t)((uintptr_t)s & 3u);  // s's offset in its aligned word
+  const uint32_t* const base = reinterpret_cast<const uint32_t*>(s - d);
+  const int32_t m0 = (lo + (int32_t)d) >> 2;  // the window's first word (floor)
+  const uint32_t off = (uint32_t)(lo + (int32_t)d - 4 * m0);
+  uint32_t w[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int32_t m = m0 + k, first = 4 * m - (int32_t)d;  // read position of the word's first byte
+    w[k] = (first + 4 > 0 && first < (int32_t)L) ? base[m] : 0u;
+  }
+  // the window's bytes in order (reverse: last byte first)
+  uint32_t x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = (uint32_t)((((uint64_t)w[k + 1] << 32) | w[k]) >> (8 * off));
+  if (strand) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t t = x[k];
+      x[k] = __builtin_bswap32(x[7 - k]);
+      x[7 - k] = __builtin_bswap32(t);
+    }
+  }
+  const uint32_t lut = strand ? kLutRev : kLutFwd;
+  const uint32_t nvalid = L > 32 * c ? L - 32 * c : 0u;  // codes of the chunk inside the read
+  uint32_t out[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t byte = x[2 * q + (j >> 2)] >> (8 * (j & 3));
+      v |= ((lut >> ((byte & 7u) << 2)) & 15u) << (4 * j);
+    }
+    const uint32_t keep = nvalid >= 8u * (q + 1) ? 32u : (nvalid > 8u * q ? 4 * (nvalid - 8u * q) : 0u);
+    out[q] = keep == 32u ? v : (v & ((1u << keep) - 1u));
+  }
+  return make_uint4(out[0], out[1], out[2], out[3]);
+}
 
 }  // namespace gs

@@ -1107,20 +1107,7 @@ __global__ void __launch_bounds__(256) join_whole(gs::JoinParams J) {
   n_out[r] = K;
 }
 
-// 4-bit codes of both virtual strands: forward v[i] = code of byte i, reverse
-// v[i] = reverse-complement code of byte L-1-i (pack_4bit_fow / pack_4bit_rev,
-// seed_gen.cu:2536-2578); two codes per byte, even position in the low
-// nibble, 32 codes per 16-byte chunk.
-__device__ __forceinline__ uint4 pack_chunk(const uint8_t* s, uint32_t L, uint32_t strand, uint32_t c) {
-  uint32_t w[4] = {0, 0, 0, 0};
-  for (uint32_t j = 0; j < 32; ++j) {
-    const uint32_t i = c * 32 + j;
-    if (i >= L) break;
-    const uint32_t v = strand ? gs::code_rev(s[L - 1 - i]) : gs::code_fwd(s[i]);
-    w[j >> 3] |= v << ((j & 7) << 2);
-  }
-  return make_uint4(w[0], w[1], w[2], w[3]);
-}
+using gs::pack_chunk;
 
 // reads of length <= kPersistLmax: one thread per (read, strand, chunk),
 // each chunk stored in the form the persistent kernel's LDS slot holds
