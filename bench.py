@@ -552,6 +552,8 @@ def main():
     # right around each launch (seed_s also holds code packing, the work
     # queue's set-up and the join of split streams)
     kern_s = sum(s.t_kernel for s in stats) / len(stats)
+    if kern_s <= 0:  # no kernel events (an empty batch): the stage's time
+        kern_s = seed_s
     first_arrays, first_batch = batches[args.warmup]
     gpu_out = first_batch.fetch()
 
