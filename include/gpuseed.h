@@ -212,9 +212,11 @@ GS_API int gs_index_set_direct(gs_index* idx, int on);
  * which precede it (4^18 bytes = 68.7 GB of HBM, at any text length).  With
  * the 16-mer extended table and k >= 19, once a task of a read strand has
  * found its clean 19-base window absent, the next task reads the byte of its
- * window's core instead: one lookup answers whether its own window and the
- * next task's occur, so on a strand that does not match the reference two
- * tasks end per lookup instead of one.  The output is unchanged.  Built from
+ * window's core instead, and with 22 clean codes also the byte of the core
+ * two bases to its left, in the same load: one operation answers whether the
+ * windows of up to four consecutive task ends occur, so on a strand that does
+ * not match the reference up to four tasks end per lookup instead of one.
+ * The output is unchanged.  Built from
  * the text of the direct-extension structures, or from a text derived from
  * the index by the same walk.  No counterpart in the reference. */
 GS_API int gs_index_set_pairs(gs_index* idx, int on);
