@@ -44,3 +44,11 @@ def test_counter_file_agrees_with_the_markdown_summary():
     assert counter("WRITE_SIZE") == kv["write_kib"]
     assert counter("TCC_HIT_sum") == kv["tcc_hit"]
     assert counter("TCC_MISS_sum") == kv["tcc_miss"]
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    import subprocess
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, timeout=120)
+    assert p.returncode != 0 and p.stdout == b""
+    assert b"--gpus 1" in p.stderr
