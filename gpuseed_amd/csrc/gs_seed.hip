@@ -860,6 +860,8 @@ static void destroy_batch(gs_batch* B) {
   if (B->h_tot) (void)hipHostFree(B->h_tot);
   for (auto& e : B->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : B->kev)
+    if (e) (void)hipEventDestroy(e);
   if (B->stream) (void)hipStreamDestroy(B->stream);
   gs_index* X = B->X;
   const bool holds = B->holds;
