@@ -10,10 +10,12 @@
  * Pinning: the tests check every function against brute force on small texts
  * (sorted suffixes, substring search), hand-worked known answers and the
  * reference's literal constants.  Two behaviours follow from reading the
- * reference alone, and are parity unpinned: N-transparency after the 13-base
- * window (the never-assigned outer `base`, seed_gen.cu:2125, 2246, 2387) and
- * the stability of the device segmented sorts (seed_gen.cu:1233, 3570),
- * assumed stable.  The reference cannot run here, and its repository holds
+ * reference alone, and are parity unpinned: that a code above 3 after the
+ * 13-base window never ends a task (the test at seed_gen.cu:2246/2387 reads
+ * the never-assigned outer `base` of 2125; that such a code leaves the
+ * interval unchanged is pinned, from find_occ_gpu's 32-bit arithmetic at 333
+ * and 2173-2174, in test_oracle_rank.py) and the stability of the device
+ * segmented sorts (seed_gen.cu:1233, 3570), assumed stable.  The reference cannot run here, and its repository holds
  * no tests or golden vectors.
  */
 #ifndef GS_ORACLE_H

@@ -56,9 +56,24 @@ def test_backward_step_gives_sa_intervals(pair):
 
 
 def test_step_leaves_interval_unchanged_for_codes_above_3(pair):
+    """seed_gen.cu:333 returns (l-1-L2[c], u-L2[c]) for c > 3 and 2173-2174
+    add L2[c] back (l: +1); in 32-bit arithmetic that is (l, u) whatever
+    L2[c] holds (c = 5..15 read past L2_gpu's five entries), l = 0 included.
+    The oracle's step must give the same on arbitrary intervals."""
     bi, ox = pair
+    M = 0xFFFFFFFF
+    rng = random.Random(17)
     for c in (4, 5, 9, 15):
         assert ox.step(1, bi.n, c) == (1, bi.n)
+        for _ in range(200):
+            l, u = rng.randrange(0, bi.n + 1), rng.randrange(0, bi.n + 1)
+            if rng.random() < 0.1:
+                l = 0
+            L2c = rng.choice([0, bi.n, M, rng.randrange(0, M + 1)])
+            x, y = (l - 1 - L2c) & M, (u - L2c) & M  # find_occ_gpu, c > 3
+            want = ((L2c + x + 1) & M, (L2c + y) & M)
+            assert want == (l, u)
+            assert ox.step(l, u, c) == want, (l, u, c)
 
 
 def test_lf_matches_true_lf_except_row_after_primary(pair):
